@@ -1,0 +1,78 @@
+// ref_shim_mbfpa.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// extern "C" entry points onto the reference's own MBFPA functions.  This
+// file is compiled together with the reference sources where they lie
+// (Release_MBFPA_CPU_human/{reprojection_TOMO,prior_weight_production,
+// misc_TOMO,mallocMD}.cpp) into oracle/_ref/libref_mbfpa.so by
+// oracle/Makefile.  It only allocates the buffers the reference functions
+// expect and forwards the call; no reference code is restated here.
+#include <stdlib.h>
+#include <string.h>
+
+#include "physParamsTOMO.h"
+#include "mallocMD.h"
+#include "reprojection_TOMO.h"
+#include "virtual_projection.h"
+
+extern "C" {
+
+// M/prior_weight_production.cpp:180 (the element table there is fixed to 6)
+int ref_attenuation(const char* path, int num_material, float* att, float* pdf)
+{
+    int NE = 0;
+    attenuation_coefficient((char*)path, num_material, att, pdf, &NE, 1);
+    return NE;
+}
+
+// One energy pass of M/reprojection_TOMO.cpp:13.  dens is float [M][N][N]
+// (modified in place exactly as the reference does).  seg_counts, if not
+// NULL, receives reprojection_sum [H][W].  Returns 0, or -1 on allocation
+// failure.
+int ref_reprojection(int N, float* dens, float* proj, double pix, int W, int H,
+                     double* cosT, double* sinT, double xfactor, float* att,
+                     float* pdf, int M, int NE, int ke, int* seg_counts)
+{
+    long depth = (N >= 256) ? 2L * N : 4L * N + 64;
+    long cache = depth * (long)W * H;
+    float* cacheM = (float*)calloc(cache, sizeof(float));
+    int* cacheV = (int*)calloc(cache, sizeof(int));
+    int* cacheS = (int*)calloc((long)W * H, sizeof(int));
+    if (!cacheM || !cacheV || !cacheS) { free(cacheM); free(cacheV); free(cacheS); return -1; }
+    double xalpha[1] = {0.0};
+    char mask[1] = {1};
+    reprojection_TOMO(1, N, dens, proj, pix, W, H, cosT, sinT, 0.0, 0.0, xfactor, xalpha,
+                      mask, cacheM, cacheV, cacheS, 1, att, pdf, M, NE, ke);
+    if (seg_counts) memcpy(seg_counts, cacheS, sizeof(int) * (size_t)W * H);
+    free(cacheM); free(cacheV); free(cacheS);
+    return 0;
+}
+
+// M/main.cpp:492-506: the whole energy loop, summed in float per bin as the
+// reference driver does.  out is float [H][W].
+int ref_reprojection_full(int N, float* dens, float* out, double pix, int W, int H,
+                          double* cosT, double* sinT, double xfactor, float* att,
+                          float* pdf, int M, int NE, int ke_begin, int ke_end)
+{
+    long depth = (N >= 256) ? 2L * N : 4L * N + 64;
+    long cache = depth * (long)W * H;
+    float* cacheM = (float*)calloc(cache, sizeof(float));
+    int* cacheV = (int*)calloc(cache, sizeof(int));
+    int* cacheS = (int*)calloc((long)W * H, sizeof(int));
+    float* proj = (float*)calloc((long)W * H, sizeof(float));
+    if (!cacheM || !cacheV || !cacheS || !proj) {
+        free(cacheM); free(cacheV); free(cacheS); free(proj);
+        return -1;
+    }
+    double xalpha[1] = {0.0};
+    char mask[1] = {1};
+    memset(out, 0, sizeof(float) * (size_t)W * H);
+    for (int ke = ke_begin; ke < ke_end; ke++) {
+        reprojection_TOMO(1, N, dens, proj, pix, W, H, cosT, sinT, 0.0, 0.0, xfactor, xalpha,
+                          mask, cacheM, cacheV, cacheS, 1, att, pdf, M, NE, ke);
+        for (long i = 0; i < (long)W * H; i++) out[i] += proj[i];
+    }
+    free(cacheM); free(cacheV); free(cacheS); free(proj);
+    return 0;
+}
+
+}  // extern "C"
