@@ -1,0 +1,200 @@
+"""Benchmark: plain MBFPA + FBP on one MI355X (FP32 path, FP64 geometry).
+
+Workload (one step): a seeded synthetic 512x512 6-element body phantom
+(uint16 partial densities resident in HBM), the synthetic 6 MV spectrum of
+120 bins x 50 keV, fan beam 800 views x 609 channels; the step runs the
+tracer, the polyenergetic sum, max-normalisation with Gaussian detector
+noise (sigma 5.37e-5, Philox), the MFMA Shepp-Logan filter and the
+backprojector.  value = rays x energy bins per second over the whole step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  Per-stage times come from HIP events recorded on the
+launch stream inside the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEGMENTS_512 = 282_824_404  # sum of per-ray segments at 512^2, 0.1 cm, 800x609
+METRIC = "MBFPA ray·energy-bins/s, FBP voxel·views/s; batched grad steps/s vs plain"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(dens_f32, att, pdf, geom_c, seconds_budget=20.0):
+    """The reference compiled from source (oracle/_ref) on a bounded sample:
+    a few energy bins of its per-bin re-trace (OpenMP, all host threads) and
+    its single-threaded FBP on one sinogram."""
+    import numpy as np
+
+    from oracle.oracle import Geom, Oracle, RefFBP, RefMBFPA, ref_available
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    g = Geom()
+    o = Oracle()
+    cosT, sinT = o.view_tables(g)
+    if ref_available():
+        ref = RefMBFPA()
+        n_bins = 3
+        t0 = time.perf_counter()
+        ref.forward(g, dens_f32, att, pdf, cosT, sinT, ke_range=(0, n_bins))
+        t_m = time.perf_counter() - t0
+        mbfpa = g.V * g.W * n_bins / t_m
+        rf = RefFBP()
+        p = np.full((g.V, g.W), 0.5, np.float32)
+        t0 = time.perf_counter()
+        fl = rf.filter(p, 2)
+        t_f = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rf.backproject(g, fl, cosT, sinT)
+        t_b = time.perf_counter() - t0
+        p_mask = int(o.mask(512).sum())
+        return {
+            "value": mbfpa, "unit": "ray·energy-bins/s", "cores": threads, "kind": "reference",
+            "sample": (f"reference reprojection_TOMO (compiled from source, -O2 -fopenmp, {threads} threads) "
+                       f"for {n_bins} of the 120 energy bins on the full 512^2 phantom / 800x609 geometry "
+                       f"({t_m:.2f} s, re-traced per bin as the reference does); reference FBP single-thread: "
+                       f"filter {t_f:.2f} s + backprojection {t_b:.2f} s"),
+            "fbp_voxel_views_per_s": p_mask * g.V / (t_f + t_b),
+            "fbp_cores": 1,
+        }
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=1000)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from virtualct_amd import default_geometry, ops, phantoms, spectra
+
+    dev = torch.device("cuda", 0)
+    geom = default_geometry()
+    plan = ops.plan_for(geom)
+    ph = phantoms.body_phantom(args.seed)
+    with tempfile.TemporaryDirectory() as d:
+        att, pdf, _ = spectra.attenuation(spectra.synthetic_spectrum_file("6MV", d), 6)
+    M, NE = att.shape
+    V, W, N = geom.n_views, geom.n_channels, geom.recon_size
+    d_ph = torch.from_numpy(ph).to(dev)
+    d_att = torch.from_numpy(att).to(dev)
+    d_pdf = torch.from_numpy(pdf).to(dev)
+    sigma = 5.37e-5
+    pk, dt = ops.pack_densities(plan, d_ph)
+    torch.cuda.synchronize()
+
+    stages = ["trace", "spectral", "detector", "filter", "backproject"]
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        L = ops.trace_paths(plan, pk, dt, M)
+        if ev is not None:
+            ev[1].record()
+        raw, stats, _ = ops.spectral_sum(plan, L, M, d_att, d_pdf)
+        if ev is not None:
+            ev[2].record()
+        sino = ops.detector(plan, raw, stats, sigma, 0x5EED, 0)
+        if ev is not None:
+            ev[3].record()
+        fl = ops.filter_sinogram(plan, sino, 2)
+        if ev is not None:
+            ev[4].record()
+        img = ops.backproject(plan, fl)
+        if ev is not None:
+            ev[5].record()
+        return img
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        img = step(events[k])
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    st_ms = {s: 0.0 for s in stages}
+    for ev in events:
+        for i, s in enumerate(stages):
+            st_ms[s] += ev[i].elapsed_time(ev[i + 1])
+    st_ms = {s: v / args.steps for s, v in st_ms.items()}
+    ms_step = total * 1000.0 / args.steps
+    rays_bins = V * W * NE
+    value = rays_bins * args.steps / total
+    mbfpa_ms = st_ms["trace"] + st_ms["spectral"] + st_ms["detector"]
+    fbp_ms = st_ms["filter"] + st_ms["backproject"]
+    p_mask = int(((np.arange(N) - (N - 1) / 2.0)[:, None] ** 2 + (np.arange(N) - (N - 1) / 2.0)[None, :] ** 2
+                  <= ((N - 1) / 2.0) ** 2).sum())
+    trace_bytes = SEGMENTS_512 * M * 2 + V * W * 4
+    achieved = trace_bytes / (st_ms["trace"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "ray·energy-bins/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 geometry and path sums)",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain MBFPA+FBP, 512x512 6-element synthetic body phantom (seed %d), "
+                        "synthetic 6 MV spectrum 120 bins x 50 keV, 800 views x 609 channels, "
+                        "Gaussian detector noise sigma 5.37e-5, Shepp-Logan FBP to 512x512" % args.seed,
+            "views": V, "channels": W, "energy_bins": NE, "materials": M, "image": N,
+            "parallelism": "single GPU",
+        },
+        "stage_ms": st_ms,
+        "mbfpa_ray_energy_bins_per_s": rays_bins / (mbfpa_ms * 1e-3),
+        "fbp_voxel_views_per_s": p_mask * V / (fbp_ms * 1e-3),
+        "roofline": {
+            "kernel": "k_trace_fwd (tracer)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": trace_bytes,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            dens = phantoms.densities_f32(ph)
+            out["cpu_baseline"] = cpu_baseline(dens, att, pdf, None)
+        except Exception as e:  # the GPU figure stands without it
+            log("cpu baseline skipped:", repr(e))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+/*
+ * vct.h -- C ABI of the MI355X VirtualCT hot path (libvct.so).
+ *
+ * The library implements the two hot paths of VirtualCT on CDNA4 (gfx950):
+ * the material-based forward projection (MBFPA: per-ray elemental path
+ * integrals, polyenergetic sum, max-normalisation, Gaussian detector noise)
+ * and fan-beam FBP (Shepp-Logan / Ram-Lak filter as an MFMA GEMM and a
+ * pixel-driven backprojector), plus the adjoint of the forward projection.
+ *
+ * Conventions
+ *   - Every entry point returns a vct_status; the library never exits.
+ *     vct_last_error() gives a message for the calling thread.
+ *   - "d_" pointers are device (HBM) pointers; "h_"/unprefixed buffers of the
+ *     host entry points are host pointers owned by the caller.
+ *   - A stream argument is a hipStream_t passed as void* (NULL = HIP's
+ *     null stream, which is ordered with all blocking streams such as
+ *     PyTorch's default stream).  Calls on one stream are ordered; calls
+ *     that share the context scratch buffer must not overlap across streams.
+ *   - Inputs are never modified (the reference zeroes negative material-0
+ *     voxels in place, M/reprojection_TOMO.cpp:151; here a negative density
+ *     contributes nothing, which gives the same sums without the write).
+ *   - Sinograms are float32 [views][channels] (views = sinogram height,
+ *     channels = width), images float32 [N][N], row index = -y, exactly the
+ *     raw layouts of M/main.cpp:527-530 and F/main.cpp:187-190.
+ *
+ * Reference citations are relative to the VirtualCT repository; "M/" is
+ * Release_MBFPA_CPU_human/ and "F/" is Release_FBP_CPU/.
+ */
+#ifndef VCT_H
+#define VCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VCT_ABI_VERSION 1
+
+typedef int vct_status;
+enum {
+    VCT_OK = 0,
+    VCT_ERR_ARG = 1,          /* invalid argument / shape                     */
+    VCT_ERR_HIP = 2,          /* a HIP runtime call failed                    */
+    VCT_ERR_NOMEM = 3,        /* device or host allocation failed             */
+    VCT_ERR_IO = 4,           /* file missing or short                        */
+    VCT_ERR_UNSUPPORTED = 5,  /* geometry/option outside what is implemented  */
+    VCT_ERR_INTERNAL = 6      /* a kernel reported an internal limit (e.g. a  */
+                              /* ray exceeded its step bound)                 */
+};
+
+/* Density storage of a phantom on the device (one voxel = 8 slots). */
+enum {
+    VCT_DENS_U16 = 0,  /* uint16 [N][N][8], partial density in mg/cm^3 (the
+                          reference's int16 file value, /1000 -> g/cm^3)     */
+    VCT_DENS_F32 = 1   /* float  [N][N][8], partial density in g/cm^3         */
+};
+
+enum { VCT_FILTER_RAMLAK = 1, VCT_FILTER_SHEPP_LOGAN = 2 };
+
+typedef struct vct_ctx vct_ctx;
+typedef struct vct_plan vct_plan;
+
+/* Fan-beam geometry.  Defaults (vct_geometry_default) are the reference's
+ * compile-time constants M/physParamsTOMO.h:24-62 = F/physParamsTOMO.h:23-51
+ * and the input-txt values of M/IR_TOMO_input_virtual_projection_120kV.txt. */
+typedef struct vct_geometry {
+    int32_t recon_size;      /* N, image is N x N                        (512) */
+    int32_t n_channels;      /* W, detector channels                     (609) */
+    int32_t n_views;         /* V, views over 360 degrees                (800) */
+    int32_t reserved0;
+    double  pixel_cm;        /* pixel pitch                              (0.1) */
+    double  start_view_deg;  /* starting view angle                        (0) */
+    double  sid_cm;          /* source to isocentre                     (85.0) */
+    double  det_radius_cm;   /* detector radius of curvature           (142.6) */
+    double  det_src_iso_cm;  /* detector-arc centre to isocentre        (85.0) */
+    double  iso_channel;     /* isocentre channel                      (304.5) */
+    double  max_alpha;       /* projector fan half-angle         (0.326086956) */
+    double  max_beta;        /* FBP fan half-angle               (0.326086956) */
+    double  xfactor;         /* "Factor for output CT value"             (1)   */
+} vct_geometry;
+
+/* ------------------------------------------------------------ lifecycle */
+
+/* Fill g with the reference defaults. */
+void vct_geometry_default(vct_geometry* g);
+const char* vct_status_string(vct_status s);
+/* Message of the last error on this thread ("" if none). */
+const char* vct_last_error(void);
+int vct_abi_version(void);
+
+/* Replaces initializeGPU (M/main.cpp:289-291; live in the vendor build):
+ * binds a device and creates its streams and scratch. */
+vct_status vct_init(int device, vct_ctx** ctx);
+/* Replaces terminateGPU (M/main.cpp:670). */
+vct_status vct_terminate(vct_ctx* ctx);
+
+/* Device tables for one geometry (view cos/sin from glibc on the host, as
+ * prepareTrigonometricTable M/misc_TOMO.cpp:81-100; per-channel detector
+ * points M/reprojection_TOMO.cpp:60-71; filter matrices F/filterSinogramTOMO.cpp:9-138). */
+vct_status vct_plan_create(vct_ctx* ctx, const vct_geometry* g, vct_plan** plan);
+vct_status vct_plan_destroy(vct_plan* plan);
+
+/* ------------------------------------------------------- host-side tables */
+
+/* Replaces attenuation_coefficient (M/prior_weight_production.cpp:180-297):
+ * reads a reference-format spectrum file ("Elo Ehi weight" per line in MeV
+ * after one header line) and writes the normalised pdf[NE] and the mass
+ * attenuation attenuation[M][NE] (cm^2/g).  Elements are, in order,
+ * H C N O P Ca (the reference's table, :249-255) followed by Mg Si, so
+ * num_material <= 8.  Capacity: NE <= 1024.  Z/A may be NULL for that
+ * table or give num_material custom elements. */
+vct_status vct_attenuation_coefficient(const char* spectrum_path, int num_material,
+                                       const double* Z, const double* A, int nthin,
+                                       float* attenuation, float* pdf, int* NE,
+                                       double* energy_mev);
+
+/* Replaces prepareConstMap (F/filterSinogramTOMO.cpp:9-138): C[W][W] in FP64. */
+vct_status vct_const_map(const vct_geometry* g, int filter, double* C);
+
+/* ---------------------------------------------- device-pointer hot path */
+
+/* Pack a reference-layout int16 [M][N][N] phantom (values = 1000 x g/cm^3,
+ * the Weight_input_*.raw format read at M/main.cpp:421-429) into the
+ * voxel-interleaved uint16 [N][N][8] layout; negatives become 0. */
+vct_status vct_pack_densities_i16(vct_plan* plan, const int16_t* d_dens, int M,
+                                  uint16_t* d_packed, void* stream);
+/* Same for float [M][N][N] g/cm^3 -> float [N][N][8]. */
+vct_status vct_pack_densities_f32(vct_plan* plan, const float* d_dens, int M,
+                                  float* d_packed, void* stream);
+
+/* Ray tracer, replacing the geometry of reprojection_TOMO
+ * (M/reprojection_TOMO.cpp:13-289): for every ray (view, channel) writes
+ * the per-element path integrals d_L[V][W][8] (float; slot m = sum over the
+ * ray's segments of rho_m * length, g/cm^2, divided by xfactor).  Each ray
+ * is traced once, independent of the spectrum.  d_counts (optional, int32
+ * [V][W]) receives the reference's per-ray segment counter
+ * (reprojection_sum, :195/:280). */
+vct_status vct_trace_paths(vct_plan* plan, const void* d_packed, int dens_type, int M,
+                           float* d_L, int32_t* d_counts, void* stream);
+
+/* Polyenergetic detector sum, replacing the energy loop M/main.cpp:492-506
+ * and the per-bin output M/reprojection_TOMO.cpp:196-197:
+ *   raw[s][ray] = sum_E pdf[s][E] * exp(-sum_m att[s][m][E] * L[ray][m])
+ * for n_spectra spectra of NE bins each (att [S][M][NE], pdf [S][NE]).
+ * d_stats[s] = {max, min} of raw[s] as float bit patterns (inputs to the
+ * normalisation and the noise clamp).  d_dIdL (optional, float
+ * [S][V][W][8]) receives d raw / d L_m for the adjoint. */
+vct_status vct_spectral_sum(vct_plan* plan, const float* d_L, int M,
+                            const float* d_att, const float* d_pdf, int NE, int n_spectra,
+                            float* d_raw, uint32_t* d_stats, float* d_dIdL, void* stream);
+
+/* Max-normalisation and detector noise, replacing M/main.cpp:511-525 and
+ * gaussiannoise0 (:51-76): out = raw / max + sigma * z, clamped as the
+ * reference (x < 0 -> (min/max)/10, x > 1 -> 1).  z is standard normal from
+ * Philox4x32-10 keyed by seed, with counter (element, stream_id + s).
+ * sigma = 0 gives the plain normalisation. */
+vct_status vct_detector(vct_plan* plan, const float* d_raw, const uint32_t* d_stats,
+                        int n_spectra, double sigma, uint64_t seed, uint32_t stream_id,
+                        float* d_out, void* stream);
+
+/* Whole MBFPA forward for one phantom and n_spectra spectra:
+ * trace + spectral sum + detector.  d_work must hold
+ * vct_forward_workspace_bytes(plan, n_spectra) bytes (or be NULL to use the
+ * context scratch). */
+size_t vct_forward_workspace_bytes(const vct_plan* plan, int n_spectra);
+vct_status vct_mbfpa_forward(vct_plan* plan, const void* d_packed, int dens_type, int M,
+                             const float* d_att, const float* d_pdf, int NE, int n_spectra,
+                             double sigma, uint64_t seed, uint32_t stream_id,
+                             float* d_sino, void* d_work, void* stream);
+
+/* FBP filter, replacing filterSinogram (F/filterSinogramTOMO.cpp:205-275):
+ * fltd[v][m] = sum_k C[m][k] * (-ln proj[v][k] * sid * cos((k-W/2) dbeta)) * dbeta
+ * as an FP32 MFMA GEMM over n_sino stacked sinograms.  in_mode != 0 applies
+ * the "IN" threshold of :250-255. */
+vct_status vct_filter_sinogram(vct_plan* plan, const float* d_proj, int n_sino, int filter,
+                               int in_mode, float* d_fltd, void* stream);
+/* Pixel-driven fan backprojection, replacing reconstructImage
+ * (F/reconstructImageTOMO.cpp:11-105), n_sino images. */
+vct_status vct_backproject(vct_plan* plan, const float* d_fltd, int n_sino, float* d_img,
+                           void* stream);
+
+/* Adjoint of vct_trace_paths weighted per ray:
+ *   grad[v][m] += sum_rays w[ray][m] * sum_{segments of ray in voxel v} length
+ * where w[ray][m] = d_dIdL[ray][m] * d_g[ray] (d_g may be NULL = 1).  grad is
+ * float [N][N][8]; it is accumulated into (clear it first).  Segments whose
+ * voxel has a negative density of element m contribute nothing to slot m. */
+vct_status vct_trace_adjoint(vct_plan* plan, const void* d_packed, int dens_type, int M,
+                             const float* d_dIdL, const float* d_g, float* d_grad,
+                             void* stream);
+
+/* ------------------------------------------------ host-buffer drop-ins */
+
+/* Replaces reprojectionOnGPU_TOMO (M/main.cpp:439-446) = the energy loop
+ * over reprojection_TOMO (M/main.cpp:492-506): dens float [M][N][N] g/cm^3,
+ * att [M][NE], pdf [NE]; sino_full float [V][W] = the un-normalised sum. */
+vct_status vct_reprojection(vct_ctx* ctx, const vct_geometry* g, const float* dens, int M,
+                            const float* att, const float* pdf, int NE, float* sino_full);
+/* Replaces the normalisation and gaussiannoise0 (M/main.cpp:511-525),
+ * in place on a host sinogram of n floats. */
+vct_status vct_gaussian_noise(vct_ctx* ctx, float* sino, int n, double sigma, uint64_t seed);
+/* Replaces filterSinogram (F/filterSinogramTOMO.cpp:205); host [V][W]. */
+vct_status vct_filter_sinogram_host(vct_ctx* ctx, const vct_geometry* g, const float* proj,
+                                    int filter, int in_mode, float* fltd);
+/* Replaces reconstructImage (F/reconstructImageTOMO.cpp:11); host buffers. */
+vct_status vct_reconstruct_image_host(vct_ctx* ctx, const vct_geometry* g, const float* fltd,
+                                      float* img);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VCT_H */

@@ -75,6 +75,9 @@ class Oracle:
         L.orc_trace_ray.argtypes = [G, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int,
                                     _i32p, _f64p, C.c_int, C.POINTER(C.c_int)]
         L.orc_trace_ray.restype = C.c_int
+        L.orc_trace_ray_iks.argtypes = [G, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int,
+                                        _i32p, _f64p, _i32p, C.c_int, C.POINTER(C.c_int)]
+        L.orc_trace_ray_iks.restype = C.c_int
         L.orc_segment_counts.argtypes = [G, _i32p]
         L.orc_segment_counts.restype = C.c_int64
         L.orc_attenuation.argtypes = [C.c_char_p, C.c_int, _f64p, _f64p, C.c_int,
@@ -109,12 +112,12 @@ class Oracle:
         c, s = tables[0] if tables else self.view_tables(g)
         px0, py0 = tables[1] if tables else self.channel_table(g)
         ms = 4 * g.N + 64
-        vox = np.zeros(ms, np.int32); rr = np.zeros(ms, np.float64)
+        vox = np.zeros(ms, np.int32); rr = np.zeros(ms, np.float64); idx = np.zeros(ms, np.int32)
         iks = C.c_int(0)
-        n = self.lib.orc_trace_ray(C.byref(g.c()), c, s, px0, py0, view, ch, vox, rr, ms, C.byref(iks))
+        n = self.lib.orc_trace_ray_iks(C.byref(g.c()), c, s, px0, py0, view, ch, vox, rr, idx, ms, C.byref(iks))
         if n < 0:
             raise RuntimeError("segment buffer overflow")
-        return vox[:n].copy(), rr[:n].copy(), iks.value
+        return vox[:n].copy(), rr[:n].copy(), iks.value, idx[:n].copy()
 
     def segment_counts(self, g: Geom):
         out = np.zeros((g.V, g.W), np.int32)

@@ -76,3 +76,43 @@ int ref_reprojection_full(int N, float* dens, float* out, double pix, int W, int
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Segment lists the reference caches for a set of rays during one energy
+// pass (M/reprojection_TOMO.cpp:160-163, :186-188; index = segment counter).
+// For ray r = rays[i] (= view*W + channel), writes count[i] = the counter
+// value (reprojection_sum) and, for c in 1..count[i], vox/len[i*stride + c-1]
+// = the cached (vnum, rr) at counter c (0/0 where nothing was cached).
+int ref_segments(int N, float* dens, double pix, int W, int H, double* cosT, double* sinT,
+                 float* att, float* pdf, int M, int NE, const int* rays, int nrays,
+                 int stride, int* count, int* vox, float* len)
+{
+    long depth = (N >= 256) ? 2L * N : 4L * N + 64;
+    long cache = depth * (long)W * H;
+    float* cacheM = (float*)calloc(cache, sizeof(float));
+    int* cacheV = (int*)calloc(cache, sizeof(int));
+    int* cacheS = (int*)calloc((long)W * H, sizeof(int));
+    float* proj = (float*)calloc((long)W * H, sizeof(float));
+    if (!cacheM || !cacheV || !cacheS || !proj) {
+        free(cacheM); free(cacheV); free(cacheS); free(proj);
+        return -1;
+    }
+    double xalpha[1] = {0.0};
+    char mask[1] = {1};
+    reprojection_TOMO(1, N, dens, proj, pix, W, H, cosT, sinT, 0.0, 0.0, 1.0, xalpha,
+                      mask, cacheM, cacheV, cacheS, 1, att, pdf, M, NE, 0);
+    for (int i = 0; i < nrays; i++) {
+        const long r = rays[i];
+        const int n = cacheS[r];
+        count[i] = n;
+        for (int c = 1; c <= n && c <= stride; c++) {
+            vox[(long)i * stride + c - 1] = cacheV[(long)c * W * H + r];
+            len[(long)i * stride + c - 1] = cacheM[(long)c * W * H + r];
+        }
+    }
+    free(cacheM); free(cacheV); free(cacheS); free(proj);
+    return 0;
+}
+
+}  // extern "C"

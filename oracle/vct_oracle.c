@@ -85,12 +85,13 @@ typedef struct {
     double D, rmax;
     int32_t* vox;
     double* rr;
+    int32_t* iks;  /* optional: reference segment counter of each segment */
     int maxseg, nseg, overflow;
 } seg_sink;
 
 /* One candidate segment (a0,b0)->(a1,b1) in (major, minor) coordinates:
  * M/reprojection_TOMO.cpp:141-166 and :168-189 (x-major), :228-274 (y-major). */
-static void emit_segment(seg_sink* s, double a0, double b0, double a1, double b1)
+static void emit_segment(seg_sink* s, int iks, double a0, double b0, double a1, double b1)
 {
     double xb0, yb0, xb1, yb1;
     if (s->xmajor) { xb0 = a0; yb0 = b0; xb1 = a1; yb1 = b1; }
@@ -104,6 +105,7 @@ static void emit_segment(seg_sink* s, double a0, double b0, double a1, double b1
         if (s->nseg >= s->maxseg) { s->overflow = 1; return; }
         s->vox[s->nseg] = xmc + ymc * s->N;
         s->rr[s->nseg] = rr;
+        if (s->iks) s->iks[s->nseg] = iks;
         s->nseg++;
     }
 }
@@ -111,6 +113,13 @@ static void emit_segment(seg_sink* s, double a0, double b0, double a1, double b1
 int orc_trace_ray(const orc_geom* g, const double* cosT, const double* sinT,
                   const double* px0t, const double* py0t, int view, int ch,
                   int32_t* vox, double* rrv, int maxseg, int* n_iks)
+{
+    return orc_trace_ray_iks(g, cosT, sinT, px0t, py0t, view, ch, vox, rrv, NULL, maxseg, n_iks);
+}
+
+int orc_trace_ray_iks(const orc_geom* g, const double* cosT, const double* sinT,
+                      const double* px0t, const double* py0t, int view, int ch,
+                      int32_t* vox, double* rrv, int32_t* seg_iks, int maxseg, int* n_iks)
 {
     /* M/reprojection_TOMO.cpp:25-37 */
     const int N = g->N;
@@ -153,7 +162,7 @@ int orc_trace_ray(const orc_geom* g, const double* cosT, const double* sinT,
     sink.xmajor = fabs(xi - xf) > fabs(yi - yf); /* :113 */
     sink.N = N; sink.nxm = nxm; sink.nym = nym; sink.D = D;
     sink.rmax = N * 0.5 * 1.0; /* :37 */
-    sink.vox = vox; sink.rr = rrv; sink.maxseg = maxseg; sink.nseg = 0; sink.overflow = 0;
+    sink.vox = vox; sink.rr = rrv; sink.iks = seg_iks; sink.maxseg = maxseg; sink.nseg = 0; sink.overflow = 0;
 
     /* The two branches of the reference (:113-200 x-major, :201-283 y-major)
      * are the same march with the axes exchanged: a = major, b = minor. */
@@ -191,11 +200,11 @@ int orc_trace_ray(const orc_geom* g, const double* cosT, const double* sinT,
         }
         if (icheck == 1 && chy != 0) {
             iks++;
-            emit_segment(&sink, ab0, bb0, aiii, biii);
+            emit_segment(&sink, iks, ab0, bb0, aiii, biii);
             ab0 = aiii; bb0 = biii;
         }
         iks++;
-        emit_segment(&sink, ab0, bb0, aii, bii);
+        emit_segment(&sink, iks, ab0, bb0, aii, bii);
         ab0 = aii; bb0 = bii;
         ik++;
     }

@@ -52,6 +52,12 @@ int orc_trace_ray(const orc_geom* g, const double* cosT, const double* sinT,
                   const double* px0, const double* py0, int view, int ch,
                   int32_t* vox, double* rr, int maxseg, int* n_iks);
 
+/* Same, also writing each segment's reference counter value (the index
+ * under which M/reprojection_TOMO.cpp:160-163 caches it). */
+int orc_trace_ray_iks(const orc_geom* g, const double* cosT, const double* sinT,
+                      const double* px0, const double* py0, int view, int ch,
+                      int32_t* vox, double* rr, int32_t* seg_iks, int maxseg, int* n_iks);
+
 /* Per-ray candidate segment counts over the whole sinogram [V][W]. */
 int64_t orc_segment_counts(const orc_geom* g, int32_t* counts);
 

@@ -1,0 +1,236 @@
+"""GPU parity of the MBFPA forward path (tracer, polyenergetic sum,
+normalisation, detector noise) against the oracle and the reference's golden
+vectors.  Every call goes through the C ABI of libvct.so."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2, rel_max
+from oracle.oracle import Geom
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # relative parity bar for FP32 sinograms (max|d| / max|ref|)
+
+
+def _ops():
+    from virtualct_amd import ops
+    return ops
+
+
+def _geom(N=512, pix=0.1, V=800):
+    from virtualct_amd import default_geometry
+    return default_geometry(recon_size=N, pixel_cm=pix, n_views=V)
+
+
+def _t(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _lnrel(a, b, floor=1e-6):
+    """max relative error in the -ln domain over rays with b > floor"""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    m = b > floor
+    la, lb = -np.log(np.maximum(a[m], 1e-300)), -np.log(b[m])
+    return float(np.abs(la - lb).max() / max(np.abs(lb).max(), 1e-12))
+
+
+def test_segment_counts_match_reference(gpu, golden, oracle):
+    ops = _ops()
+    plan = ops.plan_for(_geom())
+    pk, dt = ops.pack_densities(plan, _t(golden.icru110(), gpu))
+    L, cnt = ops.trace_paths(plan, pk, dt, 6, counts=True)
+    cnt = cnt.cpu().numpy()
+    ref_cnt, total = oracle.segment_counts(Geom())
+    assert int(cnt.astype(np.int64).sum()) == 282_824_404 == total
+    np.testing.assert_array_equal(cnt, ref_cnt)
+
+
+def test_path_integrals_match_fp64_oracle(gpu, golden, oracle):
+    ops = _ops()
+    ph = golden.icru110()
+    plan = ops.plan_for(_geom())
+    pk, dt = ops.pack_densities(plan, _t(ph, gpu))
+    L = ops.trace_paths(plan, pk, dt, 6).cpu().numpy()[..., :6]
+    dens = ph.astype(np.float32) / np.float32(1000)
+    att = np.zeros((6, 1), np.float32)
+    _, Lo = oracle.forward_f64(Geom(), dens, att, np.ones(1, np.float32), want_L=True)
+    assert rel_max(L, Lo) < 1e-6
+    # float densities give the same integrals
+    pkf, dtf = ops.pack_densities(plan, _t(dens, gpu))
+    Lf = ops.trace_paths(plan, pkf, dtf, 6).cpu().numpy()[..., :6]
+    assert rel_max(Lf, Lo) < 1e-6
+
+
+@pytest.mark.parametrize("case,N,pix,V", [("n64", 64, 0.7, 80), ("n128", 128, 0.35, 160)])
+def test_small_geometry_matches_reference(gpu, golden, case, N, pix, V):
+    ops = _ops()
+    t = golden.npz("ref_small.npz")
+    plan = ops.plan_for(_geom(N, pix, V))
+    pk, dt = ops.pack_densities(plan, _t(t[case + "_phantom"], gpu))
+    L, cnt = ops.trace_paths(plan, pk, dt, 6, counts=True)
+    np.testing.assert_array_equal(cnt.cpu().numpy(), t[case + "_counts"])
+    raw, stats, _ = ops.spectral_sum(plan, L, 6, _t(t["att"], gpu), _t(t["pdf"], gpu))
+    raw = raw[0].cpu().numpy()
+    ref = t[case + "_raw"]
+    assert rel_max(raw, ref) < TOL
+    assert _lnrel(raw, ref) < TOL
+    mx, mn = ops.stats_values(stats)[0].tolist()
+    assert mx == raw.max() and mn == raw.min()
+
+
+def test_icru110_120kV_noise_free_matches_reference(gpu, golden, oracle):
+    ops = _ops()
+    from virtualct_amd import spectra
+    att, pdf, _ = spectra.attenuation(golden.path("spectrum_120kV.txt"), 6)
+    plan = ops.plan_for(_geom())
+    sino = ops.mbfpa_forward(plan, _t(golden.icru110(), gpu), _t(att, gpu), _t(pdf, gpu), sigma=0.0)
+    sino = sino[0].cpu().numpy()
+    ref = golden.sigma0_120kv()
+    assert rel_max(sino, ref) < TOL
+    assert rel_l2(sino, ref) < TOL
+    assert _lnrel(sino, ref) < TOL
+    # against the FP64 oracle
+    f64 = oracle.forward_f64(Geom(), golden.icru110().astype(np.float32) / np.float32(1000), att, pdf)
+    assert rel_max(sino, f64 / f64.max()) < TOL
+    # rays that miss the image are exactly air after normalisation
+    assert np.all(sino[ref == ref.max()] == sino.max())
+
+
+def test_shipped_6MV_650_bins(gpu, golden, oracle):
+    ops = _ops()
+    from virtualct_amd import spectra
+    att, pdf, _ = spectra.attenuation(golden.path("spectrum_6MV.txt"), 6)
+    assert att.shape == (6, 650)
+    plan = ops.plan_for(_geom())
+    sino = ops.mbfpa_forward(plan, _t(golden.icru110(), gpu), _t(att, gpu), _t(pdf, gpu))[0].cpu().numpy()
+    f64 = oracle.forward_f64(Geom(), golden.icru110().astype(np.float32) / np.float32(1000), att, pdf)
+    assert rel_max(sino, f64 / f64.max()) < TOL
+    assert _lnrel(sino, f64 / f64.max()) < TOL
+
+
+def test_dual_spectrum_synthetic_body_phantom(gpu, golden, oracle, tmp_path):
+    """120 kV and 6 MV synthetic spectra from one trace (n_spectra = 2)."""
+    ops = _ops()
+    from virtualct_amd import phantoms, spectra
+    ph = phantoms.body_phantom(1000)
+    tabs = [spectra.attenuation(spectra.synthetic_spectrum_file(n, str(tmp_path)), 6) for n in ("120kV", "6MV")]
+    NE = max(a.shape[1] for a, _, _ in tabs)
+    att = np.zeros((2, 6, NE), np.float32)
+    pdf = np.zeros((2, NE), np.float32)
+    for s, (a, p, _) in enumerate(tabs):
+        att[s, :, : a.shape[1]] = a
+        pdf[s, : p.shape[0]] = p
+    plan = ops.plan_for(_geom())
+    sino = ops.mbfpa_forward(plan, _t(ph, gpu), _t(att, gpu), _t(pdf, gpu)).cpu().numpy()
+    dens = phantoms.densities_f32(ph)
+    for s, (a, p, _) in enumerate(tabs):
+        f64 = oracle.forward_f64(Geom(), dens, a, p)
+        assert rel_max(sino[s], f64 / f64.max()) < TOL, s
+        assert _lnrel(sino[s], f64 / f64.max()) < TOL, s
+
+
+def test_eight_element_calibration_phantom(gpu, oracle, tmp_path):
+    ops = _ops()
+    from virtualct_amd import phantoms, spectra
+    ph = phantoms.calibration_phantom(2000)
+    att, pdf, _ = spectra.attenuation(spectra.synthetic_spectrum_file("120kV", str(tmp_path)), 8)
+    plan = ops.plan_for(_geom())
+    sino = ops.mbfpa_forward(plan, _t(ph, gpu), _t(att, gpu), _t(pdf, gpu))[0].cpu().numpy()
+    f64 = oracle.forward_f64(Geom(), phantoms.densities_f32(ph), att, pdf)
+    assert rel_max(sino, f64 / f64.max()) < TOL
+    assert _lnrel(sino, f64 / f64.max()) < TOL
+
+
+def test_noise_matches_philox_box_muller(gpu, golden, oracle):
+    """Noise of element r is sigma * sqrt(-2 ln U1) sin(2 pi U2) with U from
+    Philox4x32-10(ctr = (r, stream, 0, 0), key = seed)."""
+    import torch
+    ops = _ops()
+    plan = ops.plan_for(_geom())
+    clean = golden.sigma0_120kv()
+    raw = _t(clean[None], gpu)
+    mx = np.float32(1.0)
+    stats = torch.tensor([[np.float32(mx).view(np.int32), np.float32(clean.min()).view(np.int32)]],
+                         dtype=torch.int32, device=gpu)
+    sigma, seed, sid = 5.37e-5, 0x1234_5678_9ABC, 7
+    noisy = ops.detector(plan, raw, stats, sigma, seed, sid)[0].cpu().numpy().reshape(-1)
+    flat = clean.reshape(-1)
+    idx = np.random.default_rng(0).choice(flat.size, 2000, replace=False)
+    for r in idx:
+        u = oracle.philox([r, sid, 0, 0], [seed & 0xFFFFFFFF, seed >> 32])
+        u1 = (float(u[0]) + 1.0) / 4294967297.0
+        u2 = (float(u[1]) + 1.0) / 4294967297.0
+        z = np.sqrt(-2.0 * np.log(u1)) * np.sin(2.0 * np.pi * u2)
+        x = float(flat[r]) + sigma * z
+        exp = np.float32(x)
+        if x < 0:
+            exp = np.float32(float(np.float32(clean.min() / mx)) / 10)
+        if x > 1.0:
+            exp = np.float32(1.0)
+        assert abs(float(noisy[r]) - float(exp)) <= 2 * np.spacing(np.float32(exp)), r
+
+
+def test_noise_statistics_per_bin(gpu, golden):
+    """K independent streams: per-bin mean is the clean value, variance sigma^2,
+    and bins at 1.0 are clamped half of the time."""
+    import torch
+    ops = _ops()
+    plan = ops.plan_for(_geom())
+    clean = golden.sigma0_120kv()
+    K, sigma = 200, 5.37e-5
+    raw = _t(clean, gpu).unsqueeze(0).expand(K, -1, -1).contiguous()
+    st = torch.tensor([[np.float32(1.0).view(np.int32), np.float32(clean.min()).view(np.int32)]] * K,
+                      dtype=torch.int32, device=gpu)
+    noisy = ops.detector(plan, raw, st, sigma, 0x5EED, 0).double()
+    c = torch.from_numpy(clean).to(gpu).double()
+    mean = noisy.mean(0)
+    var = noisy.var(0)
+    ok = (c > 6 * sigma) & (c < 1 - 6 * sigma)
+    z = ((mean - c) / (sigma / K ** 0.5))[ok]
+    assert abs(z.mean().item()) < 0.02
+    assert abs(z.std().item() - 1.0) < 0.02
+    assert z.abs().max().item() < 6.5
+    ratio = (var / sigma ** 2)[ok]
+    assert abs(ratio.mean().item() - 1.0) < 0.005
+    assert abs(ratio.std().item() - (2.0 / (K - 1)) ** 0.5) < 0.01
+    air = c == 1.0
+    frac = (noisy[:, air] == 1.0).double().mean().item()
+    assert abs(frac - 0.5) < 0.01
+    assert noisy.max().item() <= 1.0 and noisy.min().item() >= 0.0
+    # same seed and stream -> identical draw; another stream -> different
+    a = ops.detector(plan, raw[:1], st[:1], sigma, 99, 3)
+    b = ops.detector(plan, raw[:1], st[:1], sigma, 99, 3)
+    d = ops.detector(plan, raw[:1], st[:1], sigma, 99, 4)
+    assert torch.equal(a, b) and not torch.equal(a, d)
+
+
+def test_sigma_zero_is_plain_normalisation(gpu, golden):
+    import torch
+    ops = _ops()
+    plan = ops.plan_for(_geom())
+    clean = golden.sigma0_120kv() * np.float32(0.731)
+    raw = _t(clean[None], gpu)
+    st = torch.tensor([[np.float32(clean.max()).view(np.int32), np.float32(clean.min()).view(np.int32)]],
+                      dtype=torch.int32, device=gpu)
+    out = ops.detector(plan, raw, st, 0.0)[0].cpu().numpy()
+    np.testing.assert_array_equal(out, clean / clean.max())
+
+
+def test_argument_errors_raise(gpu):
+    ops = _ops()
+    import torch
+    from virtualct_amd import VctError
+    plan = ops.plan_for(_geom())
+    with pytest.raises(ValueError):
+        ops.pack_densities(plan, torch.zeros((6, 256, 256), dtype=torch.int16, device=gpu))
+    pk, dt = ops.pack_densities(plan, torch.zeros((6, 512, 512), dtype=torch.int16, device=gpu))
+    L = ops.trace_paths(plan, pk, dt, 6)
+    with pytest.raises(VctError):
+        ops.spectral_sum(plan, L, 6, torch.zeros((6, 2000), device=gpu), torch.zeros(2000, device=gpu))
+    with pytest.raises(VctError):
+        ops.detector(plan, torch.zeros((1, 800, 609), device=gpu), torch.zeros((1, 2), dtype=torch.int32,
+                                                                                  device=gpu), -1.0)

@@ -1,0 +1,165 @@
+// vct_fbp.hip -- fan-beam FBP for CDNA4 (gfx950).
+//
+//   k_filter_prep  q[r][k] = -ln p[r][k] * sid * cos((k - W/2) * dbeta), the
+//                  reference's per-channel pre-weight (F/filterSinogramTOMO.cpp:236-242),
+//                  written straight into the register layout of the MFMA A
+//                  operand (zero padded to Kp).
+//   k_filter_gemm  fltd = q . C'^T with v_mfma_f32_32x32x2f32, where
+//                  C'[m][k] = C[m][k] * dbeta is the reference's dense filter
+//                  matrix (prepareConstMap, F/filterSinogramTOMO.cpp:9-138;
+//                  Shepp-Logan or Ram-Lak with edge-row renormalisation, so
+//                  it is stored dense rather than generated as a Toeplitz
+//                  stencil).  One wave owns a 32x32 output tile and half of
+//                  K; operands are pre-packed so each lane issues one 16-byte
+//                  load per four MFMAs; the two halves meet in LDS.
+//   k_backproject  pixel-driven fan backprojection
+//                  (F/reconstructImageTOMO.cpp:46-100): a 16x16 pixel tile per
+//                  workgroup, chunks of filtered views staged in LDS, FP64
+//                  geometry and the reference's float accumulation order.
+#include "vct_internal.h"
+
+#include <climits>
+#include <cmath>
+
+namespace vct {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kBpTile = 16;   // pixel tile edge
+constexpr int kBpViews = 16;  // views staged in LDS per chunk
+
+__device__ __forceinline__ int trunc_i32_bp(double v)
+{
+    return (v > -2147483649.0 && v < 2147483648.0) ? (int)v : INT_MIN;
+}
+
+// Packed operand index: tile (32 rows or columns), K position k, row/col in
+// tile rr.  Lane l = rr + 32*(k&1) holds step t = k>>1; four consecutive
+// steps share one float4.
+__host__ __device__ __forceinline__ size_t pack_index(int tile, int k, int rr, int Kp)
+{
+    const int t = k >> 1, hi = k & 1;
+    const int q = t >> 2, j = t & 3;
+    const int l = rr + 32 * hi;
+    return ((((size_t)tile * (Kp / 8) + q) * 64 + l) * 4) + j;
+}
+
+__global__ __launch_bounds__(256) void k_filter_prep(FilterPrepArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (k >= a.Kp) return;
+    float v = 0.f;
+    if (r < a.R && k < a.W) {
+        const double p = a.proj[(size_t)r * a.W + k];
+        v = (float)(-log(p) * a.coeff[k]);
+    }
+    a.apk[pack_index(r >> 5, k, r & 31, a.Kp)] = v;
+}
+
+__global__ __launch_bounds__(128) void k_filter_gemm(FilterGemmArgs a)
+{
+    __shared__ float part[16][64];
+    const int tr = blockIdx.x, tc = blockIdx.y;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int Q = a.Kp / 8;  // quads of 4 MFMA steps
+    const int q0 = wave * (Q / 2), q1 = (wave + 1) * (Q / 2);
+    const float4* A = reinterpret_cast<const float4*>(a.apk) + ((size_t)tr * Q) * 64 + lane;
+    const float4* B = reinterpret_cast<const float4*>(a.bpk) + ((size_t)tc * Q) * 64 + lane;
+    floatx16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = 0.f;
+#pragma unroll 2
+    for (int q = q0; q < q1; q++) {
+        const float4 av = A[(size_t)q * 64];
+        const float4 bv = B[(size_t)q * 64];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bv.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bv.y, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bv.z, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bv.w, acc, 0, 0, 0);
+    }
+    if (wave == 1) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) part[i][lane] = acc[i];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const int j = lane & 31;
+        const int col = tc * 32 + j;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int row_in = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+            const int row = tr * 32 + row_in;
+            float v = acc[i] + part[i][lane];
+            if (a.in_mode && (v < 1000.0f || v > 300000.0f)) v = 0.f;  // :250-255
+            if (row < a.R && col < a.W) a.out[(size_t)row * a.W + col] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_backproject(BackprojArgs a)
+{
+    extern __shared__ float sv[];  // [kBpViews][W]
+    const int tx = threadIdx.x & (kBpTile - 1), ty = threadIdx.x / kBpTile;
+    const int x = blockIdx.x * kBpTile + tx;
+    const int y = blockIdx.y * kBpTile + ty;
+    const int s = blockIdx.z;
+    const float* fl = a.fltd + (size_t)s * a.V * a.W;
+    const bool in_img = x < a.N && y < a.N;
+    // prepareMaskData, F/misc.cpp:20-36
+    const double dyc = (double)y - a.mask_r, dxc = (double)x - a.mask_r;
+    const bool masked = in_img && (dyc * dyc + dxc * dxc <= a.mask_r2);
+    const double xx = -(x - a.center) * a.pix;  // :52-53
+    const double yy = -(y - a.center) * a.pix;
+    float acc = 0.f;
+    for (int th0 = 0; th0 < a.use_views; th0 += kBpViews) {
+        const int nv = min(kBpViews, a.use_views - th0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nv * a.W; i += blockDim.x)
+            sv[i] = fl[(size_t)th0 * a.W + i];
+        __syncthreads();
+        if (masked) {
+            for (int t = 0; t < nv; t++) {
+                const int th = th0 + t;
+                const double c = a.cosT[th], sn = a.sinT[th];
+                const double X = xx * c + yy * sn;
+                const double Y = -xx * sn + yy * c;
+                const double L2 = (-Y + a.sid) * (-Y + a.sid) + X * X;
+                const double fGAM = atan(X / (a.sid - Y)) / a.dbeta_ch + a.cent;
+                const int GAM = trunc_i32_bp(fGAM);
+                const double w = fGAM - GAM;
+                if (GAM >= 0 && GAM < a.W - 1) {
+                    const float* row = sv + t * a.W;
+                    const double sum1 = (1 - w) * row[GAM] + w * row[GAM + 1];
+                    const double sum = sum1 + 0.0;
+                    acc += (float)(1.0 / L2 * sum * a.dbeta_view * a.xfactor);
+                }
+            }
+        }
+    }
+    if (in_img) a.img[((size_t)s * a.N + y) * a.N + x] = acc;
+}
+
+vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipStream_t s)
+{
+    if (p.Kp % 16 != 0) return set_error(VCT_ERR_INTERNAL, "filter K padding %d not a multiple of 16", p.Kp);
+    const int Rp = (p.R + 31) / 32 * 32;
+    hipLaunchKernelGGL(k_filter_prep, dim3((p.Kp + 255) / 256, Rp), dim3(256), 0, s, p);
+    VCT_HIP(hipGetLastError());
+    const int Wp = (g.W + 31) / 32 * 32;
+    hipLaunchKernelGGL(k_filter_gemm, dim3(Rp / 32, Wp / 32), dim3(128), 0, s, g);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s)
+{
+    const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, S);
+    const size_t lds = sizeof(float) * (size_t)kBpViews * a.W;
+    if (lds > 64 * 1024) return set_error(VCT_ERR_UNSUPPORTED, "W=%d too large for the LDS view chunk", a.W);
+    hipLaunchKernelGGL(k_backproject, grid, dim3(kBpTile * kBpTile), lds, s, a);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+}  // namespace vct

@@ -1,0 +1,149 @@
+// vct_internal.h -- internal types shared by the libvct translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "vct.h"
+
+namespace vct {
+
+// Thread-local error message + status helper.
+vct_status set_error(vct_status s, const char* fmt, ...);
+void clear_error();
+
+#define VCT_HIP(call)                                                               \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return ::vct::set_error(VCT_ERR_HIP, "%s failed: %s (%s:%d)", #call,    \
+                                    hipGetErrorString(e_), __FILE__, __LINE__);     \
+    } while (0)
+
+#define VCT_CHECK(st)                         \
+    do {                                      \
+        vct_status s_ = (st);                 \
+        if (s_ != VCT_OK) return s_;          \
+    } while (0)
+
+// Filter GEMM padding: K and W are padded to a multiple of kFilterPad.
+constexpr int kFilterPad = 64;
+// Maximum spectra evaluated by one spectral launch and maximum bins.
+constexpr int kMaxBins = 1024;
+
+// Tracer launch parameters (passed by value).
+struct TraceArgs {
+    const double* cosT;   // [V]
+    const double* sinT;   // [V]
+    const double* px0;    // [W] detector point before rotation (x)
+    const double* py0;    // [W] detector point before rotation (y)
+    const void* dens;     // packed [N][N][8] (u16 or f32)
+    float* L;             // [V][W][8] path integrals (forward)
+    int32_t* counts;      // [V][W] optional segment counter
+    const float* wts;     // [V][W][8] adjoint weights (adjoint)
+    const float* g;       // [V][W] optional upstream gradient (adjoint)
+    float* grad;          // [N][N][8] (adjoint)
+    int* err;             // device error flag (step bound exceeded)
+    int N, W, V, VT;      // VT = views traced = V/2*2
+    int nxm;              // (int)(N*0.5)
+    int max_steps;        // hard bound on marching steps per ray
+    int rcheck;           // 1 when the reference's r >= rmax test can fire
+    int ray_offset;       // first ray handled by this launch
+    int n_rays;           // rays handled by this launch
+    double D;             // pixel pitch
+    double invD;          // 1/D (fast path for the voxel index, with exact fallback)
+    double rmax;          // N*0.5 (reference :37)
+    double sid;           // source to isocentre
+    double out_scale;     // factor applied to the accumulated sums (1/1000 for u16, then /xfactor)
+};
+
+struct SpectralArgs {
+    const float* L;       // [V][W][8]
+    const float* att;     // [S][M][NE]
+    const float* pdf;     // [S][NE]
+    float* raw;           // [S][V][W]
+    uint32_t* stats;      // [S][2] = {max bits, min bits}
+    float* dIdL;          // [S][V][W][8] optional
+    int W, V, VT, NE, S, M;
+};
+
+struct DetectorArgs {
+    const float* raw;       // [S][V*W]
+    const uint32_t* stats;  // [S][2]
+    float* out;             // [S][V*W]
+    int n;                  // V*W
+    int S;
+    double sigma;
+    uint32_t key0, key1;    // Philox key (seed)
+    uint32_t stream_id;
+};
+
+struct FilterPrepArgs {
+    const float* proj;    // [R][W]
+    const double* coeff;  // [W] sid*cos((k - W/2) * dbeta)
+    float* apk;           // packed A operand
+    int R, W, Kp;
+};
+
+struct FilterGemmArgs {
+    const float* apk;     // packed A [R/32][Kp/8][64][4]
+    const float* bpk;     // packed B [Wp/32][Kp/8][64][4]
+    float* out;           // [R][W]
+    int R, W, Kp, in_mode;
+};
+
+struct BackprojArgs {
+    const float* fltd;    // [S][V][W]
+    const double* cosT;   // [V]
+    const double* sinT;   // [V]
+    float* img;           // [S][N][N]
+    int N, W, V, use_views, cent, half;
+    double pix, sid, dbeta_ch, dbeta_view, xfactor, center, mask_r2, mask_r;
+};
+
+}  // namespace vct
+
+struct vct_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int* d_err = nullptr;          // kernel error flag
+    void* scratch = nullptr;       // grows on demand
+    size_t scratch_bytes = 0;
+    int num_cu = 0;
+    vct_status ensure_scratch(size_t bytes);
+};
+
+struct vct_plan {
+    vct_ctx* ctx = nullptr;
+    vct_geometry g{};
+    int half = 0;       // (W-1)/2
+    int cent = 0;       // W/2
+    int nxm = 0;        // (int)(N*0.5)
+    int VT = 0;         // views traced
+    int Kp = 0;         // padded K for the filter GEMM
+    int Wp = 0;         // padded W for the filter GEMM
+    double* d_cos = nullptr;
+    double* d_sin = nullptr;
+    double* d_px0 = nullptr;
+    double* d_py0 = nullptr;
+    double* d_coeff = nullptr;
+    float* d_bpk[3] = {nullptr, nullptr, nullptr};  // packed filter matrix per type
+};
+
+namespace vct {
+// kernels launchers (implemented in the .hip files)
+vct_status launch_trace(const TraceArgs& a, int dens_type, int M, bool adjoint, hipStream_t s);
+vct_status launch_spectral(const SpectralArgs& a, hipStream_t s);
+vct_status launch_stats_init(uint32_t* stats, int S, hipStream_t s);
+vct_status launch_detector(const DetectorArgs& a, hipStream_t s);
+vct_status launch_minmax(const float* x, int n, uint32_t* stats, hipStream_t s);
+vct_status launch_pack_i16(const int16_t* src, int M, int N, uint16_t* dst, hipStream_t s);
+vct_status launch_pack_f32(const float* src, int M, int N, float* dst, hipStream_t s);
+vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipStream_t s);
+vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s);
+vct_status plan_filter_matrix(vct_plan* plan, int filter, float** d_bpk);
+TraceArgs make_trace_args(const vct_plan* plan);
+}  // namespace vct

@@ -1,0 +1,641 @@
+// vct_trace.hip -- MBFPA ray tracer, polyenergetic detector sum, detector
+// noise and phantom packing for CDNA4 (gfx950).
+//
+// Design (one thread per ray, 64 adjacent channels of one view per wave):
+//   k_trace      marches each fan ray once through the image in FP64 and
+//                accumulates the per-element path integrals
+//                L_m = sum_seg rho_m(voxel) * length in FP64 registers; the
+//                segment decomposition (entry/exit clip, integer-cm start
+//                plane, midpoint voxel, minor-axis crossing split) follows
+//                M/reprojection_TOMO.cpp:53-283 operation for operation, so
+//                it is bit-identical to the reference when compiled with
+//                -ffp-contract=off.  The reference re-traces every ray once
+//                per energy bin (M/main.cpp:492); here the trace is done once
+//                and the spectrum is applied afterwards.
+//   k_spectral   I_s = sum_E pdf_s(E) exp(-sum_m mu_s,m(E) L_m) with the
+//                spectrum tables of spectrum s in LDS, plus the per-spectrum
+//                max/min used by the normalisation and the noise clamp.
+//   k_detector   max-normalisation + Gaussian noise (Philox4x32-10 +
+//                Box-Muller) + the reference's clamps.
+//   k_trace<adjoint>  the same march, scattering w_m * length into the
+//                gradient map with hardware float atomics.
+#include "vct_internal.h"
+
+#include <climits>
+#include <cmath>
+
+namespace vct {
+
+// x86-64 cvttsd2si semantics (the reference's (int) casts): NaN and values
+// outside int range give INT_MIN.
+__device__ __forceinline__ int trunc_i32(double v)
+{
+    return (v > -2147483649.0 && v < 2147483648.0) ? (int)v : INT_MIN;
+}
+
+// trunc(fl(fl(h / D) + c)) -- the voxel index expressions of the reference
+// (M/reprojection_TOMO.cpp:130,146-147,217).  The product h * (1/D) differs
+// from the correctly rounded quotient by a few ulp at most, so its
+// truncation can only differ when the value lies within a few ulp of an
+// integer; such values (|frac| < 1e-9 from an integer, ~1e4 x the error
+// bound for |values| <= 2^11) take the exact division.
+template <bool FAST>
+__device__ __forceinline__ int index_of(double h, double D, double invD, double c)
+{
+    if (FAST) {
+        const double q = h * invD + c;
+        const double t = trunc(q);
+        const double f = fabs(q - t);
+        if (f > 1e-9 && f < 1.0 - 1e-9 && fabs(q) < 1.0e6) return (int)t;
+    }
+    return trunc_i32(h / D + c);
+}
+
+// (u/fabs(u) != v/fabs(v)) of M/reprojection_TOMO.cpp:134,221 for finite
+// operands, without the divisions: true if either is zero or NaN (0/0 is
+// NaN, which compares unequal) or the signs differ.
+__device__ __forceinline__ bool sign_split(double u, double v)
+{
+    const bool up = u > 0.0, un = u < 0.0, vp = v > 0.0, vn = v < 0.0;
+    return !(up || un) || !(vp || vn) || (up != vp);
+}
+
+// ------------------------------------------------------------------ sinks
+
+template <int M>
+struct SinkU16 {
+    const uint4* __restrict__ vox;
+    double acc[M];
+    __device__ __forceinline__ void init(const void* p)
+    {
+        vox = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[m] = 0.0;
+    }
+    __device__ __forceinline__ void add(int v, double rr)
+    {
+        const uint4 q = vox[v];
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const uint32_t u = (m & 1) ? (w[m >> 1] >> 16) : (w[m >> 1] & 0xffffu);
+            acc[m] = __builtin_fma((double)u, rr, acc[m]);
+        }
+    }
+};
+
+template <int M>
+struct SinkF32 {
+    const float4* __restrict__ vox;
+    double acc[M];
+    __device__ __forceinline__ void init(const void* p)
+    {
+        vox = reinterpret_cast<const float4*>(p);
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[m] = 0.0;
+    }
+    __device__ __forceinline__ void add(int v, double rr)
+    {
+        const float4 a = vox[2 * v];
+        float r[8] = {a.x, a.y, a.z, a.w, 0.f, 0.f, 0.f, 0.f};
+        if (M > 4) {
+            const float4 b = vox[2 * v + 1];
+            r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            // a negative density contributes nothing (reference ph = 0, :155)
+            const double rho = r[m] > 0.f ? (double)r[m] : 0.0;
+            acc[m] = __builtin_fma(rho, rr, acc[m]);
+        }
+    }
+};
+
+// Adjoint sink: grad[v][m] += w[m] * rr for every in-image segment.
+template <int M>
+struct SinkAdj {
+    float* __restrict__ grad;
+    const float4* __restrict__ dens;  // f32 densities for the sign test, or null
+    float w[M];
+    __device__ __forceinline__ void add(int v, double rr)
+    {
+        const float r = (float)rr;
+        float keep[8] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+        if (dens) {
+            const float4 a = dens[2 * v];
+            keep[0] = a.x >= 0.f; keep[1] = a.y >= 0.f; keep[2] = a.z >= 0.f; keep[3] = a.w >= 0.f;
+            if (M > 4) {
+                const float4 b = dens[2 * v + 1];
+                keep[4] = b.x >= 0.f; keep[5] = b.y >= 0.f; keep[6] = b.z >= 0.f; keep[7] = b.w >= 0.f;
+            }
+        }
+        float* dst = grad + (size_t)v * 8;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const float val = w[m] * r * keep[m];
+            if (val != 0.f) unsafeAtomicAdd(dst + m, val);
+        }
+    }
+};
+
+// ------------------------------------------------------------------ march
+
+// Trace ray (view, ch).  Calls sink.add(voxel, length) for every segment
+// whose midpoint voxel lies in the image, in the reference's order, and
+// returns the reference's per-ray segment counter (iks).  *ovf is set when
+// the step bound is hit (never for valid geometry).
+template <bool FAST, class Sink>
+__device__ __forceinline__ int march_ray(const TraceArgs& a, int view, int ch, Sink& sink, bool& ovf)
+{
+    // source and detector point, M/reprojection_TOMO.cpp:53-74
+    const double c = a.cosT[view], s = a.sinT[view];
+    const double sx = a.sid * s;
+    const double sy = a.sid * c;
+    const double px0 = a.px0[ch], py0 = a.py0[ch];
+    const double px = +px0 * c + py0 * s;
+    const double py = -px0 * s + py0 * c;
+    const double Lb = (double)a.nxm * a.D;
+    const double D = a.D;
+
+    // clip against the image square, :76-101 (only the first two complete
+    // intersections are used)
+    int m = 0;
+    double h0x = 0.0, h0y = 0.0, h1x = 0.0, h1y = 0.0;
+    {
+        const double y = Lb;
+        const double x = (y - sy) / (py - sy) * (px - sx) + sx;
+        if (fabs(x) < Lb) { h0x = x; h0y = y; m = 1; }
+    }
+    {
+        const double y = -Lb;
+        const double x = (y - sy) / (py - sy) * (px - sx) + sx;
+        if (fabs(x) < Lb) {
+            if (m == 0) { h0x = x; h0y = y; } else { h1x = x; h1y = y; }
+            m++;
+        }
+    }
+    {
+        const double x = Lb;
+        const double y = (x - sx) / (px - sx) * (py - sy) + sy;
+        if (fabs(y) < Lb) {
+            if (m == 0) { h0x = x; h0y = y; } else if (m == 1) { h1x = x; h1y = y; }
+            m++;
+        }
+    }
+    {
+        const double x = -Lb;
+        const double y = (x - sx) / (px - sx) * (py - sy) + sy;
+        if (fabs(y) < Lb) {
+            if (m == 0) { h0x = x; h0y = y; } else if (m == 1) { h1x = x; h1y = y; }
+            m++;
+        }
+    }
+    if (m < 2) return 0;  // the ray misses the image: no segment, transmission 1
+
+    // entry = the intersection closer to the source, :103-108
+    const double d0 = (h0x - sx) * (h0x - sx) + (h0y - sy) * (h0y - sy);
+    const double d1 = (h1x - sx) * (h1x - sx) + (h1y - sy) * (h1y - sy);
+    const bool first = d0 < d1;
+    const double xi = first ? h0x : h1x, yi = first ? h0y : h1y;
+    const double xf = first ? h1x : h0x, yf = first ? h1y : h0y;
+
+    // dominant axis, :113.  a = major axis, b = minor axis; the reference's
+    // two branches (:113-200, :201-283) are this march with x and y exchanged.
+    const bool xmajor = fabs(xi - xf) > fabs(yi - yf);
+    const double ai = xmajor ? xi : yi, af = xmajor ? xf : yf;
+    const double bi = xmajor ? yi : xi, bf = xmajor ? yf : xf;
+    const double sa = xmajor ? sx : sy, sb = xmajor ? sy : sx;
+    const double pa = xmajor ? px : py, pb = xmajor ? py : px;
+
+    const double pha = (af - ai) / fabs(ai - af);
+    int ain;
+    if ((ai < 0 && (af - ai) < 0) || (ai > 0 && (af - ai) > 0))
+        ain = trunc_i32(trunc_i32(ai) + (af - ai) / fabs(af - ai));
+    else
+        ain = trunc_i32(ai);
+    const double bdir = (bf - bi) / fabs(bf - bi);
+    const bool b_out = (bf - bi) > 0;  // minor axis moves towards +
+    const bool b_in = (bf - bi) < 0;
+
+    double aii = ain;
+    double bii = (aii - sa) / (pa - sa) * (pb - sb) + sb;
+    double ab0 = ai, bb0 = bi;
+    int bma = 0;
+    int iks = 0;
+
+    // one candidate segment (a0,b0)->(a1,b1), :141-166 / :168-189
+    auto emit = [&](double a0, double b0, double a1, double b1) {
+        const double xb0 = xmajor ? a0 : b0, yb0 = xmajor ? b0 : a0;
+        const double xb1 = xmajor ? a1 : b1, yb1 = xmajor ? b1 : a1;
+        const int xmc = index_of<FAST>((xb0 + xb1) * 0.5, D, a.invD, (double)a.nxm);
+        const int ymc = index_of<FAST>((-yb0 - yb1) * 0.5, D, a.invD, (double)a.nxm);
+        if (xmc >= 0 && ymc >= 0 && xmc <= a.N - 1 && ymc <= a.N - 1) {
+            if (a.rcheck) {
+                const double r = sqrt((xb0 + xb1) * (xb0 + xb1) + (-yb0 - yb1) * (-yb0 - yb1)) * 0.5;
+                if (r >= a.rmax) return;
+            }
+            const double rr = sqrt((xb0 - xb1) * (xb0 - xb1) + (yb0 - yb1) * (yb0 - yb1));
+            sink.add(xmc + ymc * a.N, rr);
+        }
+    };
+
+    for (int ik = 0;; ik++) {
+        if (!(aii <= Lb && aii >= -Lb && bii <= Lb && bii >= -Lb)) break;
+        if (ik >= a.max_steps) { ovf = true; break; }
+        aii = ain + pha * ik * D;  // :126
+        const int bmb = bma;
+        const double bj = bii;
+        bii = (aii - sa) / (pa - sa) * (pb - sb) + sb;  // :129
+        bma = index_of<FAST>(bii, D, a.invD, 0.0);      // :130
+        if (ik != 0 && (bma != bmb || sign_split(bii, bj))) {  // :134
+            double biii;
+            if ((bii < 0 && b_in) || (bii > 0 && b_out))
+                biii = (double)(bma * D);
+            else
+                biii = (double)(bma * D) - bdir * D;
+            const double aiii = (biii - sb) / (pb - sb) * (pa - sa) + sa;
+            const double chy = (aii - aiii) * (aii - aiii) + (bii - biii) * (bii - biii);
+            if (chy != 0) {  // :141
+                iks++;
+                emit(ab0, bb0, aiii, biii);
+                ab0 = aiii;
+                bb0 = biii;
+            }
+        }
+        iks++;
+        emit(ab0, bb0, aii, bii);
+        ab0 = aii;
+        bb0 = bii;
+    }
+    return iks;
+}
+
+// ---------------------------------------------------------------- kernels
+
+template <int M, int DT, bool FAST>
+__global__ __launch_bounds__(256) void k_trace_fwd(TraceArgs a)
+{
+    const int local = blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= a.n_rays) return;
+    const int ray = a.ray_offset + local;
+    const int view = ray / a.W;
+    const int ch = ray - view * a.W;
+    float out[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int iks = 0;
+    if (view < a.VT) {
+        bool ovf = false;
+        if (DT == VCT_DENS_U16) {
+            SinkU16<M> sink;
+            sink.init(a.dens);
+            iks = march_ray<FAST>(a, view, ch, sink, ovf);
+#pragma unroll
+            for (int m = 0; m < M; m++) out[m] = (float)(sink.acc[m] * a.out_scale);
+        } else {
+            SinkF32<M> sink;
+            sink.init(a.dens);
+            iks = march_ray<FAST>(a, view, ch, sink, ovf);
+#pragma unroll
+            for (int m = 0; m < M; m++) out[m] = (float)(sink.acc[m] * a.out_scale);
+        }
+        if (ovf) atomicOr(a.err, 1);
+    }
+    float4* dst = reinterpret_cast<float4*>(a.L) + (size_t)ray * 2;
+    dst[0] = make_float4(out[0], out[1], out[2], out[3]);
+    dst[1] = make_float4(out[4], out[5], out[6], out[7]);
+    if (a.counts) a.counts[ray] = iks;
+}
+
+template <int M, bool FAST>
+__global__ __launch_bounds__(256) void k_trace_adj(TraceArgs a)
+{
+    const int local = blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= a.n_rays) return;
+    const int ray = a.ray_offset + local;
+    const int view = ray / a.W;
+    const int ch = ray - view * a.W;
+    if (view >= a.VT) return;
+    SinkAdj<M> sink;
+    sink.grad = a.grad;
+    sink.dens = reinterpret_cast<const float4*>(a.dens);
+    const float gscale = (a.g ? a.g[ray] : 1.f) * (float)a.out_scale;
+    const float4* wv = reinterpret_cast<const float4*>(a.wts) + (size_t)ray * 2;
+    const float4 w0 = wv[0], w1 = wv[1];
+    const float wa[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    bool any = false;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        sink.w[m] = wa[m] * gscale;
+        any = any || (sink.w[m] != 0.f);
+    }
+    if (!any) return;
+    bool ovf = false;
+    march_ray<FAST>(a, view, ch, sink, ovf);
+    if (ovf) atomicOr(a.err, 1);
+}
+
+// Polyenergetic sum: one spectrum per blockIdx.y; tables in LDS as
+// mu2[e][8] = -mu_m(E_e) * log2(e) and pdf[e].
+template <int M, bool DI>
+__global__ __launch_bounds__(256) void k_spectral(SpectralArgs a)
+{
+    extern __shared__ float lds[];
+    float* mu2 = lds;
+    float* pdf = lds + a.NE * 8;
+    __shared__ uint32_t red[2][4];
+    const int s = blockIdx.y;
+    const int NE = a.NE;
+    const float kLog2e = 1.4426950408889634f;
+    for (int i = threadIdx.x; i < NE * 8; i += blockDim.x) {
+        const int e = i >> 3, m = i & 7;
+        mu2[i] = (m < M) ? -a.att[((size_t)s * M + m) * NE + e] * kLog2e : 0.f;
+    }
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) pdf[e] = a.pdf[(size_t)s * NE + e];
+    __syncthreads();
+
+    const int nr = a.V * a.W;
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = ray < nr;
+    float I = 0.f;
+    float d[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) d[m] = 0.f;
+    if (valid && ray / a.W < a.VT) {
+        const float4* Lv = reinterpret_cast<const float4*>(a.L) + (size_t)ray * 2;
+        const float4 l0 = Lv[0], l1 = Lv[1];
+        const float Lm[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+        for (int e = 0; e < NE; e++) {
+            const float* mu = mu2 + e * 8;
+            float x = 0.f;
+#pragma unroll
+            for (int m = 0; m < M; m++) x = fmaf(mu[m], Lm[m], x);
+            const float t = pdf[e] * exp2f(x);
+            I += t;
+            if (DI) {
+#pragma unroll
+                for (int m = 0; m < M; m++) d[m] = fmaf(t, mu[m], d[m]);
+            }
+        }
+    }
+    if (valid) {
+        a.raw[(size_t)s * nr + ray] = I;
+        if (DI) {
+            const float kLn2 = 0.69314718055994531f;
+            float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int m = 0; m < M; m++) o[m] = d[m] * kLn2;
+            float4* dst = reinterpret_cast<float4*>(a.dIdL) + ((size_t)s * nr + ray) * 2;
+            dst[0] = make_float4(o[0], o[1], o[2], o[3]);
+            dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+        }
+    }
+    // block max / min of I (non-negative floats order like their bits)
+    uint32_t mx = valid ? __float_as_uint(I) : 0u;
+    uint32_t mn = valid ? __float_as_uint(I) : 0x7f800000u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, 64));
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[0][wave] = mx; red[1][wave] = mn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 1; w < nw; w++) { mx = max(mx, red[0][w]); mn = min(mn, red[1][w]); }
+        atomicMax(a.stats + 2 * s, mx);
+        atomicMin(a.stats + 2 * s + 1, mn);
+    }
+}
+
+__global__ void k_stats_init(uint32_t* stats, int S)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S) {
+        stats[2 * i] = 0u;
+        stats[2 * i + 1] = 0x7f800000u;
+    }
+}
+
+// Order-preserving uint image of a float (for atomic min/max of any sign).
+__device__ __forceinline__ uint32_t ord_of(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float float_of_ord(uint32_t o)
+{
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+__global__ void k_minmax_init(uint32_t* st)
+{
+    st[0] = 0u;           // max, ordered
+    st[1] = 0xffffffffu;  // min, ordered
+}
+
+__global__ __launch_bounds__(256) void k_minmax(const float* __restrict__ x, int n, uint32_t* st)
+{
+    __shared__ uint32_t red[2][4];
+    uint32_t mx = 0u, mn = 0xffffffffu;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t o = ord_of(x[i]);
+        mx = max(mx, o);
+        mn = min(mn, o);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, 64));
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[0][wave] = mx; red[1][wave] = mn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) { mx = max(mx, red[0][w]); mn = min(mn, red[1][w]); }
+        atomicMax(st, mx);
+        atomicMin(st + 1, mn);
+    }
+}
+
+// Converts the ordered images back to float bit patterns {max, min}.
+__global__ void k_minmax_final(uint32_t* st)
+{
+    st[0] = __float_as_uint(float_of_ord(st[0]));
+    st[1] = __float_as_uint(float_of_ord(st[1]));
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    }
+    return c;
+}
+
+__global__ __launch_bounds__(256) void k_detector(DetectorArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n * a.S) return;
+    const int s = i / a.n;
+    const int r = i - s * a.n;
+    const float mx = __uint_as_float(a.stats[2 * s]);
+    const float p = a.raw[i] / mx;  // M/main.cpp:521
+    float o = p;
+    if (a.sigma != 0.0) {
+        // M/main.cpp:63-74 with Philox uniforms in (0,1) in place of rand()
+        const uint4 u = philox4x32_10(make_uint4((uint32_t)r, a.stream_id + (uint32_t)s, 0u, 0u), a.key0, a.key1);
+        const double u1 = ((double)u.x + 1.0) / 4294967297.0;
+        const double u2 = ((double)u.y + 1.0) / 4294967297.0;
+        const double z = sqrt(-2.0 * log(u1)) * sin(2.0 * 3.141592653589793238462643383279 * u2);
+        const double nstar = (double)p + (0 + a.sigma * z);
+        o = (float)nstar;
+        if (nstar < 0) {
+            const float pmin = __uint_as_float(a.stats[2 * s + 1]) / mx;
+            o = (float)((double)pmin / 10);
+        }
+        if (nstar > 1.0) o = 1.0f;
+    }
+    a.out[i] = o;
+}
+
+__global__ __launch_bounds__(256) void k_pack_i16(const int16_t* __restrict__ src, int M, int NN,
+                                                  uint16_t* __restrict__ dst)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NN) return;
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    for (int m = 0; m < M; m++) {
+        const int x = src[(size_t)m * NN + v];
+        const uint32_t u = x > 0 ? (uint32_t)x : 0u;
+        w[m >> 1] |= (m & 1) ? (u << 16) : u;
+    }
+    reinterpret_cast<uint4*>(dst)[v] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__global__ __launch_bounds__(256) void k_pack_f32(const float* __restrict__ src, int M, int NN,
+                                                  float* __restrict__ dst)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= NN) return;
+    float r[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < M; m++) r[m] = src[(size_t)m * NN + v];
+    float4* d = reinterpret_cast<float4*>(dst) + (size_t)v * 2;
+    d[0] = make_float4(r[0], r[1], r[2], r[3]);
+    d[1] = make_float4(r[4], r[5], r[6], r[7]);
+}
+
+// --------------------------------------------------------------- launchers
+
+template <int M>
+static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint, hipStream_t s)
+{
+    const dim3 block(256);
+    const dim3 grid((a.n_rays + 255) / 256);
+    if (adjoint) {
+        hipLaunchKernelGGL((k_trace_adj<M, true>), grid, block, 0, s, a);
+    } else if (dens_type == VCT_DENS_U16) {
+        hipLaunchKernelGGL((k_trace_fwd<M, VCT_DENS_U16, true>), grid, block, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_trace_fwd<M, VCT_DENS_F32, true>), grid, block, 0, s, a);
+    }
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status launch_trace(const TraceArgs& a, int dens_type, int M, bool adjoint, hipStream_t s)
+{
+    if (a.n_rays <= 0) return VCT_OK;
+    switch (M) {
+    case 1: return launch_trace_m<1>(a, dens_type, adjoint, s);
+    case 2: return launch_trace_m<2>(a, dens_type, adjoint, s);
+    case 3: return launch_trace_m<3>(a, dens_type, adjoint, s);
+    case 4: return launch_trace_m<4>(a, dens_type, adjoint, s);
+    case 5: return launch_trace_m<5>(a, dens_type, adjoint, s);
+    case 6: return launch_trace_m<6>(a, dens_type, adjoint, s);
+    case 7: return launch_trace_m<7>(a, dens_type, adjoint, s);
+    case 8: return launch_trace_m<8>(a, dens_type, adjoint, s);
+    default: return set_error(VCT_ERR_ARG, "number of materials %d outside 1..8", M);
+    }
+}
+
+template <int M>
+static void launch_spectral_m(const SpectralArgs& a, hipStream_t s)
+{
+    const int nr = a.V * a.W;
+    const dim3 grid((nr + 255) / 256, a.S);
+    const size_t lds = sizeof(float) * (size_t)a.NE * 9;
+    if (a.dIdL)
+        hipLaunchKernelGGL((k_spectral<M, true>), grid, dim3(256), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_spectral<M, false>), grid, dim3(256), lds, s, a);
+}
+
+vct_status launch_spectral(const SpectralArgs& a, hipStream_t s)
+{
+    if (a.NE <= 0 || a.NE > kMaxBins) return set_error(VCT_ERR_ARG, "NE=%d outside 1..%d", a.NE, kMaxBins);
+    if (a.S <= 0 || a.S > 65535) return set_error(VCT_ERR_ARG, "n_spectra=%d invalid", a.S);
+    switch (a.M) {
+    case 1: launch_spectral_m<1>(a, s); break;
+    case 2: launch_spectral_m<2>(a, s); break;
+    case 3: launch_spectral_m<3>(a, s); break;
+    case 4: launch_spectral_m<4>(a, s); break;
+    case 5: launch_spectral_m<5>(a, s); break;
+    case 6: launch_spectral_m<6>(a, s); break;
+    case 7: launch_spectral_m<7>(a, s); break;
+    case 8: launch_spectral_m<8>(a, s); break;
+    default: return set_error(VCT_ERR_ARG, "number of materials %d outside 1..8", a.M);
+    }
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status launch_stats_init(uint32_t* stats, int S, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stats_init, dim3((S + 63) / 64), dim3(64), 0, s, stats, S);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status launch_minmax(const float* x, int n, uint32_t* stats, hipStream_t s)
+{
+    if (n <= 0) return set_error(VCT_ERR_ARG, "empty array");
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, s, stats);
+    const int blocks = min((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_minmax, dim3(blocks), dim3(256), 0, s, x, n, stats);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(1), 0, s, stats);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status launch_detector(const DetectorArgs& a, hipStream_t s)
+{
+    const long long total = (long long)a.n * a.S;
+    if (total <= 0 || total > INT_MAX) return set_error(VCT_ERR_ARG, "detector size %lld invalid", total);
+    hipLaunchKernelGGL(k_detector, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status launch_pack_i16(const int16_t* src, int M, int N, uint16_t* dst, hipStream_t s)
+{
+    const int NN = N * N;
+    hipLaunchKernelGGL(k_pack_i16, dim3((NN + 255) / 256), dim3(256), 0, s, src, M, NN, dst);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status launch_pack_f32(const float* src, int M, int N, float* dst, hipStream_t s)
+{
+    const int NN = N * N;
+    hipLaunchKernelGGL(k_pack_f32, dim3((NN + 255) / 256), dim3(256), 0, s, src, M, NN, dst);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+}  // namespace vct
