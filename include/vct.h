@@ -198,8 +198,10 @@ vct_status vct_trace_adjoint(vct_plan* plan, const void* d_packed, int dens_type
 vct_status vct_reprojection(vct_ctx* ctx, const vct_geometry* g, const float* dens, int M,
                             const float* att, const float* pdf, int NE, float* sino_full);
 /* Replaces the normalisation and gaussiannoise0 (M/main.cpp:511-525),
- * in place on a host sinogram of n floats. */
-vct_status vct_gaussian_noise(vct_ctx* ctx, float* sino, int n, double sigma, uint64_t seed);
+ * in place on a host sinogram of n floats.  stats (optional, float[2])
+ * receives {max, min} of the input (projection_min = min / max). */
+vct_status vct_gaussian_noise(vct_ctx* ctx, float* sino, int n, double sigma, uint64_t seed,
+                              float* stats);
 /* Replaces filterSinogram (F/filterSinogramTOMO.cpp:205); host [V][W]. */
 vct_status vct_filter_sinogram_host(vct_ctx* ctx, const vct_geometry* g, const float* proj,
                                     int filter, int in_mode, float* fltd);

@@ -75,6 +75,10 @@ def main():
                     os.path.join(HERE, "spectrum_120kV.txt"))
     shutil.copyfile(os.path.join(M_, "Spectrum", "PhaseSpace_photon_6MV_0cm.txt"),
                     os.path.join(HERE, "spectrum_6MV.txt"))
+    for src, dst in ((os.path.join(M_, "IR_TOMO_input_virtual_projection_120kV.txt"), "input_mbfpa_120kV.txt"),
+                     (os.path.join(M_, "IR_TOMO_input_virtual_projection_6MV.txt"), "input_mbfpa_6MV.txt"),
+                     (os.path.join(F_, "TOMO_input_virtual_projection_human.txt"), "input_fbp_human.txt")):
+        shutil.copyfile(src, os.path.join(HERE, dst))
     kf = np.fromfile(os.path.join(F_, "reprojection_float.raw"), np.float32)
     xz_write(os.path.join(HERE, "kat_f_input.f32.xz"), kf.tobytes())
     hashes = {

@@ -73,7 +73,7 @@ _SIGS = {
     "vct_backproject": (C.c_int, [_P, _P, C.c_int, _P, _P]),
     "vct_trace_adjoint": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
     "vct_reprojection": (C.c_int, [_P, C.POINTER(Geometry), _P, C.c_int, _P, _P, C.c_int, _P]),
-    "vct_gaussian_noise": (C.c_int, [_P, _P, C.c_int, C.c_double, C.c_uint64]),
+    "vct_gaussian_noise": (C.c_int, [_P, _P, C.c_int, C.c_double, C.c_uint64, _P]),
     "vct_filter_sinogram_host": (C.c_int, [_P, C.POINTER(Geometry), _P, C.c_int, C.c_int, _P]),
     "vct_reconstruct_image_host": (C.c_int, [_P, C.POINTER(Geometry), _P, _P]),
 }

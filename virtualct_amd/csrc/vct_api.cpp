@@ -633,7 +633,7 @@ vct_status vct_reprojection(vct_ctx* ctx, const vct_geometry* g, const float* de
     return st;
 }
 
-vct_status vct_gaussian_noise(vct_ctx* ctx, float* sino, int n, double sigma, uint64_t seed)
+vct_status vct_gaussian_noise(vct_ctx* ctx, float* sino, int n, double sigma, uint64_t seed, float* stats_out)
 {
     // M/main.cpp:511-525: max-normalise then add noise, on the device.
     clear_error();
@@ -667,6 +667,8 @@ vct_status vct_gaussian_noise(vct_ctx* ctx, float* sino, int n, double sigma, ui
         st = launch_detector(a, s);
     }
     if (st == VCT_OK) hip(hipMemcpyAsync(sino, d_out, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s), "copy");
+    if (st == VCT_OK && stats_out)
+        hip(hipMemcpyAsync(stats_out, d_stats, 2 * sizeof(float), hipMemcpyDeviceToHost, s), "copy");
     if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
     void* bufs[] = {d_in, d_out, d_stats};
     for (void* b : bufs)
