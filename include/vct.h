@@ -102,6 +102,13 @@ vct_status vct_terminate(vct_ctx* ctx);
 vct_status vct_plan_create(vct_ctx* ctx, const vct_geometry* g, vct_plan** plan);
 vct_status vct_plan_destroy(vct_plan* plan);
 
+/* Plan options (no counterpart in the reference).
+ * VCT_OPT_EXACT_TRACE = 1: march every ray with correctly rounded divisions
+ * (the reference's operation sequence) instead of the default guarded
+ * reciprocal march; both give the same segments, the default is faster. */
+enum { VCT_OPT_EXACT_TRACE = 1 };
+vct_status vct_plan_set_option(vct_plan* plan, int option, int value);
+
 /* ------------------------------------------------------- host-side tables */
 
 /* Replaces attenuation_coefficient (M/prior_weight_production.cpp:180-297):
@@ -189,6 +196,56 @@ vct_status vct_backproject(vct_plan* plan, const float* d_fltd, int n_sino, floa
 vct_status vct_trace_adjoint(vct_plan* plan, const void* d_packed, int dens_type, int M,
                              const float* d_dIdL, const float* d_g, float* d_grad,
                              void* stream);
+
+/* ------------------------------------------------- batched execution */
+
+/* Cached segment lists of one geometry (the "system matrix").  The reference
+ * caches the same (voxel, length) pairs per segment counter during its trace
+ * (reprojection_systemMatrixfloat / reprojection_vnum / reprojection_sum,
+ * M/reprojection_TOMO.cpp:160-163, :186-188, :195) and replays them in
+ * reprojection_TOMO_2 (:291-352); here they are built once on the device by
+ * the exact tracer, stored ray-major (CSR), and with transpose != 0 also
+ * voxel-major (CSC, rays in increasing order within each voxel). */
+typedef struct vct_sysmat vct_sysmat;
+vct_status vct_sysmat_build(vct_plan* plan, int transpose, vct_sysmat** out, void* stream);
+vct_status vct_sysmat_destroy(vct_sysmat* m);
+int64_t vct_sysmat_segments(const vct_sysmat* m);
+size_t vct_sysmat_bytes(const vct_sysmat* m);
+
+/* One group of phantoms sharing a system matrix.  Batched arrays are
+ * phantom-interleaved: densities [N*N][B][8], path integrals and adjoint
+ * weights [rays][B][8], gradients [N*N][B][8] (float32). */
+typedef struct vct_batch_group {
+    const vct_sysmat* sysmat;
+    const float* d_in;   /* paths: densities; adjoint: weights            */
+    float* d_out;        /* paths: path integrals; adjoint: gradient (overwritten) */
+    int32_t n_phantoms;  /* B <= lanes_per_row                           */
+    int32_t reserved;
+} vct_batch_group;
+
+/* Batched replay of the cached segments, replacing reprojection_TOMO_2
+ * (M/reprojection_TOMO.cpp:291-352) for many phantoms and geometries in one
+ * persistent launch (up to 16 groups; lanes_per_row in {8,16,32,64}; blocks
+ * <= 0 picks the persistent grid).  out[ray][b][m] = sum_seg in[vox][b][m]*len. */
+vct_status vct_batch_paths(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row,
+                           int blocks, void* stream);
+/* Batched transpose (no counterpart in the reference, whose iterative loop
+ * backprojects with lengthproj_TOMO_2, M/lengthproj_TOMO.cpp:188-276):
+ * out[vox][b][m] = sum_{rays through vox} in[ray][b][m] * len, atomic-free. */
+vct_status vct_batch_adjoint(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row,
+                             int blocks, void* stream);
+/* Polyenergetic sum for B phantoms x S spectra (as vct_spectral_sum):
+ * L [rays][B][8] -> raw [B][S][rays], stats [B][S][2]. */
+vct_status vct_batch_spectral(vct_plan* plan, const float* d_L, int B, int M, const float* d_att,
+                              const float* d_pdf, int NE, int S, float* d_raw, uint32_t* d_stats,
+                              void* stream);
+/* Dual-energy decomposition loss (no counterpart in the reference): for
+ * targets t [B][S][rays] and normalisation I_max,s = sum_E pdf_s,
+ *   loss[b] = sum_s sum_rays (ln(I/I_max) - ln t)^2,
+ *   w[ray][b][m] = d loss / d L_m  (input of vct_batch_adjoint). */
+vct_status vct_batch_loss(vct_plan* plan, const float* d_L, int B, int M, const float* d_att,
+                          const float* d_pdf, int NE, int S, const float* d_target, float* d_w,
+                          float* d_loss, void* stream);
 
 /* ------------------------------------------------ host-buffer drop-ins */
 

@@ -1,0 +1,168 @@
+"""Batched execution over cached system matrices against the plain
+single-phantom path (which the other GPU tests pin to the reference)."""
+from __future__ import annotations
+
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+
+def _mods():
+    from virtualct_amd import batch, decompose, ops
+    return ops, batch, decompose
+
+
+def _geom(N=512, pix=0.1, V=800):
+    from virtualct_amd import default_geometry
+    return default_geometry(recon_size=N, pixel_cm=pix, n_views=V)
+
+
+def _spectra(dev, names=("120kV", "6MV"), M=6):
+    import torch
+    from virtualct_amd import spectra
+    with tempfile.TemporaryDirectory() as d:
+        tabs = [spectra.attenuation(spectra.synthetic_spectrum_file(n, d), M) for n in names]
+    NE = max(a.shape[1] for a, _, _ in tabs)
+    att = torch.zeros((len(tabs), M, NE))
+    pdf = torch.zeros((len(tabs), NE))
+    for s, (a, p, _) in enumerate(tabs):
+        att[s, :, : a.shape[1]] = torch.from_numpy(a)
+        pdf[s, : p.shape[0]] = torch.from_numpy(p)
+    return att.to(dev), pdf.to(dev)
+
+
+def _phantoms(B, N, pix, dev, seed=3000):
+    import torch
+    from virtualct_amd import phantoms
+    return torch.stack([torch.from_numpy(phantoms.densities_f32(phantoms.body_phantom(seed + b, N=N, pix_cm=pix)))
+                        for b in range(B)]).to(dev)
+
+
+@pytest.fixture(scope="module")
+def small_setup(gpu):
+    ops, batch, _ = _mods()
+    plan = ops.plan_for(_geom(128, 0.35, 160))
+    sm = batch.SystemMatrix(plan, transpose=True)
+    return plan, sm
+
+
+def test_csr_replay_equals_tracer(gpu, small_setup):
+    ops, batch, _ = _mods()
+    plan, sm = small_setup
+    x = _phantoms(3, 128, 0.35, gpu)
+    L = torch_empty = None
+    import torch
+    L = torch.empty((plan.V * plan.W, 3, 8), device=gpu)
+    batch.batch_paths([(sm, batch.interleave(x), L, 3)], 6)
+    for b in range(3):
+        pk, dt = ops.pack_densities(plan, x[b].contiguous())
+        ref = ops.trace_paths(plan, pk, dt, 6).reshape(-1, 8)
+        assert rel_max(L[:, b].cpu().numpy(), ref.cpu().numpy()) < 2e-6
+
+
+def test_csc_adjoint_equals_atomic_adjoint(gpu, small_setup):
+    import torch
+    ops, batch, _ = _mods()
+    plan, sm = small_setup
+    g = torch.Generator(device=gpu).manual_seed(5)
+    w = torch.zeros((plan.V * plan.W, 2, 8), device=gpu)
+    w[..., :6] = torch.randn((plan.V * plan.W, 2, 6), generator=g, device=gpu)
+    out = torch.empty((128 * 128, 2, 8), device=gpu)
+    batch.batch_adjoint([(sm, w, out, 2)], 6)
+    for b in range(2):
+        ref = ops.trace_adjoint(plan, w[:, b].contiguous().view(plan.V, plan.W, 8), 6).reshape(-1, 8)
+        assert rel_l2(out[:, b].cpu().numpy(), ref.cpu().numpy()) < 1e-5
+    # deterministic: a second launch gives identical bits
+    out2 = torch.empty_like(out)
+    batch.batch_adjoint([(sm, w, out2, 2)], 6)
+    assert torch.equal(out, out2)
+
+
+def test_batch_loss_matches_plain_ops(gpu, small_setup):
+    import torch
+    ops, batch, decompose = _mods()
+    plan, sm = small_setup
+    att, pdf = _spectra(gpu)
+    x = _phantoms(2, 128, 0.35, gpu)
+    tgt = torch.stack([ops.mbfpa_forward(plan, (x[b] * 0.97).contiguous(), att, pdf) for b in range(2)])
+    L = torch.empty((plan.V * plan.W, 2, 8), device=gpu)
+    batch.batch_paths([(sm, batch.interleave(x), L, 2)], 6)
+    w, loss = batch.batch_loss(plan, L, 6, att, pdf, tgt)
+    for b in range(2):
+        Lb = L[:, b].contiguous().view(plan.V, plan.W, 8)
+        raw, stats, dIdL = ops.spectral_sum(plan, Lb, 6, att, pdf, want_dIdL=True)
+        imax = pdf.sum(1).view(-1, 1, 1)
+        r, dI = decompose.log_residual(raw, imax, tgt[b])
+        assert loss[b].item() == pytest.approx(r.square().sum().item(), rel=1e-4)
+        wref = (dIdL * dI.unsqueeze(-1)).sum(0).reshape(-1, 8)
+        assert rel_l2(w[:, b].cpu().numpy(), wref.cpu().numpy()) < 1e-4
+
+
+def test_batched_solver_matches_plain_loop_and_graph(gpu, small_setup):
+    import torch
+    ops, batch, decompose = _mods()
+    plan, sm = small_setup
+    att, pdf = _spectra(gpu)
+    truth = _phantoms(2, 128, 0.35, gpu, seed=3100)
+    tgt = torch.stack([ops.mbfpa_forward(plan, truth[b].contiguous(), att, pdf, sigma=5.37e-5, stream_id=9 * b)
+                       for b in range(2)])
+    init = truth * 0.9
+    plain = decompose.PlainLoop(plan, tgt, att, pdf, init)
+    solver = batch.BatchedSolver(sm, tgt, att, pdf, init)
+    for it in range(3):
+        lp = plain.step()
+        lb = solver.step().clone()
+        assert rel_max(lb.cpu().numpy(), lp.cpu().numpy()) < 1e-3, it
+    assert rel_l2(solver.densities().cpu().numpy(), plain.x.cpu().numpy()) < 1e-3
+    # graph: replays interleaved with eager work must equal the eager solver
+    eager = batch.BatchedSolver(sm, tgt, att, pdf, init)
+    graphed = batch.BatchedSolver(sm, tgt, att, pdf, init)
+    eager.step()
+    graphed.capture()  # performs the first step itself
+    for it in range(4):
+        le = eager.step().clone()
+        lg = graphed.step().clone()
+        torch.cuda.synchronize()
+        _ = (truth * 2).sum().item()  # unrelated eager work between replays
+        assert torch.allclose(lg, le, rtol=1e-5, atol=0), it
+        assert rel_l2(graphed.x.cpu().numpy(), eager.x.cpu().numpy()) < 1e-5, it
+    # the loss falls
+    assert graphed.loss.sum().item() < lp.sum().item() * 5
+
+
+def test_batch_forward_fbp_uneven_sizes(gpu):
+    import torch
+    ops, batch, _ = _mods()
+    att, pdf = _spectra(gpu, ("80kV", "120kV", "6MV"))
+    sizes = [(64, 50.0 / 64), (96, 50.0 / 96)]
+    groups = []
+    for k, (N, pix) in enumerate(sizes):
+        from virtualct_amd import phantoms
+        ph = torch.stack([torch.from_numpy(phantoms.body_phantom(4000 + 10 * k + b, N=N, pix_cm=pix))
+                          for b in range(2)]).to(gpu)
+        groups.append((_geom(N, pix, 160), ph))
+    run = batch.BatchForwardFBP(groups, att, pdf, sigma=5.37e-5, seed=77)
+    sinos, imgs = run.run()
+    sid = 0
+    for k, (geom, ph) in enumerate(groups):
+        plan = ops.plan_for(geom)
+        for b in range(2):
+            ref = ops.mbfpa_forward(plan, ph[b].contiguous(), att, pdf, sigma=5.37e-5, seed=77, stream_id=sid)
+            sid += 3
+            got = sinos[k][3 * b: 3 * b + 3]
+            assert rel_max(got.cpu().numpy(), ref.cpu().numpy()) < 1e-5
+            img = ops.fbp(plan, ref)
+            assert rel_max(imgs[k][3 * b: 3 * b + 3].cpu().numpy(), img.cpu().numpy()) < 1e-4
+    # graph replays reproduce the eager run
+    eager_imgs = [i.clone() for i in imgs]
+    run.capture()
+    for _ in range(3):
+        s2, i2 = run.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(i2, eager_imgs):
+            assert torch.allclose(a, b, rtol=1e-5, atol=1e-7)

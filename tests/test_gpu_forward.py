@@ -234,3 +234,21 @@ def test_argument_errors_raise(gpu):
     with pytest.raises(VctError):
         ops.detector(plan, torch.zeros((1, 800, 609), device=gpu), torch.zeros((1, 2), dtype=torch.int32,
                                                                                   device=gpu), -1.0)
+
+
+def test_fast_march_equals_exact_march(gpu, golden):
+    """The guarded reciprocal march gives the exact march's segments on every
+    ray of the full geometry and of a 768^2 / 0.0651 cm geometry."""
+    import ctypes
+    ops = _ops()
+    from virtualct_amd import _lib, phantoms
+    for N, pix, ph in ((512, 0.1, golden.icru110()),
+                       (768, 50.0 / 768, phantoms.body_phantom(9, N=768, pix_cm=50.0 / 768))):
+        plan = ops.Plan(_geom(N, pix))
+        pk, dt = ops.pack_densities(plan, _t(ph, gpu))
+        L_fast, c_fast = ops.trace_paths(plan, pk, dt, 6, counts=True)
+        _lib.call("vct_plan_set_option", plan.handle, 1, 1)
+        L_exact, c_exact = ops.trace_paths(plan, pk, dt, 6, counts=True)
+        assert np.array_equal(c_fast.cpu().numpy(), c_exact.cpu().numpy())
+        assert rel_max(L_fast.cpu().numpy(), L_exact.cpu().numpy()) < 1e-6
+        plan.close()

@@ -58,6 +58,7 @@ _SIGS = {
     "vct_terminate": (C.c_int, [_P]),
     "vct_plan_create": (C.c_int, [_P, C.POINTER(Geometry), C.POINTER(_P)]),
     "vct_plan_destroy": (C.c_int, [_P]),
+    "vct_plan_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
     "vct_attenuation_coefficient": (C.c_int, [C.c_char_p, C.c_int, _P, _P, C.c_int, _P, _P,
                                                C.POINTER(C.c_int), _P]),
     "vct_const_map": (C.c_int, [C.POINTER(Geometry), C.c_int, _P]),
@@ -77,15 +78,26 @@ _SIGS = {
     "vct_filter_sinogram_host": (C.c_int, [_P, C.POINTER(Geometry), _P, C.c_int, C.c_int, _P]),
     "vct_reconstruct_image_host": (C.c_int, [_P, C.POINTER(Geometry), _P, _P]),
 }
-# Optional entry points (present once their translation unit is built).
-_OPTIONAL = {
-    "vct_batch_create": (C.c_int, [_P, C.c_int, _P, _P, _P, C.POINTER(_P)]),
-    "vct_batch_destroy": (C.c_int, [_P]),
-    "vct_batch_forward": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, C.c_int, C.c_double,
-                                    C.c_uint64, _P, _P, _P]),
-    "vct_batch_adjoint": (C.c_int, [_P, _P, _P, _P, _P]),
-    "vct_batch_workspace_bytes": (C.c_size_t, [_P, C.c_int]),
-}
+
+
+class BatchGroup(C.Structure):
+    """Mirror of ``vct_batch_group``."""
+
+    _fields_ = [("sysmat", C.c_void_p), ("d_in", C.c_void_p), ("d_out", C.c_void_p),
+                ("n_phantoms", C.c_int32), ("reserved", C.c_int32)]
+
+
+_SIGS.update({
+    "vct_sysmat_build": (C.c_int, [_P, C.c_int, C.POINTER(_P), _P]),
+    "vct_sysmat_destroy": (C.c_int, [_P]),
+    "vct_sysmat_segments": (C.c_int64, [_P]),
+    "vct_sysmat_bytes": (C.c_size_t, [_P]),
+    "vct_batch_paths": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "vct_batch_adjoint": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "vct_batch_spectral": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P]),
+    "vct_batch_loss": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
+})
+_OPTIONAL = {}
 
 
 def lib() -> C.CDLL:

@@ -1,0 +1,313 @@
+"""Batched execution: many phantoms (and geometries) x spectra per launch.
+
+* ``SystemMatrix`` -- the cached segment lists of one geometry
+  (``vct_sysmat_build``), ray-major and optionally voxel-major.
+* ``interleave`` / ``deinterleave`` -- [B,M,N,N] <-> [N*N,B,8] (the
+  phantom-interleaved layout of the batched kernels).
+* ``BatchedSolver`` -- projected-gradient dual-energy decomposition of B
+  phantoms: one gather launch for the forward path integrals of all
+  phantoms, one fused loss/weights launch for all spectra, one atomic-free
+  gather launch for the adjoint, then the Barzilai-Borwein update; a step can
+  be captured once in a HIP graph and replayed.
+* ``BatchForwardFBP`` -- forward projection (several spectra, Gaussian
+  noise) and FBP of phantoms of several sizes: one persistent launch traces
+  every group, then the spectral sum, detector, one stacked MFMA filter GEMM
+  and the backprojections; the whole run can be captured in a HIP graph.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib, ops
+from ._lib import BatchGroup, call
+from .autotune import tuner
+
+
+def _stream(dev: torch.device) -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def _p(t: torch.Tensor | None) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr() if t is not None else 0)
+
+
+class SystemMatrix:
+    def __init__(self, plan: ops.Plan, transpose: bool = True):
+        self.plan = plan
+        self.handle = C.c_void_p()
+        dev = torch.device("cuda", plan.device)
+        with torch.cuda.device(plan.device):
+            call("vct_sysmat_build", plan.handle, int(transpose), C.byref(self.handle), _stream(dev))
+        L = _lib.lib()
+        self.segments = int(L.vct_sysmat_segments(self.handle))
+        self.bytes = int(L.vct_sysmat_bytes(self.handle))
+        self.transpose = transpose
+
+    def close(self):
+        if self.handle:
+            _lib.lib().vct_sysmat_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def interleave(dens: torch.Tensor) -> torch.Tensor:
+    """[B,M,N,N] float -> [N*N,B,8] (slots >= M zero)."""
+    B, M, N, _ = dens.shape
+    out = torch.zeros((N * N, B, 8), dtype=torch.float32, device=dens.device)
+    out[:, :, :M] = dens.permute(2, 3, 0, 1).reshape(N * N, B, M)
+    return out
+
+
+def deinterleave(x: torch.Tensor, M: int, N: int) -> torch.Tensor:
+    """[N*N,B,8] -> [B,M,N,N]."""
+    B = x.shape[1]
+    return x[:, :, :M].reshape(N, N, B, M).permute(2, 3, 0, 1).contiguous()
+
+
+def lanes_for(B: int) -> int:
+    lanes = 8
+    while lanes < B:
+        lanes *= 2
+    if lanes > 64:
+        raise ValueError("at most 64 phantoms per group")
+    return lanes
+
+
+def _groups(items):
+    arr = (BatchGroup * len(items))()
+    for k, (sm, d_in, d_out, B) in enumerate(items):
+        arr[k].sysmat = sm.handle.value
+        arr[k].d_in = d_in.data_ptr()
+        arr[k].d_out = d_out.data_ptr()
+        arr[k].n_phantoms = B
+        arr[k].reserved = 0
+    return arr
+
+
+def batch_paths(items, M: int, lanes: int | None = None, blocks: int = 0):
+    """items: [(SystemMatrix, dens [NN,B,8], out L [rays,B,8], B)] -> one launch."""
+    lanes = lanes or lanes_for(max(it[3] for it in items))
+    dev = items[0][1].device
+    call("vct_batch_paths", _groups(items), len(items), M, lanes, blocks, _stream(dev))
+
+
+def batch_adjoint(items, M: int, lanes: int | None = None, blocks: int = 0):
+    """items: [(SystemMatrix, w [rays,B,8], out grad [NN,B,8], B)] -> one launch."""
+    lanes = lanes or lanes_for(max(it[3] for it in items))
+    dev = items[0][1].device
+    call("vct_batch_adjoint", _groups(items), len(items), M, lanes, blocks, _stream(dev))
+
+
+def batch_spectral(plan: ops.Plan, L: torch.Tensor, M: int, att: torch.Tensor, pdf: torch.Tensor):
+    """L [rays,B,8] -> raw [B,S,V,W], stats [B,S,2] (float bits in int32)."""
+    B = L.shape[1]
+    S, _, NE = att.shape
+    raw = torch.empty((B, S, plan.V, plan.W), dtype=torch.float32, device=L.device)
+    stats = torch.empty((B, S, 2), dtype=torch.int32, device=L.device)
+    call("vct_batch_spectral", plan.handle, _p(L), B, M, _p(att), _p(pdf), NE, S, _p(raw), _p(stats),
+         _stream(L.device))
+    return raw, stats
+
+
+def batch_loss(plan: ops.Plan, L, M: int, att, pdf, target, w=None, loss=None):
+    """-> w [rays,B,8] (d loss / d L), loss [B]."""
+    B = L.shape[1]
+    S, _, NE = att.shape
+    w = torch.empty_like(L) if w is None else w
+    loss = torch.empty(B, dtype=torch.float32, device=L.device) if loss is None else loss
+    call("vct_batch_loss", plan.handle, _p(L), B, M, _p(att), _p(pdf), NE, S, _p(target), _p(w), _p(loss),
+         _stream(L.device))
+    return w, loss
+
+
+# ---------------------------------------------------------------- solver
+
+class BatchedSolver:
+    """Projected Barzilai-Borwein descent on B phantoms of one geometry."""
+
+    def __init__(self, sysmat: SystemMatrix, targets: torch.Tensor, att: torch.Tensor, pdf: torch.Tensor,
+                 init: torch.Tensor, alpha0: float = 1e-4, lanes: int | None = None, blocks: int = 0):
+        if not sysmat.transpose:
+            raise ValueError("the solver needs the voxel-major system matrix (transpose=True)")
+        self.sm = sysmat
+        self.plan = sysmat.plan
+        self.B, self.M = init.shape[0], init.shape[1]
+        self.N = init.shape[-1]
+        self.att = att.contiguous()
+        self.pdf = pdf.contiguous()
+        self.target = targets.contiguous()  # [B,S,V,W]
+        dev = init.device
+        self.x = interleave(init.float())
+        rays = self.plan.V * self.plan.W
+        self.L = torch.empty((rays, self.B, 8), device=dev)
+        self.w = torch.empty_like(self.L)
+        self.g = torch.empty_like(self.x)
+        self.loss = torch.empty(self.B, device=dev)
+        self.prev_x = torch.zeros_like(self.x)
+        self.prev_g = torch.zeros_like(self.x)
+        self.alpha = torch.full((self.B,), alpha0, device=dev)
+        self.have_prev = False
+        key = ("gather", self.N, self.B, self.M)
+        cfg = tuner().best(key, lambda: self._candidates(), self._time_config) if lanes is None else \
+            {"lanes": lanes, "blocks": blocks}
+        self.lanes, self.blocks = cfg["lanes"], cfg["blocks"]
+        self.graph = None
+
+    # -- autotuning of the gather launch shape
+    def _candidates(self):
+        base = lanes_for(self.B)
+        cus = torch.cuda.get_device_properties(self.x.device).multi_processor_count
+        out = []
+        for lanes in sorted({base, min(64, base * 2)}):
+            for per_cu in (2, 4, 8, 16):
+                out.append({"lanes": lanes, "blocks": cus * per_cu})
+        return out
+
+    def _time_config(self, cfg):
+        items = [(self.sm, self.x, self.L, self.B)]
+        batch_paths(items, self.M, cfg["lanes"], cfg["blocks"])
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            batch_paths(items, self.M, cfg["lanes"], cfg["blocks"])
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 3
+
+    # -- one iteration
+    def _forward_loss_grad(self):
+        batch_paths([(self.sm, self.x, self.L, self.B)], self.M, self.lanes, self.blocks)
+        batch_loss(self.plan, self.L, self.M, self.att, self.pdf, self.target, self.w, self.loss)
+        batch_adjoint([(self.sm, self.w, self.g, self.B)], self.M, self.lanes, self.blocks)
+
+    def _update(self):
+        s = self.x - self.prev_x
+        y = self.g - self.prev_g
+        sy = (s * y).sum(dim=(0, 2))
+        ss = (s * s).sum(dim=(0, 2))
+        bb = torch.where(sy > 0, ss / sy.clamp_min(1e-30), self.alpha * 2)
+        self.alpha.copy_(bb.clamp(1e-8, 1e2))
+        self.prev_x.copy_(self.x)
+        self.prev_g.copy_(self.g)
+        self.x.copy_((self.x - self.alpha.view(1, -1, 1) * self.g).clamp_min(0.0))
+
+    def step(self):
+        """One iteration; returns the per-phantom loss before the update."""
+        if self.graph is not None:
+            self.graph.replay()
+            return self.loss
+        self._forward_loss_grad()
+        if not self.have_prev:
+            self.prev_x.copy_(self.x)
+            self.prev_g.copy_(self.g)
+            self.x.copy_((self.x - self.alpha.view(1, -1, 1) * self.g).clamp_min(0.0))
+            self.have_prev = True
+        else:
+            self._update()
+        return self.loss
+
+    def capture(self):
+        """Capture the steady-state iteration in a HIP graph (after one step)."""
+        if not self.have_prev:
+            self.step()
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream(self.x.device)
+        side.wait_stream(torch.cuda.current_stream(self.x.device))
+        with torch.cuda.stream(side):  # warm the allocator outside capture
+            self._forward_loss_grad()
+            self._update()
+        torch.cuda.current_stream(self.x.device).wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._forward_loss_grad()
+            self._update()
+        self.graph = g
+        return g
+
+    def densities(self) -> torch.Tensor:
+        return deinterleave(self.x, self.M, self.N)
+
+
+# ---------------------------------------------------- batched forward + FBP
+
+@dataclass
+class Group:
+    plan: ops.Plan
+    sysmat: SystemMatrix
+    dens: torch.Tensor    # [NN,B,8]
+    L: torch.Tensor       # [rays,B,8]
+    B: int
+
+
+class BatchForwardFBP:
+    """MBFPA (S spectra, noise) + FBP for phantoms of several sizes."""
+
+    def __init__(self, groups, att: torch.Tensor, pdf: torch.Tensor, sigma: float = 0.0, seed: int = 0x5EED,
+                 filt: int = 2, lanes: int | None = None, blocks: int = 0):
+        """groups: [(Geometry, densities [B,M,N,N] int16 or float32 tensor)]."""
+        self.att = att.contiguous()
+        self.pdf = pdf.contiguous()
+        self.S = att.shape[0]
+        self.M = att.shape[1]
+        self.sigma, self.seed, self.filt = sigma, seed, filt
+        self.groups: list[Group] = []
+        for geom, dens in groups:
+            plan = ops.plan_for(geom)
+            sm = SystemMatrix(plan, transpose=False)
+            d = dens.float() / 1000.0 if dens.dtype == torch.int16 else dens.float()
+            x = interleave(d)
+            L = torch.empty((plan.V * plan.W, x.shape[1], 8), device=x.device)
+            self.groups.append(Group(plan, sm, x, L, x.shape[1]))
+        V = {g.plan.V for g in self.groups}
+        W = {g.plan.W for g in self.groups}
+        if len(V) != 1 or len(W) != 1:
+            raise ValueError("all groups must share the detector geometry (views x channels)")
+        self.lanes = lanes or lanes_for(max(g.B for g in self.groups))
+        self.blocks = blocks
+        self.graph = None
+        self.out = None
+
+    def run(self):
+        gs = self.groups
+        batch_paths([(g.sysmat, g.dens, g.L, g.B) for g in gs], self.M, self.lanes, self.blocks)
+        sinos = []
+        sid = 0
+        for g in gs:
+            raw, stats = batch_spectral(g.plan, g.L, self.M, self.att, self.pdf)
+            B, S = raw.shape[:2]
+            sino = ops.detector(g.plan, raw.view(B * S, g.plan.V, g.plan.W), stats.view(B * S, 2),
+                                self.sigma, self.seed, sid)
+            sid += B * S
+            sinos.append(sino)
+        stack = torch.cat(sinos, 0) if len(sinos) > 1 else sinos[0]
+        fltd = ops.filter_sinogram(gs[0].plan, stack, self.filt)
+        imgs = []
+        off = 0
+        for g, s in zip(gs, sinos):
+            n = s.shape[0]
+            imgs.append(ops.backproject(g.plan, fltd[off:off + n]))
+            off += n
+        self.out = (sinos, imgs)
+        return self.out
+
+    def capture(self):
+        self.run()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            out = self.run()
+        self.graph = g
+        self.out = out
+        return g
+
+    def replay(self):
+        self.graph.replay()
+        return self.out

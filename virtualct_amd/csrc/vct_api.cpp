@@ -72,6 +72,7 @@ TraceArgs make_trace_args(const vct_plan* p)
     a.max_steps = 4 * g.recon_size + (int)(4.0 / g.pixel_cm) + 64;
     a.ray_offset = 0;
     a.n_rays = g.n_views * g.n_channels;
+    a.exact = p->exact_trace;
     return a;
 }
 
@@ -307,6 +308,16 @@ vct_status vct_plan_create(vct_ctx* ctx, const vct_geometry* g, vct_plan** out)
     }
     *out = p;
     return VCT_OK;
+}
+
+vct_status vct_plan_set_option(vct_plan* p, int option, int value)
+{
+    clear_error();
+    if (!p) return set_error(VCT_ERR_ARG, "plan is NULL");
+    switch (option) {
+    case VCT_OPT_EXACT_TRACE: p->exact_trace = value ? 1 : 0; return VCT_OK;
+    default: return set_error(VCT_ERR_ARG, "unknown option %d", option);
+    }
 }
 
 vct_status vct_plan_destroy(vct_plan* p)

@@ -1,0 +1,569 @@
+// vct_batch.hip -- batched execution over cached system matrices.
+//
+// A system matrix is the exact segment decomposition of one geometry, traced
+// once with the same march as the on-the-fly tracer (vct_march.h), stored
+//   ray-major   (CSR: row = ray,   entry = (voxel, length))  for the forward,
+//   voxel-major (CSC: row = voxel, entry = (ray, length))    for the adjoint.
+// The voxel-major copy comes from a stable radix sort of the ray-major
+// entries by voxel, so each column lists its rays in increasing order and
+// the gather adjoint is deterministic and atomic-free.
+//
+// Batched data is phantom-interleaved: densities [N*N][B][8] and path
+// integrals / adjoint weights [rays][B][8], so the B lanes that share a row
+// read one contiguous B*32-byte block per entry.  One persistent kernel
+// (k_gather) serves both directions over up to kMaxGroups groups of
+// phantoms (one group per geometry), pulling work items round-robin.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "vct_internal.h"
+#include "vct_march.h"
+
+struct vct_sysmat {
+    vct_plan* plan = nullptr;
+    int64_t nseg = 0;
+    int nrays = 0;
+    int NN = 0;
+    uint32_t* row_ptr = nullptr;  // [nrays+1]
+    uint2* csr = nullptr;         // [nseg] (voxel, float bits of length)
+    uint32_t* col_ptr = nullptr;  // [NN+1]
+    uint2* csc = nullptr;         // [nseg] (ray, float bits of length)
+};
+
+namespace vct {
+
+constexpr int kMaxGroups = 16;
+
+struct CountSink {
+    int n = 0;
+    __device__ __forceinline__ void add(int, double) { n++; }
+};
+
+struct FillSink {
+    uint2* out;
+    uint32_t* keys;   // voxel of each entry (optional)
+    uint32_t* vals;   // entry index (optional)
+    uint32_t* eray;   // ray of each entry (optional)
+    uint32_t* colcnt; // entries per voxel (optional)
+    uint32_t base, ray, k;
+    __device__ __forceinline__ void add(int v, double rr)
+    {
+        const uint32_t e = base + k++;
+        out[e] = make_uint2((uint32_t)v, __float_as_uint((float)rr));
+        if (keys) {
+            keys[e] = (uint32_t)v;
+            vals[e] = e;
+            eray[e] = ray;
+            atomicAdd(colcnt + v, 1u);
+        }
+    }
+};
+
+__global__ __launch_bounds__(256) void k_sm_count(TraceArgs a, uint32_t* counts)
+{
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= a.n_rays) return;
+    const int view = ray / a.W, ch = ray - view * a.W;
+    CountSink sink;
+    if (view < a.VT) {
+        bool ovf = false;
+        march_ray<true>(a, view, ch, sink, ovf);
+        if (ovf) atomicOr(a.err, 1);
+    }
+    counts[ray] = (uint32_t)sink.n;
+}
+
+__global__ __launch_bounds__(256) void k_sm_fill(TraceArgs a, const uint32_t* row_ptr, uint2* out, uint32_t* keys,
+                                                 uint32_t* vals, uint32_t* eray, uint32_t* colcnt)
+{
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= a.n_rays) return;
+    const int view = ray / a.W, ch = ray - view * a.W;
+    if (view >= a.VT) return;
+    FillSink sink{out, keys, vals, eray, colcnt, row_ptr[ray], (uint32_t)ray, 0u};
+    bool ovf = false;
+    march_ray<true>(a, view, ch, sink, ovf);
+    if (ovf || sink.k != row_ptr[ray + 1] - row_ptr[ray]) atomicOr(a.err, 2);
+}
+
+__global__ __launch_bounds__(256) void k_csc_gather(const uint32_t* __restrict__ order, const uint32_t* __restrict__ eray,
+                                                    const uint2* __restrict__ csr, uint2* __restrict__ csc, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t e = order[i];
+    csc[i] = make_uint2(eray[e], csr[e].y);
+}
+
+// ------------------------------------------------------------ gather SpMM
+
+struct GatherGroup {
+    const uint32_t* ptr;  // [rows+1]
+    const uint2* ent;     // entries (index, float bits of value)
+    const float* in;      // [index][B][8]
+    float* out;           // [row][B][8]
+    int B;
+    int rows;
+    int item_begin;       // first work item of this group
+    int pad;
+};
+
+struct GatherArgs {
+    GatherGroup g[kMaxGroups];
+    int n_groups;
+    int n_items;
+    int rows_per_item;
+    float scale;          // applied to every output
+};
+
+// out[row][b][m] = scale * sum_{(idx,val) in row} in[idx][b][m] * val
+// LPR lanes per row (one per phantom, B <= LPR), 256/LPR rows per item.
+template <int M, int LPR>
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a)
+{
+    const int lane_row = threadIdx.x / LPR;
+    const int b = threadIdx.x % LPR;
+    for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
+        int gi = 0;
+#pragma unroll 1
+        for (int k = 1; k < a.n_groups; k++)
+            if (item >= a.g[k].item_begin) gi = k;
+        const GatherGroup& G = a.g[gi];
+        const int row = (item - G.item_begin) * a.rows_per_item + lane_row;
+        if (row >= G.rows || b >= G.B) continue;
+        const uint32_t j0 = G.ptr[row], j1 = G.ptr[row + 1];
+        double acc[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[m] = 0.0;
+        const float4* in4 = reinterpret_cast<const float4*>(G.in);
+        uint32_t j = j0;
+        for (; j + 1 < j1; j += 2) {
+            const uint2 e0 = G.ent[j], e1 = G.ent[j + 1];
+            const size_t i0 = ((size_t)e0.x * G.B + b) * 2, i1 = ((size_t)e1.x * G.B + b) * 2;
+            const float4 a0 = in4[i0], a1 = in4[i1];
+            float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
+            if (M > 4) { c0 = in4[i0 + 1]; c1 = in4[i1 + 1]; }
+            const double v0 = __uint_as_float(e0.y), v1 = __uint_as_float(e1.y);
+            const float x0[8] = {a0.x, a0.y, a0.z, a0.w, c0.x, c0.y, c0.z, c0.w};
+            const float x1[8] = {a1.x, a1.y, a1.z, a1.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                acc[m] = __builtin_fma((double)x0[m], v0, acc[m]);
+                acc[m] = __builtin_fma((double)x1[m], v1, acc[m]);
+            }
+        }
+        if (j < j1) {
+            const uint2 e0 = G.ent[j];
+            const size_t i0 = ((size_t)e0.x * G.B + b) * 2;
+            const float4 a0 = in4[i0];
+            float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (M > 4) c0 = in4[i0 + 1];
+            const double v0 = __uint_as_float(e0.y);
+            const float x0[8] = {a0.x, a0.y, a0.z, a0.w, c0.x, c0.y, c0.z, c0.w};
+#pragma unroll
+            for (int m = 0; m < M; m++) acc[m] = __builtin_fma((double)x0[m], v0, acc[m]);
+        }
+        float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int m = 0; m < M; m++) o[m] = (float)(acc[m] * a.scale);
+        float4* dst = reinterpret_cast<float4*>(G.out) + ((size_t)row * G.B + b) * 2;
+        dst[0] = make_float4(o[0], o[1], o[2], o[3]);
+        dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+    }
+}
+
+// ---------------------------------------------------- batched spectral sum
+
+struct BatchSpecArgs {
+    const float* L;       // [rays][B][8]
+    const float* att;     // [S][M][NE]
+    const float* pdf;     // [S][NE]
+    const float* target;  // [B][S][rays] (loss mode)
+    float* raw;           // [B][S][rays] (forward mode)
+    uint32_t* stats;      // [B][S][2]
+    float* w;             // [rays][B][8] (loss mode)
+    float* loss;          // [B] (loss mode)
+    int rays, W, VT, B, S, M, NE;
+};
+
+// Forward mode: raw and per-(b,s) max/min, one (b,s) per blockIdx.y/z.
+template <int M>
+__global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
+{
+    extern __shared__ float lds[];
+    float* mu2 = lds;
+    float* pdf = lds + a.NE * 8;
+    __shared__ uint32_t red[2][4];
+    const int s = blockIdx.y, b = blockIdx.z, NE = a.NE;
+    const float kLog2e = 1.4426950408889634f;
+    for (int i = threadIdx.x; i < NE * 8; i += blockDim.x) {
+        const int e = i >> 3, m = i & 7;
+        mu2[i] = (m < M) ? -a.att[((size_t)s * M + m) * NE + e] * kLog2e : 0.f;
+    }
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) pdf[e] = a.pdf[(size_t)s * NE + e];
+    __syncthreads();
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = ray < a.rays;
+    float I = 0.f;
+    if (valid && ray / a.W < a.VT) {
+        const float4* Lv = reinterpret_cast<const float4*>(a.L) + ((size_t)ray * a.B + b) * 2;
+        const float4 l0 = Lv[0], l1 = Lv[1];
+        const float Lm[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+        for (int e = 0; e < NE; e++) {
+            const float* mu = mu2 + e * 8;
+            float x = 0.f;
+#pragma unroll
+            for (int m = 0; m < M; m++) x = fmaf(mu[m], Lm[m], x);
+            I += pdf[e] * exp2f(x);
+        }
+    }
+    const size_t bs = (size_t)b * a.S + s;
+    if (valid) a.raw[bs * a.rays + ray] = I;
+    uint32_t mx = valid ? __float_as_uint(I) : 0u;
+    uint32_t mn = valid ? __float_as_uint(I) : 0x7f800000u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, 64));
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[0][wave] = mx; red[1][wave] = mn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) { mx = max(mx, red[0][w]); mn = min(mn, red[1][w]); }
+        atomicMax(a.stats + 2 * bs, mx);
+        atomicMin(a.stats + 2 * bs + 1, mn);
+    }
+}
+
+// Loss mode: per (ray, phantom) over all spectra,
+//   r_s = ln(I_s / I_max,s) - ln t_s,  loss_b += sum_s r_s^2,
+//   w[ray][b][m] = sum_s (2 r_s / I_s) dI_s/dL_m,
+// with I_max,s = sum_E pdf_s(E) (the value of every ray that misses the object).
+template <int M>
+__global__ __launch_bounds__(256) void k_batch_loss(BatchSpecArgs a, int lpr)
+{
+    extern __shared__ float lds[];
+    const int NE = a.NE, S = a.S;
+    float* mu2 = lds;                    // [S][NE][8]
+    float* pdf = lds + (size_t)S * NE * 8;  // [S][NE]
+    float* imax = pdf + (size_t)S * NE;     // [S]
+    float* lsum = imax + S;                 // [lpr]
+    const float kLog2e = 1.4426950408889634f, kLn2 = 0.69314718055994531f;
+    for (int i = threadIdx.x; i < S * NE * 8; i += blockDim.x) {
+        const int s = i / (NE * 8), r = i - s * NE * 8, e = r >> 3, m = r & 7;
+        mu2[i] = (m < M) ? -a.att[((size_t)s * M + m) * NE + e] * kLog2e : 0.f;
+    }
+    for (int i = threadIdx.x; i < S * NE; i += blockDim.x) pdf[i] = a.pdf[i];
+    for (int i = threadIdx.x; i < lpr; i += blockDim.x) lsum[i] = 0.f;
+    __syncthreads();
+    if (threadIdx.x < S) {
+        float t = 0.f;
+        for (int e = 0; e < NE; e++) t += pdf[threadIdx.x * NE + e] * exp2f(0.f);
+        imax[threadIdx.x] = t;
+    }
+    __syncthreads();
+    const int b = threadIdx.x % lpr;
+    const int ray = blockIdx.x * (blockDim.x / lpr) + threadIdx.x / lpr;
+    float wout[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float l = 0.f;
+    const bool valid = ray < a.rays && b < a.B;
+    if (valid && ray / a.W < a.VT) {
+        const float4* Lv = reinterpret_cast<const float4*>(a.L) + ((size_t)ray * a.B + b) * 2;
+        const float4 l0 = Lv[0], l1 = Lv[1];
+        const float Lm[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+        for (int s = 0; s < S; s++) {
+            float I = 0.f, d[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) d[m] = 0.f;
+            const float* mus = mu2 + (size_t)s * NE * 8;
+            const float* ps = pdf + (size_t)s * NE;
+            for (int e = 0; e < NE; e++) {
+                const float* mu = mus + e * 8;
+                float x = 0.f;
+#pragma unroll
+                for (int m = 0; m < M; m++) x = fmaf(mu[m], Lm[m], x);
+                const float t = ps[e] * exp2f(x);
+                I += t;
+#pragma unroll
+                for (int m = 0; m < M; m++) d[m] = fmaf(t, mu[m], d[m]);
+            }
+            const float tg = a.target[((size_t)b * S + s) * a.rays + ray];
+            const float r = logf(I / imax[s]) - logf(tg);
+            l += r * r;
+            const float gI = 2.f * r / I;
+#pragma unroll
+            for (int m = 0; m < M; m++) wout[m] = fmaf(gI, d[m] * kLn2, wout[m]);
+        }
+    }
+    if (valid) {
+        float4* dst = reinterpret_cast<float4*>(a.w) + ((size_t)ray * a.B + b) * 2;
+        dst[0] = make_float4(wout[0], wout[1], wout[2], wout[3]);
+        dst[1] = make_float4(wout[4], wout[5], wout[6], wout[7]);
+        atomicAdd(lsum + b, l);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < min(lpr, a.B); i += blockDim.x) atomicAdd(a.loss + i, lsum[i]);
+}
+
+__global__ void k_batch_stats_init(uint32_t* stats, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        stats[2 * i] = 0u;
+        stats[2 * i + 1] = 0x7f800000u;
+    }
+}
+
+__global__ void k_zero_f32(float* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+}  // namespace vct
+
+using namespace vct;
+
+// ---------------------------------------------------------------- host API
+
+extern "C" {
+
+vct_status vct_sysmat_build(vct_plan* p, int transpose, vct_sysmat** out, void* stream)
+{
+    clear_error();
+    if (!p || !out) return set_error(VCT_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    const vct_geometry& g = p->g;
+    const int nrays = g.n_views * g.n_channels;
+    const int NN = g.recon_size * g.recon_size;
+    vct_sysmat* m = new vct_sysmat();
+    m->plan = p;
+    m->nrays = nrays;
+    m->NN = NN;
+    uint32_t* counts = nullptr;
+    uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *eray = nullptr, *colcnt = nullptr;
+    void* tmp = nullptr;
+    vct_status st = VCT_OK;
+    auto hip = [&](hipError_t e, const char* what) {
+        if (st == VCT_OK && e != hipSuccess) st = set_error(VCT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    };
+    TraceArgs a = make_trace_args(p);
+    hip(hipMalloc(&counts, sizeof(uint32_t) * (nrays + 1)), "hipMalloc counts");
+    hip(hipMalloc(&m->row_ptr, sizeof(uint32_t) * (nrays + 1)), "hipMalloc row_ptr");
+    if (st == VCT_OK) {
+        hipLaunchKernelGGL(k_sm_count, dim3((nrays + 255) / 256), dim3(256), 0, s, a, counts);
+        hip(hipGetLastError(), "k_sm_count");
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(1), 0, s, (float*)(counts + nrays), 1);
+        size_t tb = 0;
+        hip(rocprim::exclusive_scan(nullptr, tb, counts, m->row_ptr, 0u, (size_t)nrays + 1, rocprim::plus<uint32_t>(), s),
+            "scan size");
+        hip(hipMalloc(&tmp, tb + 16), "hipMalloc scan");
+        hip(rocprim::exclusive_scan(tmp, tb, counts, m->row_ptr, 0u, (size_t)nrays + 1, rocprim::plus<uint32_t>(), s),
+            "scan");
+    }
+    uint32_t total = 0;
+    if (st == VCT_OK) hip(hipMemcpyAsync(&total, m->row_ptr + nrays, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "copy");
+    if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
+    if (tmp) { (void)hipFree(tmp); tmp = nullptr; }
+    m->nseg = total;
+    if (st == VCT_OK) hip(hipMalloc(&m->csr, sizeof(uint2) * (size_t)total + 8), "hipMalloc csr");
+    if (st == VCT_OK && transpose) {
+        hip(hipMalloc(&keys, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc keys");
+        hip(hipMalloc(&vals, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc vals");
+        hip(hipMalloc(&eray, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc eray");
+        hip(hipMalloc(&colcnt, sizeof(uint32_t) * (NN + 1)), "hipMalloc colcnt");
+        if (st == VCT_OK)
+            hipLaunchKernelGGL(k_zero_f32, dim3((NN + 256) / 256), dim3(256), 0, s, (float*)colcnt, NN + 1);
+    }
+    if (st == VCT_OK) {
+        hipLaunchKernelGGL(k_sm_fill, dim3((nrays + 255) / 256), dim3(256), 0, s, a, m->row_ptr, m->csr, keys, vals, eray,
+                           colcnt);
+        hip(hipGetLastError(), "k_sm_fill");
+    }
+    if (st == VCT_OK && transpose) {
+        hip(hipMalloc(&keys2, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc keys2");
+        hip(hipMalloc(&vals2, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc vals2");
+        unsigned end_bit = 1;
+        while ((1u << end_bit) < (unsigned)NN) end_bit++;
+        size_t tb = 0;
+        hip(rocprim::radix_sort_pairs(nullptr, tb, keys, keys2, vals, vals2, (size_t)total, 0u, end_bit, s), "sort size");
+        hip(hipMalloc(&tmp, tb + 16), "hipMalloc sort");
+        hip(rocprim::radix_sort_pairs(tmp, tb, keys, keys2, vals, vals2, (size_t)total, 0u, end_bit, s), "sort");
+        if (tmp) { (void)hipStreamSynchronize(s); (void)hipFree(tmp); tmp = nullptr; }
+        hip(hipMalloc(&m->col_ptr, sizeof(uint32_t) * (NN + 1)), "hipMalloc col_ptr");
+        tb = 0;
+        hip(rocprim::exclusive_scan(nullptr, tb, colcnt, m->col_ptr, 0u, (size_t)NN + 1, rocprim::plus<uint32_t>(), s),
+            "scan size");
+        hip(hipMalloc(&tmp, tb + 16), "hipMalloc scan");
+        hip(rocprim::exclusive_scan(tmp, tb, colcnt, m->col_ptr, 0u, (size_t)NN + 1, rocprim::plus<uint32_t>(), s), "scan");
+        hip(hipMalloc(&m->csc, sizeof(uint2) * (size_t)total + 8), "hipMalloc csc");
+        if (st == VCT_OK) {
+            hipLaunchKernelGGL(k_csc_gather, dim3((unsigned)(((int64_t)total + 255) / 256)), dim3(256), 0, s, vals2, eray,
+                               m->csr, m->csc, (int64_t)total);
+            hip(hipGetLastError(), "k_csc_gather");
+        }
+    }
+    int herr = 0;
+    if (st == VCT_OK) hip(hipMemcpyAsync(&herr, p->ctx->d_err, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+    if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
+    if (st == VCT_OK && herr) st = set_error(VCT_ERR_INTERNAL, "system matrix build reported error %d", herr);
+    void* bufs[] = {counts, keys, vals, keys2, vals2, eray, colcnt, tmp};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (st != VCT_OK) {
+        vct_sysmat_destroy(m);
+        return st;
+    }
+    *out = m;
+    return VCT_OK;
+}
+
+vct_status vct_sysmat_destroy(vct_sysmat* m)
+{
+    if (!m) return VCT_OK;
+    void* bufs[] = {m->row_ptr, m->csr, m->col_ptr, m->csc};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete m;
+    return VCT_OK;
+}
+
+int64_t vct_sysmat_segments(const vct_sysmat* m) { return m ? m->nseg : -1; }
+
+size_t vct_sysmat_bytes(const vct_sysmat* m)
+{
+    if (!m) return 0;
+    size_t b = sizeof(uint32_t) * (m->nrays + 1) + sizeof(uint2) * (size_t)m->nseg;
+    if (m->csc) b += sizeof(uint32_t) * (m->NN + 1) + sizeof(uint2) * (size_t)m->nseg;
+    return b;
+}
+
+static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int M, int lanes, int blocks,
+                                bool adjoint, void* stream)
+{
+    if (!groups || n_groups < 1 || n_groups > kMaxGroups)
+        return set_error(VCT_ERR_ARG, "n_groups %d outside 1..%d", n_groups, kMaxGroups);
+    if (M < 1 || M > 8) return set_error(VCT_ERR_ARG, "M=%d outside 1..8", M);
+    if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+        return set_error(VCT_ERR_ARG, "lanes_per_row %d not 8/16/32/64", lanes);
+    GatherArgs a{};
+    a.n_groups = n_groups;
+    a.rows_per_item = 256 / lanes;
+    a.scale = 1.0f;
+    int items = 0;
+    for (int k = 0; k < n_groups; k++) {
+        const vct_batch_group& G = groups[k];
+        if (!G.sysmat || !G.d_in || !G.d_out) return set_error(VCT_ERR_ARG, "group %d has NULL pointers", k);
+        if (G.n_phantoms < 1 || G.n_phantoms > lanes)
+            return set_error(VCT_ERR_ARG, "group %d: %d phantoms with %d lanes per row", k, G.n_phantoms, lanes);
+        const vct_sysmat* m = G.sysmat;
+        if (adjoint && !m->csc) return set_error(VCT_ERR_ARG, "group %d: system matrix built without transpose", k);
+        GatherGroup& d = a.g[k];
+        d.ptr = adjoint ? m->col_ptr : m->row_ptr;
+        d.ent = adjoint ? m->csc : m->csr;
+        d.rows = adjoint ? m->NN : m->nrays;
+        d.in = G.d_in;
+        d.out = G.d_out;
+        d.B = G.n_phantoms;
+        d.item_begin = items;
+        items += (d.rows + a.rows_per_item - 1) / a.rows_per_item;
+    }
+    a.n_items = items;
+    if (blocks <= 0) blocks = 2048;
+    blocks = blocks < items ? blocks : items;
+    hipStream_t s = (hipStream_t)stream;
+#define VCT_GATHER_CASE(MM, LL) \
+    if (M == MM && lanes == LL) hipLaunchKernelGGL((k_gather<MM, LL>), dim3(blocks), dim3(256), 0, s, a);
+#define VCT_GATHER_M(MM) VCT_GATHER_CASE(MM, 8) VCT_GATHER_CASE(MM, 16) VCT_GATHER_CASE(MM, 32) VCT_GATHER_CASE(MM, 64)
+    VCT_GATHER_M(1) VCT_GATHER_M(2) VCT_GATHER_M(3) VCT_GATHER_M(4)
+    VCT_GATHER_M(5) VCT_GATHER_M(6) VCT_GATHER_M(7) VCT_GATHER_M(8)
+#undef VCT_GATHER_M
+#undef VCT_GATHER_CASE
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status vct_batch_paths(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row, int blocks,
+                           void* stream)
+{
+    clear_error();
+    return launch_gather(groups, n_groups, M, lanes_per_row, blocks, false, stream);
+}
+
+vct_status vct_batch_adjoint(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row, int blocks,
+                             void* stream)
+{
+    clear_error();
+    return launch_gather(groups, n_groups, M, lanes_per_row, blocks, true, stream);
+}
+
+static vct_status check_spec(vct_plan* p, const float* L, int B, int M, const float* att, const float* pdf, int NE,
+                             int S)
+{
+    if (!p || !L || !att || !pdf) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (M < 1 || M > 8) return set_error(VCT_ERR_ARG, "M=%d outside 1..8", M);
+    if (NE < 1 || NE > kMaxBins) return set_error(VCT_ERR_ARG, "NE=%d outside 1..%d", NE, kMaxBins);
+    if (B < 1 || B > 65535 || S < 1 || S > 64) return set_error(VCT_ERR_ARG, "B=%d / S=%d invalid", B, S);
+    return VCT_OK;
+}
+
+vct_status vct_batch_spectral(vct_plan* p, const float* d_L, int B, int M, const float* d_att, const float* d_pdf,
+                              int NE, int S, float* d_raw, uint32_t* d_stats, void* stream)
+{
+    clear_error();
+    VCT_CHECK(check_spec(p, d_L, B, M, d_att, d_pdf, NE, S));
+    if (!d_raw || !d_stats) return set_error(VCT_ERR_ARG, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    BatchSpecArgs a{};
+    a.L = d_L; a.att = d_att; a.pdf = d_pdf; a.raw = d_raw; a.stats = d_stats;
+    a.rays = p->g.n_views * p->g.n_channels; a.W = p->g.n_channels; a.VT = p->VT;
+    a.B = B; a.S = S; a.M = M; a.NE = NE;
+    hipLaunchKernelGGL(k_batch_stats_init, dim3((B * S + 63) / 64), dim3(64), 0, s, d_stats, B * S);
+    const dim3 grid((a.rays + 255) / 256, S, B);
+    const size_t lds = sizeof(float) * (size_t)NE * 9;
+    switch (M) {
+#define VCT_SPEC_CASE(MM) \
+    case MM: hipLaunchKernelGGL((k_batch_spectral<MM>), grid, dim3(256), lds, s, a); break;
+        VCT_SPEC_CASE(1) VCT_SPEC_CASE(2) VCT_SPEC_CASE(3) VCT_SPEC_CASE(4)
+        VCT_SPEC_CASE(5) VCT_SPEC_CASE(6) VCT_SPEC_CASE(7) VCT_SPEC_CASE(8)
+#undef VCT_SPEC_CASE
+    }
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status vct_batch_loss(vct_plan* p, const float* d_L, int B, int M, const float* d_att, const float* d_pdf, int NE,
+                          int S, const float* d_target, float* d_w, float* d_loss, void* stream)
+{
+    clear_error();
+    VCT_CHECK(check_spec(p, d_L, B, M, d_att, d_pdf, NE, S));
+    if (!d_target || !d_w || !d_loss) return set_error(VCT_ERR_ARG, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    int lpr = 1;
+    while (lpr < B) lpr <<= 1;
+    if (lpr > 64) return set_error(VCT_ERR_UNSUPPORTED, "more than 64 phantoms in one loss launch");
+    const size_t lds = sizeof(float) * ((size_t)S * NE * 9 + S + lpr);
+    if (lds > 160 * 1024) return set_error(VCT_ERR_UNSUPPORTED, "spectrum tables too large for LDS");
+    BatchSpecArgs a{};
+    a.L = d_L; a.att = d_att; a.pdf = d_pdf; a.target = d_target; a.w = d_w; a.loss = d_loss;
+    a.rays = p->g.n_views * p->g.n_channels; a.W = p->g.n_channels; a.VT = p->VT;
+    a.B = B; a.S = S; a.M = M; a.NE = NE;
+    hipLaunchKernelGGL(k_zero_f32, dim3((B + 63) / 64), dim3(64), 0, s, d_loss, B);
+    const int rays_per_block = 256 / lpr;
+    const dim3 grid((a.rays + rays_per_block - 1) / rays_per_block);
+    switch (M) {
+#define VCT_LOSS_CASE(MM) \
+    case MM: hipLaunchKernelGGL((k_batch_loss<MM>), grid, dim3(256), lds, s, a, lpr); break;
+        VCT_LOSS_CASE(1) VCT_LOSS_CASE(2) VCT_LOSS_CASE(3) VCT_LOSS_CASE(4)
+        VCT_LOSS_CASE(5) VCT_LOSS_CASE(6) VCT_LOSS_CASE(7) VCT_LOSS_CASE(8)
+#undef VCT_LOSS_CASE
+    }
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+}  // extern "C"

@@ -51,6 +51,7 @@ struct TraceArgs {
     int nxm;              // (int)(N*0.5)
     int max_steps;        // hard bound on marching steps per ray
     int rcheck;           // 1 when the reference's r >= rmax test can fire
+    int exact;            // 1 forces the exact-division march
     int ray_offset;       // first ray handled by this launch
     int n_rays;           // rays handled by this launch
     double D;             // pixel pitch
@@ -131,6 +132,7 @@ struct vct_plan {
     double* d_py0 = nullptr;
     double* d_coeff = nullptr;
     float* d_bpk[3] = {nullptr, nullptr, nullptr};  // packed filter matrix per type
+    int exact_trace = 0;  // VCT_OPT_EXACT_TRACE
 };
 
 namespace vct {

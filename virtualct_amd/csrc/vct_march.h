@@ -1,0 +1,294 @@
+// vct_march.h -- the fan-ray march shared by the tracer, the adjoint and the
+// system-matrix builder (device code, included by .hip files only).
+//
+// march_ray<false> evaluates the reference's segment decomposition
+// (M/reprojection_TOMO.cpp:53-283) operation for operation in FP64 with
+// correctly rounded divisions; compiled with -ffp-contract=off it produces
+// the reference's segments bit for bit.
+//
+// march_ray<true> produces the same segments (same voxels, same count, same
+// order) with far fewer FP64 divisions.  The per-step divisions by the ray's
+// direction components are replaced by products with per-ray reciprocals,
+// which moves a computed coordinate by at most ~1.2e-13 cm (|quotient| <= 1.2,
+// |multiplier| <= 143 cm, plus two roundings of values below 2^7).  Every
+// discrete decision taken from such a coordinate -- the loop bound +-L, the
+// minor-axis cell index, the sign split at 0, the crossing test chy != 0 and
+// the midpoint voxel index -- is first compared against a margin of 1e-9 (cm,
+// or voxel units for indices, i.e. > 6000x the error bound for pixels down to
+// 10 um); inside the margin the coordinates involved are recomputed with the
+// reference's exact expression and the decision is taken on those, so every
+// decision equals the reference's.  Segment lengths become |da| * sec(ray),
+// which differs from the reference's sqrt by rounding only and never enters a
+// decision.
+#pragma once
+
+#include <climits>
+#include <cmath>
+
+#include "vct_internal.h"
+
+namespace vct {
+
+constexpr double kMarchMargin = 1e-9;
+
+// x86-64 cvttsd2si semantics (the reference's (int) casts): NaN and values
+// outside int range give INT_MIN.
+__device__ __forceinline__ int trunc_i32(double v)
+{
+    return (v > -2147483649.0 && v < 2147483648.0) ? (int)v : INT_MIN;
+}
+
+// trunc(fl(fl(h / D) + c)), the voxel index expressions of the reference
+// (M/reprojection_TOMO.cpp:130,146-147,217), for an exactly known h.  The
+// product h * (1/D) differs from the correctly rounded quotient by a few ulp,
+// so its truncation can only differ within a few ulp of an integer; such
+// values take the exact division.
+template <bool FAST>
+__device__ __forceinline__ int index_of(double h, double D, double invD, double c)
+{
+    if (FAST) {
+        const double q = h * invD + c;
+        const double t = trunc(q);
+        const double f = fabs(q - t);
+        if (f > kMarchMargin && f < 1.0 - kMarchMargin && fabs(q) < 1.0e6) return (int)t;
+    }
+    return trunc_i32(h / D + c);
+}
+
+// Index of a value known only to ~1e-13: returns false (undecided) when the
+// value is within the margin of an integer boundary.
+__device__ __forceinline__ bool index_checked(double h, double invD, double c, int& out)
+{
+    const double q = h * invD + c;
+    const double t = trunc(q);
+    const double f = fabs(q - t);
+    out = (int)t;
+    return f > kMarchMargin && f < 1.0 - kMarchMargin && fabs(q) < 1.0e6;
+}
+
+// (u/fabs(u) != v/fabs(v)) of M/reprojection_TOMO.cpp:134,221 for finite
+// operands, without the divisions: true if either is zero or NaN (0/0 is
+// NaN, which compares unequal) or the signs differ.
+__device__ __forceinline__ bool sign_split(double u, double v)
+{
+    const bool up = u > 0.0, un = u < 0.0, vp = v > 0.0, vn = v < 0.0;
+    return !(up || un) || !(vp || vn) || (up != vp);
+}
+
+// A point of the march in (major, minor) coordinates.  Plane points have an
+// exact major coordinate, crossing points an exact minor coordinate, the
+// entry point is exact; `exact` marks the other coordinate as exact too.
+struct MarchPoint {
+    double a, b;
+    int kind;  // 0 entry, 1 plane, 2 crossing
+    bool exact;
+};
+
+struct RayConst {
+    bool xmajor;
+    double ai, af, bi, bf;  // entry and exit, major/minor
+    double sa, sb;          // source, major/minor
+    double dA, dB;          // detector - source, major/minor
+    double Lb;              // half width of the image square
+};
+
+// The march from the entry point (M/reprojection_TOMO.cpp:113-283, either
+// branch) for one ray.  FAST selects reciprocal products with guarded
+// decisions (see the file comment) over exact divisions.
+template <bool FAST, class Sink>
+__device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r, Sink& sink, bool& ovf)
+{
+    const bool xmajor = r.xmajor;
+    const double ai = r.ai, af = r.af, bi = r.bi, bf = r.bf;
+    const double sa = r.sa, sb = r.sb, dA = r.dA, dB = r.dB, Lb = r.Lb;
+    const double D = a.D;
+    const double rA = FAST ? 1.0 / dA : 0.0;
+    const double rB = FAST ? 1.0 / dB : 0.0;
+    double sec = 0.0;
+    if (FAST) {
+        const double t = dB / dA;
+        sec = sqrt(1.0 + t * t);
+    }
+    // minor coordinate on a major plane and major coordinate on a minor
+    // plane: exact (:121, :129, :137) and fast forms
+    auto b_exact = [&](double a_) { return (a_ - sa) / dA * dB + sb; };
+    auto a_exact = [&](double b_) { return (b_ - sb) / dB * dA + sa; };
+    auto b_calc = [&](double a_) { return FAST ? (a_ - sa) * rA * dB + sb : b_exact(a_); };
+    auto a_calc = [&](double b_) { return FAST ? (b_ - sb) * rB * dA + sa : a_exact(b_); };
+    auto make_exact = [&](MarchPoint& p) {
+        if (p.exact) return;
+        if (p.kind == 1) p.b = b_exact(p.a);
+        else if (p.kind == 2) p.a = a_exact(p.b);
+        p.exact = true;
+    };
+
+    const double pha = (af - ai) / fabs(ai - af);
+    int ain;
+    if ((ai < 0 && (af - ai) < 0) || (ai > 0 && (af - ai) > 0))
+        ain = trunc_i32(trunc_i32(ai) + (af - ai) / fabs(af - ai));
+    else
+        ain = trunc_i32(ai);
+    const double bdir = (bf - bi) / fabs(bf - bi);
+    const bool b_pos = (bf - bi) > 0;
+    const bool b_neg = (bf - bi) < 0;
+
+    // one candidate segment p0 -> p1, :141-166 / :168-189
+    auto emit = [&](MarchPoint& p0, MarchPoint& p1) {
+        int xmc, ymc;
+        bool decided = false;
+        if (FAST) {
+            const double xb0 = xmajor ? p0.a : p0.b, yb0 = xmajor ? p0.b : p0.a;
+            const double xb1 = xmajor ? p1.a : p1.b, yb1 = xmajor ? p1.b : p1.a;
+            decided = index_checked((xb0 + xb1) * 0.5, a.invD, (double)a.nxm, xmc) &&
+                      index_checked((-yb0 - yb1) * 0.5, a.invD, (double)a.nxm, ymc);
+            if (!decided) {
+                make_exact(p0);
+                make_exact(p1);
+            }
+        }
+        const double xb0 = xmajor ? p0.a : p0.b, yb0 = xmajor ? p0.b : p0.a;
+        const double xb1 = xmajor ? p1.a : p1.b, yb1 = xmajor ? p1.b : p1.a;
+        if (!decided) {
+            xmc = index_of<FAST>((xb0 + xb1) * 0.5, D, a.invD, (double)a.nxm);
+            ymc = index_of<FAST>((-yb0 - yb1) * 0.5, D, a.invD, (double)a.nxm);
+        }
+        if (xmc >= 0 && ymc >= 0 && xmc <= a.N - 1 && ymc <= a.N - 1) {
+            if (a.rcheck) {
+                const double r = sqrt((xb0 + xb1) * (xb0 + xb1) + (-yb0 - yb1) * (-yb0 - yb1)) * 0.5;
+                if (r >= a.rmax) return;
+            }
+            const double rr = FAST ? fabs(p1.a - p0.a) * sec
+                                   : sqrt((xb0 - xb1) * (xb0 - xb1) + (yb0 - yb1) * (yb0 - yb1));
+            sink.add(xmc + ymc * a.N, rr);
+        }
+    };
+
+    MarchPoint p0{ai, bi, 0, true};
+    MarchPoint pc{(double)ain, 0.0, 1, !FAST};
+    pc.b = b_calc(pc.a);
+    if (FAST && fabs(fabs(pc.b) - Lb) < kMarchMargin) make_exact(pc);
+    int bma = 0;
+    int iks = 0;
+
+    for (int ik = 0;; ik++) {
+        if (!(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) break;
+        if (ik >= a.max_steps) { ovf = true; break; }
+        const int bmb = bma;
+        const double bj = pc.b;
+        pc.a = ain + pha * ik * D;  // :126
+        pc.b = b_calc(pc.a);        // :129
+        pc.exact = !FAST;
+        if (FAST) {
+            const double ab = fabs(pc.b);
+            if (ab < kMarchMargin || fabs(ab - Lb) < kMarchMargin) make_exact(pc);
+            if (!index_checked(pc.b, a.invD, 0.0, bma)) {
+                make_exact(pc);
+                bma = trunc_i32(pc.b / D);  // :130
+            }
+        } else {
+            bma = index_of<false>(pc.b, D, a.invD, 0.0);
+        }
+        const double bii = pc.b;
+        if (ik != 0 && (bma != bmb || sign_split(bii, bj))) {  // :134
+            MarchPoint px{0.0, 0.0, 2, !FAST};
+            if ((bii < 0 && b_neg) || (bii > 0 && b_pos))
+                px.b = (double)(bma * D);
+            else
+                px.b = (double)(bma * D) - bdir * D;
+            px.a = a_calc(px.b);  // :137
+            if (FAST && fabs(pc.a - px.a) < kMarchMargin && fabs(pc.b - px.b) < kMarchMargin) {
+                make_exact(pc);
+                make_exact(px);
+            }
+            const double chy = (pc.a - px.a) * (pc.a - px.a) + (pc.b - px.b) * (pc.b - px.b);
+            if (chy != 0) {  // :141
+                iks++;
+                emit(p0, px);
+                p0 = px;
+            }
+        }
+        iks++;
+        emit(p0, pc);
+        p0 = pc;
+    }
+    return iks;
+}
+
+// Trace ray (view, ch).  Calls sink.add(voxel, length) for every segment
+// whose midpoint voxel lies in the image, in the reference's order, and
+// returns the reference's per-ray segment counter (iks).  ovf is set when
+// the step bound is hit (never for valid geometry).
+template <bool FAST_IN, class Sink>
+__device__ __forceinline__ int march_ray(const TraceArgs& a, int view, int ch, Sink& sink, bool& ovf)
+{
+    // source and detector point, M/reprojection_TOMO.cpp:53-74
+    const double c = a.cosT[view], s = a.sinT[view];
+    const double sx = a.sid * s;
+    const double sy = a.sid * c;
+    const double px0 = a.px0[ch], py0 = a.py0[ch];
+    const double px = +px0 * c + py0 * s;
+    const double py = -px0 * s + py0 * c;
+    const double Lb = (double)a.nxm * a.D;
+    const double D = a.D;
+
+    // clip against the image square, :76-101 (only the first two complete
+    // intersections are used)
+    int m = 0;
+    double h0x = 0.0, h0y = 0.0, h1x = 0.0, h1y = 0.0;
+    {
+        const double y = Lb;
+        const double x = (y - sy) / (py - sy) * (px - sx) + sx;
+        if (fabs(x) < Lb) { h0x = x; h0y = y; m = 1; }
+    }
+    {
+        const double y = -Lb;
+        const double x = (y - sy) / (py - sy) * (px - sx) + sx;
+        if (fabs(x) < Lb) {
+            if (m == 0) { h0x = x; h0y = y; } else { h1x = x; h1y = y; }
+            m++;
+        }
+    }
+    {
+        const double x = Lb;
+        const double y = (x - sx) / (px - sx) * (py - sy) + sy;
+        if (fabs(y) < Lb) {
+            if (m == 0) { h0x = x; h0y = y; } else if (m == 1) { h1x = x; h1y = y; }
+            m++;
+        }
+    }
+    {
+        const double x = -Lb;
+        const double y = (x - sx) / (px - sx) * (py - sy) + sy;
+        if (fabs(y) < Lb) {
+            if (m == 0) { h0x = x; h0y = y; } else if (m == 1) { h1x = x; h1y = y; }
+            m++;
+        }
+    }
+    if (m < 2) return 0;  // the ray misses the image: no segment, transmission 1
+
+    // entry = the intersection closer to the source, :103-108
+    const double d0 = (h0x - sx) * (h0x - sx) + (h0y - sy) * (h0y - sy);
+    const double d1 = (h1x - sx) * (h1x - sx) + (h1y - sy) * (h1y - sy);
+    const bool first = d0 < d1;
+    const double xi = first ? h0x : h1x, yi = first ? h0y : h1y;
+    const double xf = first ? h1x : h0x, yf = first ? h1y : h0y;
+
+    // dominant axis, :113.  a = major axis, b = minor axis; the reference's
+    // two branches (:113-200, :201-283) are this march with x and y exchanged.
+    const bool xmajor = fabs(xi - xf) > fabs(yi - yf);
+    const double ai = xmajor ? xi : yi, af = xmajor ? xf : yf;
+    const double bi = xmajor ? yi : xi, bf = xmajor ? yf : xf;
+    const double sa = xmajor ? sx : sy, sb = xmajor ? sy : sx;
+    const double pa = xmajor ? px : py, pb = xmajor ? py : px;
+
+    RayConst r;
+    r.xmajor = xmajor;
+    r.ai = ai; r.af = af; r.bi = bi; r.bf = bf;
+    r.sa = sa; r.sb = sb; r.dA = pa - sa; r.dB = pb - sb;
+    r.Lb = Lb;
+    // the fast path needs finite reciprocals and no r >= rmax test
+    if (FAST_IN && r.dB != 0.0 && !a.rcheck && !a.exact) return march_loop<true>(a, r, sink, ovf);
+    return march_loop<false>(a, r, sink, ovf);
+}
+
+}  // namespace vct

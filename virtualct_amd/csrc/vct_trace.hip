@@ -20,45 +20,12 @@
 //   k_trace<adjoint>  the same march, scattering w_m * length into the
 //                gradient map with hardware float atomics.
 #include "vct_internal.h"
+#include "vct_march.h"
 
 #include <climits>
 #include <cmath>
 
 namespace vct {
-
-// x86-64 cvttsd2si semantics (the reference's (int) casts): NaN and values
-// outside int range give INT_MIN.
-__device__ __forceinline__ int trunc_i32(double v)
-{
-    return (v > -2147483649.0 && v < 2147483648.0) ? (int)v : INT_MIN;
-}
-
-// trunc(fl(fl(h / D) + c)) -- the voxel index expressions of the reference
-// (M/reprojection_TOMO.cpp:130,146-147,217).  The product h * (1/D) differs
-// from the correctly rounded quotient by a few ulp at most, so its
-// truncation can only differ when the value lies within a few ulp of an
-// integer; such values (|frac| < 1e-9 from an integer, ~1e4 x the error
-// bound for |values| <= 2^11) take the exact division.
-template <bool FAST>
-__device__ __forceinline__ int index_of(double h, double D, double invD, double c)
-{
-    if (FAST) {
-        const double q = h * invD + c;
-        const double t = trunc(q);
-        const double f = fabs(q - t);
-        if (f > 1e-9 && f < 1.0 - 1e-9 && fabs(q) < 1.0e6) return (int)t;
-    }
-    return trunc_i32(h / D + c);
-}
-
-// (u/fabs(u) != v/fabs(v)) of M/reprojection_TOMO.cpp:134,221 for finite
-// operands, without the divisions: true if either is zero or NaN (0/0 is
-// NaN, which compares unequal) or the signs differ.
-__device__ __forceinline__ bool sign_split(double u, double v)
-{
-    const bool up = u > 0.0, un = u < 0.0, vp = v > 0.0, vn = v < 0.0;
-    return !(up || un) || !(vp || vn) || (up != vp);
-}
 
 // ------------------------------------------------------------------ sinks
 
@@ -137,138 +104,6 @@ struct SinkAdj {
         }
     }
 };
-
-// ------------------------------------------------------------------ march
-
-// Trace ray (view, ch).  Calls sink.add(voxel, length) for every segment
-// whose midpoint voxel lies in the image, in the reference's order, and
-// returns the reference's per-ray segment counter (iks).  *ovf is set when
-// the step bound is hit (never for valid geometry).
-template <bool FAST, class Sink>
-__device__ __forceinline__ int march_ray(const TraceArgs& a, int view, int ch, Sink& sink, bool& ovf)
-{
-    // source and detector point, M/reprojection_TOMO.cpp:53-74
-    const double c = a.cosT[view], s = a.sinT[view];
-    const double sx = a.sid * s;
-    const double sy = a.sid * c;
-    const double px0 = a.px0[ch], py0 = a.py0[ch];
-    const double px = +px0 * c + py0 * s;
-    const double py = -px0 * s + py0 * c;
-    const double Lb = (double)a.nxm * a.D;
-    const double D = a.D;
-
-    // clip against the image square, :76-101 (only the first two complete
-    // intersections are used)
-    int m = 0;
-    double h0x = 0.0, h0y = 0.0, h1x = 0.0, h1y = 0.0;
-    {
-        const double y = Lb;
-        const double x = (y - sy) / (py - sy) * (px - sx) + sx;
-        if (fabs(x) < Lb) { h0x = x; h0y = y; m = 1; }
-    }
-    {
-        const double y = -Lb;
-        const double x = (y - sy) / (py - sy) * (px - sx) + sx;
-        if (fabs(x) < Lb) {
-            if (m == 0) { h0x = x; h0y = y; } else { h1x = x; h1y = y; }
-            m++;
-        }
-    }
-    {
-        const double x = Lb;
-        const double y = (x - sx) / (px - sx) * (py - sy) + sy;
-        if (fabs(y) < Lb) {
-            if (m == 0) { h0x = x; h0y = y; } else if (m == 1) { h1x = x; h1y = y; }
-            m++;
-        }
-    }
-    {
-        const double x = -Lb;
-        const double y = (x - sx) / (px - sx) * (py - sy) + sy;
-        if (fabs(y) < Lb) {
-            if (m == 0) { h0x = x; h0y = y; } else if (m == 1) { h1x = x; h1y = y; }
-            m++;
-        }
-    }
-    if (m < 2) return 0;  // the ray misses the image: no segment, transmission 1
-
-    // entry = the intersection closer to the source, :103-108
-    const double d0 = (h0x - sx) * (h0x - sx) + (h0y - sy) * (h0y - sy);
-    const double d1 = (h1x - sx) * (h1x - sx) + (h1y - sy) * (h1y - sy);
-    const bool first = d0 < d1;
-    const double xi = first ? h0x : h1x, yi = first ? h0y : h1y;
-    const double xf = first ? h1x : h0x, yf = first ? h1y : h0y;
-
-    // dominant axis, :113.  a = major axis, b = minor axis; the reference's
-    // two branches (:113-200, :201-283) are this march with x and y exchanged.
-    const bool xmajor = fabs(xi - xf) > fabs(yi - yf);
-    const double ai = xmajor ? xi : yi, af = xmajor ? xf : yf;
-    const double bi = xmajor ? yi : xi, bf = xmajor ? yf : xf;
-    const double sa = xmajor ? sx : sy, sb = xmajor ? sy : sx;
-    const double pa = xmajor ? px : py, pb = xmajor ? py : px;
-
-    const double pha = (af - ai) / fabs(ai - af);
-    int ain;
-    if ((ai < 0 && (af - ai) < 0) || (ai > 0 && (af - ai) > 0))
-        ain = trunc_i32(trunc_i32(ai) + (af - ai) / fabs(af - ai));
-    else
-        ain = trunc_i32(ai);
-    const double bdir = (bf - bi) / fabs(bf - bi);
-    const bool b_out = (bf - bi) > 0;  // minor axis moves towards +
-    const bool b_in = (bf - bi) < 0;
-
-    double aii = ain;
-    double bii = (aii - sa) / (pa - sa) * (pb - sb) + sb;
-    double ab0 = ai, bb0 = bi;
-    int bma = 0;
-    int iks = 0;
-
-    // one candidate segment (a0,b0)->(a1,b1), :141-166 / :168-189
-    auto emit = [&](double a0, double b0, double a1, double b1) {
-        const double xb0 = xmajor ? a0 : b0, yb0 = xmajor ? b0 : a0;
-        const double xb1 = xmajor ? a1 : b1, yb1 = xmajor ? b1 : a1;
-        const int xmc = index_of<FAST>((xb0 + xb1) * 0.5, D, a.invD, (double)a.nxm);
-        const int ymc = index_of<FAST>((-yb0 - yb1) * 0.5, D, a.invD, (double)a.nxm);
-        if (xmc >= 0 && ymc >= 0 && xmc <= a.N - 1 && ymc <= a.N - 1) {
-            if (a.rcheck) {
-                const double r = sqrt((xb0 + xb1) * (xb0 + xb1) + (-yb0 - yb1) * (-yb0 - yb1)) * 0.5;
-                if (r >= a.rmax) return;
-            }
-            const double rr = sqrt((xb0 - xb1) * (xb0 - xb1) + (yb0 - yb1) * (yb0 - yb1));
-            sink.add(xmc + ymc * a.N, rr);
-        }
-    };
-
-    for (int ik = 0;; ik++) {
-        if (!(aii <= Lb && aii >= -Lb && bii <= Lb && bii >= -Lb)) break;
-        if (ik >= a.max_steps) { ovf = true; break; }
-        aii = ain + pha * ik * D;  // :126
-        const int bmb = bma;
-        const double bj = bii;
-        bii = (aii - sa) / (pa - sa) * (pb - sb) + sb;  // :129
-        bma = index_of<FAST>(bii, D, a.invD, 0.0);      // :130
-        if (ik != 0 && (bma != bmb || sign_split(bii, bj))) {  // :134
-            double biii;
-            if ((bii < 0 && b_in) || (bii > 0 && b_out))
-                biii = (double)(bma * D);
-            else
-                biii = (double)(bma * D) - bdir * D;
-            const double aiii = (biii - sb) / (pb - sb) * (pa - sa) + sa;
-            const double chy = (aii - aiii) * (aii - aiii) + (bii - biii) * (bii - biii);
-            if (chy != 0) {  // :141
-                iks++;
-                emit(ab0, bb0, aiii, biii);
-                ab0 = aiii;
-                bb0 = biii;
-            }
-        }
-        iks++;
-        emit(ab0, bb0, aii, bii);
-        ab0 = aii;
-        bb0 = bii;
-    }
-    return iks;
-}
 
 // ---------------------------------------------------------------- kernels
 
