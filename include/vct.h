@@ -183,6 +183,20 @@ vct_status vct_mbfpa_forward(vct_plan* plan, const void* d_packed, int dens_type
  * the "IN" threshold of :250-255. */
 vct_status vct_filter_sinogram(vct_plan* plan, const float* d_proj, int n_sino, int filter,
                                int in_mode, float* d_fltd, void* stream);
+/* Beam-hardening correction, replacing the standalone tool
+ * M/beam_hardening_correction.cpp:21-69 (run between MBFPA and FBP):
+ * y = -ln p, y' = |y| + alpha*|y|^beta, out = exp(-y'), elementwise over n
+ * floats (in place allowed). */
+vct_status vct_beam_hardening(const float* d_in, int n, double alpha, double beta, float* d_out,
+                              void* stream);
+/* The FBP filter with that correction fused into the channel pre-weight
+ * (one pass instead of tool -> file -> filter). */
+vct_status vct_filter_sinogram_bhc(vct_plan* plan, const float* d_proj, int n_sino, int filter,
+                                   int in_mode, double alpha, double beta, float* d_fltd, void* stream);
+/* Max and min of n floats on the device, returned to host h_maxmin[2]
+ * (synchronous); the scans of M/main.cpp:512-518 and :53-60. */
+vct_status vct_minmax(const float* d_x, int n, float* h_maxmin, void* stream);
+
 /* Pixel-driven fan backprojection, replacing reconstructImage
  * (F/reconstructImageTOMO.cpp:11-105), n_sino images. */
 vct_status vct_backproject(vct_plan* plan, const float* d_fltd, int n_sino, float* d_img,

@@ -89,6 +89,8 @@ class Oracle:
         L.orc_normalize.restype = C.c_float
         L.orc_gaussian_noise_glibc.argtypes = [_f32p, C.c_int, C.c_double, C.c_int]
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_beam_hardening.argtypes = [_f32p, C.c_int, C.c_double, C.c_double,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_const_map.argtypes = [G, C.c_int, _f64p]
         L.orc_filter_ref.argtypes = [G, _f64p, _f32p, _f32p, C.c_int]
         L.orc_filter_f64.argtypes = [G, _f64p, _f32p, _f64p]
@@ -168,6 +170,12 @@ class Oracle:
         out = np.zeros(4, np.uint32)
         self.lib.orc_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
         return out
+
+    def beam_hardening(self, sino, alpha: float, beta: float):
+        s = np.ascontiguousarray(sino, np.float32).copy()
+        a, b = C.c_double(), C.c_double()
+        self.lib.orc_beam_hardening(s.reshape(-1), s.size, alpha, beta, C.byref(a), C.byref(b))
+        return s, a.value, b.value
 
     # -- FBP
     def const_map(self, g: Geom, filt: int = 2):

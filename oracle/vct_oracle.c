@@ -476,6 +476,22 @@ void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint3
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+void orc_beam_hardening(float* p, int n, double alpha, double beta, double* min_org, double* min_new)
+{
+    /* M/beam_hardening_correction.cpp:47-60 */
+    double min_nii = 1000000, min_o = 1000000;
+    for (int i = 0; i < n; i++) {
+        if (min_o > p[i]) min_o = p[i];
+        /* log() of a float is the C++ float overload there, i.e. logf */
+        double yi = -logf(p[i]);
+        yi = fabs(yi) + (alpha * pow(fabs(yi), beta));
+        p[i] = (float)exp(-yi);
+        if (min_nii > p[i]) min_nii = p[i];
+    }
+    if (min_org) *min_org = min_o;
+    if (min_new) *min_new = min_nii;
+}
+
 /* ------------------------------------------------------------------- FBP */
 
 void orc_const_map(const orc_geom* g, int filter, double* C)

@@ -84,6 +84,9 @@ void orc_gaussian_noise_glibc(float* sino, int n, double sigma, int order);
 /* Philox4x32-10 (Salmon et al., SC'11): ctr[4], key[2] -> out[4]. */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* Beam-hardening correction tool (elementwise, in place). */
+void orc_beam_hardening(float* p, int n, double alpha, double beta, double* min_org, double* min_new);
+
 /* FBP */
 void orc_const_map(const orc_geom* g, int filter, double* C /* [W][W] */);
 void orc_filter_ref(const orc_geom* g, const double* C, const float* proj,

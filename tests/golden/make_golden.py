@@ -181,6 +181,31 @@ def main():
         xz_write(os.path.join(HERE, "icru110_120kV_sigma0.f32.xz"), norm.tobytes())
         print("full sinogram written, max", mx, flush=True)
     shutil.rmtree(tmp, ignore_errors=True)
+    bhc_goldens()
+
+
+def bhc_goldens():
+    """The reference's beam-hardening tool (oracle/_ref/bhc_ref, built as its
+    own instructions say) on the noise-free 120 kV sinogram: SHA-256 of its
+    output and the two minima it prints, for (alpha, beta) = (0.01, 2.0) and
+    (0.01, 3.7)."""
+    import subprocess
+    import tempfile
+    path = os.path.join(HERE, "icru110_120kV_sigma0.f32.xz")
+    sino = np.frombuffer(lzma.decompress(open(path, "rb").read()), np.float32)
+    with open(os.path.join(HERE, "kat_hashes.json")) as f:
+        hashes = json.load(f)
+    exe = os.path.join(ROOT, "oracle", "_ref", "bhc_ref")
+    for alpha, beta in ((0.01, 2.0), (0.01, 3.7)):
+        with tempfile.TemporaryDirectory() as d:
+            sino.tofile(os.path.join(d, "reprojection_float.raw"))
+            r = subprocess.run([exe, str(alpha), str(beta)], cwd=d, capture_output=True, text=True, check=True)
+            out = os.path.join(d, "reprojection_float_cor.raw")
+            key = f"bhc_sigma0_a{alpha}_b{beta}"
+            hashes[key + "_sha256"] = sha256(out)
+            hashes[key + "_stdout"] = r.stdout.strip()
+    with open(os.path.join(HERE, "kat_hashes.json"), "w") as f:
+        json.dump(hashes, f, indent=1)
 
 
 if __name__ == "__main__":

@@ -119,3 +119,34 @@ def test_mbfpa_then_fbp_recovers_mu_map(gpu, golden):
     mu = (ph.astype(np.float64) / 1000 * att[:, i60, None, None]).sum(0)
     r = np.corrcoef(img.ravel(), mu.ravel())[0, 1]
     assert r > 0.95
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.01, 2.0), (0.01, 3.7)])
+def test_beam_hardening_kernel_and_fused_filter(gpu, golden, oracle, alpha, beta):
+    ops = _ops()
+    p = golden.sigma0_120kv()
+    ref, _, _ = oracle.beam_hardening(p, alpha, beta)  # bit-identical to the reference tool
+    got = ops.beam_hardening(_t(p, gpu), alpha, beta).cpu().numpy()
+    assert rel_max(got, ref) < 1e-6
+    assert np.count_nonzero(got != ref) < 0.01 * got.size
+    plan = ops.plan_for(_geom())
+    fused = ops.filter_sinogram_bhc(plan, _t(p, gpu), alpha, beta)[0].cpu().numpy()
+    two_step = oracle.filter_ref(Geom(), ref, 2)
+    assert rel_max(fused, two_step) < 1e-5
+    mx, mn = ops.minmax(_t(p, gpu))
+    assert mx == p.max() and mn == p.min()
+
+
+def test_bhc_driver(gpu, golden, oracle, tmp_path):
+    import subprocess
+    from conftest import ROOT
+    import os
+    p = golden.sigma0_120kv()
+    p.tofile(tmp_path / "reprojection_float.raw")
+    r = subprocess.run([os.path.join(ROOT, "virtualct_amd", "bin", "mvct_bhc"), "0.01", "2.0"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = np.fromfile(tmp_path / "reprojection_float_cor.raw", np.float32).reshape(800, 609)
+    ref, mn0, mn1 = oracle.beam_hardening(p, 0.01, 2.0)
+    assert rel_max(out, ref) < 1e-6
+    assert r.stdout.strip() == f"{mn0:f} {mn1:f}"

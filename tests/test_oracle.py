@@ -201,3 +201,11 @@ def test_noise_order_matters(oracle, golden):
     b = oracle.noise_glibc(x, 0.01, order=1)
     assert not np.array_equal(a, b)
     assert np.array_equal(oracle.noise_glibc(x, 0.0, 0), x)
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.01, 2.0), (0.01, 3.7)])
+def test_beam_hardening_matches_reference_tool(oracle, golden, alpha, beta):
+    out, mn0, mn1 = oracle.beam_hardening(golden.sigma0_120kv(), alpha, beta)
+    key = f"bhc_sigma0_a{alpha}_b{beta}"
+    assert sha(out) == golden.hashes()[key + "_sha256"]
+    assert f"{mn0:f} {mn1:f}" == golden.hashes()[key + "_stdout"]

@@ -26,7 +26,7 @@ ARCH = os.environ.get("VCT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 LIB_SOURCES = ["vct_api.cpp", "vct_trace.hip", "vct_fbp.hip", "vct_adjoint.hip", "vct_batch.hip"]
-DRIVERS = {"mvct_mbfpa": "mvct_mbfpa.cpp", "mvct_fbp": "mvct_fbp.cpp"}
+DRIVERS = {"mvct_mbfpa": "mvct_mbfpa.cpp", "mvct_fbp": "mvct_fbp.cpp", "mvct_bhc": "mvct_bhc.cpp"}
 
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

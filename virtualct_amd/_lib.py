@@ -72,6 +72,9 @@ _SIGS = {
                                      C.c_uint64, C.c_uint32, _P, _P, _P]),
     "vct_filter_sinogram": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "vct_backproject": (C.c_int, [_P, _P, C.c_int, _P, _P]),
+    "vct_beam_hardening": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, _P, _P]),
+    "vct_filter_sinogram_bhc": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P, _P]),
+    "vct_minmax": (C.c_int, [_P, C.c_int, _P, _P]),
     "vct_trace_adjoint": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
     "vct_reprojection": (C.c_int, [_P, C.POINTER(Geometry), _P, C.c_int, _P, _P, C.c_int, _P]),
     "vct_gaussian_noise": (C.c_int, [_P, _P, C.c_int, C.c_double, C.c_uint64, _P]),

@@ -219,12 +219,17 @@ class BatchedSolver:
         if not self.have_prev:
             self.step()
         torch.cuda.synchronize()
+        state = [t.clone() for t in (self.x, self.prev_x, self.prev_g, self.alpha)]
         side = torch.cuda.Stream(self.x.device)
         side.wait_stream(torch.cuda.current_stream(self.x.device))
         with torch.cuda.stream(side):  # warm the allocator outside capture
             self._forward_loss_grad()
             self._update()
         torch.cuda.current_stream(self.x.device).wait_stream(side)
+        torch.cuda.synchronize()
+        for dst, src in zip((self.x, self.prev_x, self.prev_g, self.alpha), state):
+            dst.copy_(src)  # the warm-up must not advance the iteration
+        torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self._forward_loss_grad()

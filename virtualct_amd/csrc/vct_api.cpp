@@ -524,8 +524,8 @@ vct_status vct_mbfpa_forward(vct_plan* p, const void* d_packed, int dens_type, i
     return vct_detector(p, raw, stats, S, sigma, seed, stream_id, d_sino, s);
 }
 
-vct_status vct_filter_sinogram(vct_plan* p, const float* d_proj, int n_sino, int filter, int in_mode, float* d_fltd,
-                               void* stream)
+static vct_status filter_impl(vct_plan* p, const float* d_proj, int n_sino, int filter, int in_mode, int bhc,
+                              double alpha, double beta, float* d_fltd, void* stream)
 {
     clear_error();
     if (!p || !d_proj || !d_fltd) return set_error(VCT_ERR_ARG, "NULL argument");
@@ -544,6 +544,9 @@ vct_status vct_filter_sinogram(vct_plan* p, const float* d_proj, int n_sino, int
     pa.R = (int)R;
     pa.W = p->g.n_channels;
     pa.Kp = p->Kp;
+    pa.bhc = bhc;
+    pa.bhc_alpha = alpha;
+    pa.bhc_beta = beta;
     FilterGemmArgs ga{};
     ga.apk = pa.apk;
     ga.bpk = bpk;
@@ -553,6 +556,44 @@ vct_status vct_filter_sinogram(vct_plan* p, const float* d_proj, int n_sino, int
     ga.Kp = p->Kp;
     ga.in_mode = in_mode ? 1 : 0;
     return launch_filter(pa, ga, pick_stream(p, stream));
+}
+
+vct_status vct_filter_sinogram(vct_plan* p, const float* d_proj, int n_sino, int filter, int in_mode, float* d_fltd,
+                               void* stream)
+{
+    return filter_impl(p, d_proj, n_sino, filter, in_mode, 0, 0.0, 0.0, d_fltd, stream);
+}
+
+vct_status vct_filter_sinogram_bhc(vct_plan* p, const float* d_proj, int n_sino, int filter, int in_mode, double alpha,
+                                   double beta, float* d_fltd, void* stream)
+{
+    if (!std::isfinite(alpha) || !std::isfinite(beta)) return set_error(VCT_ERR_ARG, "alpha/beta must be finite");
+    return filter_impl(p, d_proj, n_sino, filter, in_mode, 1, alpha, beta, d_fltd, stream);
+}
+
+vct_status vct_beam_hardening(const float* d_in, int n, double alpha, double beta, float* d_out, void* stream)
+{
+    clear_error();
+    if (!d_in || !d_out) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (!std::isfinite(alpha) || !std::isfinite(beta)) return set_error(VCT_ERR_ARG, "alpha/beta must be finite");
+    return launch_beam_hardening(d_in, d_out, n, alpha, beta, (hipStream_t)stream);
+}
+
+vct_status vct_minmax(const float* d_x, int n, float* h_maxmin, void* stream)
+{
+    clear_error();
+    if (!d_x || !h_maxmin || n < 1) return set_error(VCT_ERR_ARG, "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* d_st = nullptr;
+    VCT_HIP(hipMallocAsync((void**)&d_st, 2 * sizeof(uint32_t), s));
+    vct_status st = launch_minmax(d_x, n, d_st, s);
+    if (st == VCT_OK) {
+        hipError_t e = hipMemcpyAsync(h_maxmin, d_st, 2 * sizeof(float), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = set_error(VCT_ERR_HIP, "minmax copy: %s", hipGetErrorString(e));
+    }
+    (void)hipFreeAsync(d_st, s);
+    return st;
 }
 
 vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* d_img, void* stream)
@@ -575,6 +616,7 @@ vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* 
     a.pix = g.pixel_cm;
     a.sid = g.sid_cm;
     a.dbeta_ch = g.max_beta / p->half;
+    a.inv_dbeta_ch = 1.0 / a.dbeta_ch;
     a.dbeta_view = 2.0 * kPi / (g.n_views);
     a.xfactor = g.xfactor;
     a.center = (double)g.recon_size / 2.0 - 0.5;

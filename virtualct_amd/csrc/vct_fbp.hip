@@ -52,7 +52,16 @@ __global__ __launch_bounds__(256) void k_filter_prep(FilterPrepArgs a)
     float v = 0.f;
     if (r < a.R && k < a.W) {
         const double p = a.proj[(size_t)r * a.W + k];
-        v = (float)(-log(p) * a.coeff[k]);
+        double y;
+        if (a.bhc) {
+            // beam-hardening correction, M/beam_hardening_correction.cpp:51-55 (whose
+            // log of a float is the single-precision overload)
+            y = -(double)logf((float)p);
+            y = fabs(y) + (a.bhc_alpha * pow(fabs(y), a.bhc_beta));
+        } else {
+            y = -log(p);
+        }
+        v = (float)(y * a.coeff[k]);
     }
     a.apk[pack_index(r >> 5, k, r & 31, a.Kp)] = v;
 }
@@ -97,6 +106,40 @@ __global__ __launch_bounds__(128) void k_filter_gemm(FilterGemmArgs a)
     }
 }
 
+// atan(t) for |t| <= 0.5 as t * P(t^2), P of degree 12: the interpolant of
+// atan(sqrt(u))/sqrt(u) at 13 Chebyshev nodes of [0, 0.25], computed in 50-digit
+// arithmetic (relative error 4e-18 before rounding; <= 1.2 ulp evaluated in
+// FP64 over |t| <= 0.5).  Masked pixels see |t| <= tan(asin(r_mask / sid))
+// (0.32 for the reference geometry); larger |t| take the library atan.
+__device__ __forceinline__ double atan_small(double t)
+{
+    const double u = t * t;
+    double p = 0.010030841489968486;
+    p = __builtin_fma(p, u, -0.027302779603074177);
+    p = __builtin_fma(p, u, 0.041775850326747944);
+    p = __builtin_fma(p, u, -0.05118531938647954);
+    p = __builtin_fma(p, u, 0.05857435765165694);
+    p = __builtin_fma(p, u, -0.06663667844183857);
+    p = __builtin_fma(p, u, 0.07692057989976618);
+    p = __builtin_fma(p, u, -0.09090895058092312);
+    p = __builtin_fma(p, u, 0.11111110602305971);
+    p = __builtin_fma(p, u, -0.14285714274692063);
+    p = __builtin_fma(p, u, 0.19999999999875542);
+    p = __builtin_fma(p, u, -0.3333333333333278);
+    p = __builtin_fma(p, u, 1.0);
+    return t * p;
+}
+
+// 1/x to ~1 ulp: hardware estimate + two Newton steps.
+__device__ __forceinline__ double rcp_f64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 __global__ __launch_bounds__(256) void k_backproject(BackprojArgs a)
 {
     extern __shared__ float sv[];  // [kBpViews][W]
@@ -124,20 +167,44 @@ __global__ __launch_bounds__(256) void k_backproject(BackprojArgs a)
                 const double c = a.cosT[th], sn = a.sinT[th];
                 const double X = xx * c + yy * sn;
                 const double Y = -xx * sn + yy * c;
-                const double L2 = (-Y + a.sid) * (-Y + a.sid) + X * X;
-                const double fGAM = atan(X / (a.sid - Y)) / a.dbeta_ch + a.cent;
+                const double den = a.sid - Y;
+                const double L2 = den * den + X * X;
+                const double tg = X * rcp_f64(den);
+                const double gam = fabs(tg) <= 0.5 ? atan_small(tg) : atan(X / den);
+                const double fGAM = gam * a.inv_dbeta_ch + a.cent;
                 const int GAM = trunc_i32_bp(fGAM);
                 const double w = fGAM - GAM;
                 if (GAM >= 0 && GAM < a.W - 1) {
                     const float* row = sv + t * a.W;
                     const double sum1 = (1 - w) * row[GAM] + w * row[GAM + 1];
                     const double sum = sum1 + 0.0;
-                    acc += (float)(1.0 / L2 * sum * a.dbeta_view * a.xfactor);
+                    acc += (float)(rcp_f64(L2) * sum * a.dbeta_view * a.xfactor);
                 }
             }
         }
     }
     if (in_img) a.img[((size_t)s * a.N + y) * a.N + x] = acc;
+}
+
+// The beam-hardening tool M/beam_hardening_correction.cpp:47-60 elementwise:
+// y = -ln p (single-precision log, as the tool's C++ overload resolution
+// gives), y' = |y| + alpha |y|^beta, out = exp(-y') in FP64, stored as float.
+__global__ __launch_bounds__(256) void k_beam_hardening(const float* __restrict__ in, float* __restrict__ out, int n,
+                                                        double alpha, double beta)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double yi = -(double)logf(in[i]);  // the tool's log of a float is logf
+    yi = fabs(yi) + (alpha * pow(fabs(yi), beta));
+    out[i] = (float)exp(-yi);
+}
+
+vct_status launch_beam_hardening(const float* in, float* out, int n, double alpha, double beta, hipStream_t s)
+{
+    if (n <= 0) return set_error(VCT_ERR_ARG, "empty sinogram");
+    hipLaunchKernelGGL(k_beam_hardening, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n, alpha, beta);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
 }
 
 vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipStream_t s)

@@ -87,6 +87,8 @@ struct FilterPrepArgs {
     const double* coeff;  // [W] sid*cos((k - W/2) * dbeta)
     float* apk;           // packed A operand
     int R, W, Kp;
+    int bhc;              // fuse the beam-hardening correction
+    double bhc_alpha, bhc_beta;
 };
 
 struct FilterGemmArgs {
@@ -102,7 +104,7 @@ struct BackprojArgs {
     const double* sinT;   // [V]
     float* img;           // [S][N][N]
     int N, W, V, use_views, cent, half;
-    double pix, sid, dbeta_ch, dbeta_view, xfactor, center, mask_r2, mask_r;
+    double pix, sid, dbeta_ch, inv_dbeta_ch, dbeta_view, xfactor, center, mask_r2, mask_r;
 };
 
 }  // namespace vct
@@ -146,6 +148,7 @@ vct_status launch_pack_i16(const int16_t* src, int M, int N, uint16_t* dst, hipS
 vct_status launch_pack_f32(const float* src, int M, int N, float* dst, hipStream_t s);
 vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipStream_t s);
 vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s);
+vct_status launch_beam_hardening(const float* in, float* out, int n, double alpha, double beta, hipStream_t s);
 vct_status plan_filter_matrix(vct_plan* plan, int filter, float** d_bpk);
 TraceArgs make_trace_args(const vct_plan* plan);
 }  // namespace vct

@@ -202,8 +202,44 @@ def backproject(plan: Plan, fltd: torch.Tensor):
     return img
 
 
-def fbp(plan: Plan, proj: torch.Tensor, filt: int = FILTER_SHEPP_LOGAN, in_mode: bool = False):
-    return backproject(plan, filter_sinogram(plan, proj, filt, in_mode))
+def fbp(plan: Plan, proj: torch.Tensor, filt: int = FILTER_SHEPP_LOGAN, in_mode: bool = False,
+        bhc: tuple[float, float] | None = None):
+    """FBP; bhc=(alpha, beta) fuses the beam-hardening correction into the filter."""
+    if bhc is None:
+        return backproject(plan, filter_sinogram(plan, proj, filt, in_mode))
+    return backproject(plan, filter_sinogram_bhc(plan, proj, bhc[0], bhc[1], filt, in_mode))
+
+
+def filter_sinogram_bhc(plan: Plan, proj: torch.Tensor, alpha: float, beta: float,
+                        filt: int = FILTER_SHEPP_LOGAN, in_mode: bool = False):
+    _require_gpu(proj)
+    proj = proj.contiguous()
+    if proj.dim() == 2:
+        proj = proj.unsqueeze(0)
+    S = proj.shape[0]
+    out = torch.empty_like(proj, dtype=torch.float32)
+    call("vct_filter_sinogram_bhc", plan.handle, _ptr(proj), S, filt, int(in_mode), float(alpha), float(beta),
+         _ptr(out), _stream(plan.device))
+    return out
+
+
+def beam_hardening(sino: torch.Tensor, alpha: float, beta: float) -> torch.Tensor:
+    """y = -ln p, y' = |y| + alpha |y|^beta, exp(-y') elementwise."""
+    _require_gpu(sino)
+    sino = sino.contiguous()
+    out = torch.empty_like(sino, dtype=torch.float32)
+    call("vct_beam_hardening", _ptr(sino), sino.numel(), float(alpha), float(beta), _ptr(out),
+         C.c_void_p(torch.cuda.current_stream(sino.device).cuda_stream))
+    return out
+
+
+def minmax(x: torch.Tensor):
+    """(max, min) of a float tensor, via the device reduction."""
+    _require_gpu(x)
+    x = x.contiguous()
+    out = (C.c_float * 2)()
+    call("vct_minmax", _ptr(x), x.numel(), C.cast(out, C.c_void_p), C.c_void_p(torch.cuda.current_stream(x.device).cuda_stream))
+    return float(out[0]), float(out[1])
 
 
 # ----------------------------------------------------------------- adjoint
