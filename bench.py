@@ -201,6 +201,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-batched", action="store_true", help="skip the decomposition steps/s measurement")
     ap.add_argument("--batch-fbp", action="store_true", help="also time 64 uneven phantoms x 3 spectra")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="run one device-to-device copy of MB MiB first (calibrates PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -225,6 +227,13 @@ def main():
     d_pdf = torch.from_numpy(pdf).to(dev)
     sigma = 5.37e-5
     pk, dt = ops.pack_densities(plan, d_ph)
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     torch.cuda.synchronize()
 
     stages = ["trace", "spectral", "detector", "filter", "backproject"]

@@ -105,8 +105,14 @@ vct_status vct_plan_destroy(vct_plan* plan);
 /* Plan options (no counterpart in the reference).
  * VCT_OPT_EXACT_TRACE = 1: march every ray with correctly rounded divisions
  * (the reference's operation sequence) instead of the default guarded
- * reciprocal march; both give the same segments, the default is faster. */
-enum { VCT_OPT_EXACT_TRACE = 1 };
+ * reciprocal march; both give the same segments, the default is faster.
+ * VCT_OPT_ADJ_VIEW_STRIDE = 2: lane order of the atomic adjoint
+ * (vct_trace_adjoint).  0 gives consecutive channels of one view to
+ * consecutive lanes; P > 0 gives lanes of one channel the views
+ * j*P mod V, so that simultaneous atomic adds of a wave land on different
+ * voxels.  P must be coprime with n_views; -1 (the default) picks
+ * P automatically.  Only the order of the float additions changes. */
+enum { VCT_OPT_EXACT_TRACE = 1, VCT_OPT_ADJ_VIEW_STRIDE = 2 };
 vct_status vct_plan_set_option(vct_plan* plan, int option, int value);
 
 /* ------------------------------------------------------- host-side tables */
@@ -197,6 +203,21 @@ vct_status vct_filter_sinogram_bhc(vct_plan* plan, const float* d_proj, int n_si
  * (synchronous); the scans of M/main.cpp:512-518 and :53-60. */
 vct_status vct_minmax(const float* d_x, int n, float* h_maxmin, void* stream);
 
+/* The npf side output of the MBFPA program, replacing lengthproj_TOMO
+ * (M/lengthproj_TOMO.cpp:14-185, called at M/main.cpp:407-419 and written
+ * to npf_float.raw by print_npf_float, :78-91): for every pixel inside the
+ * reconstruction circle, npf = sum over the views of the window of
+ * proj(view, channel hit by the source->pixel ray) * chord length, float
+ * accumulation in view order; pixels outside the circle get 0.
+ * d_proj: float [V/2*2][W], row j = view (start + j) mod V of the window
+ * that starts at start = (int)(z/(couch/V) - V/2 + 1e-8) (the sinogram the
+ * reference interpolates for slice z, M/main.cpp:352-372).  iso_det_cm is
+ * the isocentre-to-detector distance (57.6 cm, M/physParamsTOMO.h:29).
+ * Rejects (VCT_ERR_ARG) a window that starts before view 0, where the
+ * reference indexes its tables with negative views.  d_npf: float [N][N]. */
+vct_status vct_lengthproj(vct_plan* plan, const float* d_proj, double z_cm, double couch_speed_cm,
+                          double iso_det_cm, float* d_npf, void* stream);
+
 /* Pixel-driven fan backprojection, replacing reconstructImage
  * (F/reconstructImageTOMO.cpp:11-105), n_sino images. */
 vct_status vct_backproject(vct_plan* plan, const float* d_fltd, int n_sino, float* d_img,
@@ -239,13 +260,16 @@ typedef struct vct_batch_group {
 
 /* Batched replay of the cached segments, replacing reprojection_TOMO_2
  * (M/reprojection_TOMO.cpp:291-352) for many phantoms and geometries in one
- * persistent launch (up to 16 groups; lanes_per_row in {8,16,32,64}; blocks
- * <= 0 picks the persistent grid).  out[ray][b][m] = sum_seg in[vox][b][m]*len. */
+ * persistent launch (up to 16 groups; lanes_per_row in {8,16,32,64}, or 1
+ * when every group holds one phantom: each row is then split over 16 lanes;
+ * blocks <= 0 picks the persistent grid).
+ * out[ray][b][m] = sum_seg in[vox][b][m]*len / xfactor (the tracer's units). */
 vct_status vct_batch_paths(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row,
                            int blocks, void* stream);
 /* Batched transpose (no counterpart in the reference, whose iterative loop
  * backprojects with lengthproj_TOMO_2, M/lengthproj_TOMO.cpp:188-276):
- * out[vox][b][m] = sum_{rays through vox} in[ray][b][m] * len, atomic-free. */
+ * out[vox][b][m] = sum_{rays through vox} in[ray][b][m] * len / xfactor,
+ * atomic-free and deterministic. */
 vct_status vct_batch_adjoint(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row,
                              int blocks, void* stream);
 /* Polyenergetic sum for B phantoms x S spectra (as vct_spectral_sum):
@@ -279,6 +303,10 @@ vct_status vct_filter_sinogram_host(vct_ctx* ctx, const vct_geometry* g, const f
 /* Replaces reconstructImage (F/reconstructImageTOMO.cpp:11); host buffers. */
 vct_status vct_reconstruct_image_host(vct_ctx* ctx, const vct_geometry* g, const float* fltd,
                                       float* img);
+/* Replaces lengthproj_TOMO (M/lengthproj_TOMO.cpp:14) as vct_lengthproj,
+ * host buffers proj [V/2*2][W] -> npf [N][N]. */
+vct_status vct_lengthproj_host(vct_ctx* ctx, const vct_geometry* g, const float* proj, double z_cm,
+                               double couch_speed_cm, double iso_det_cm, float* npf);
 
 #ifdef __cplusplus
 }

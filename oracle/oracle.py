@@ -98,6 +98,9 @@ class Oracle:
         L.orc_backproject_ref.argtypes = [G, _f32p, _f32p]
         L.orc_backproject_f64.argtypes = [G, _f64p, _f64p]
         L.orc_adjoint_f64.argtypes = [G, _f32p, C.c_int, _f32p, _f32p, C.c_int, _f64p, _f64p]
+        for fn, out in (("orc_lengthproj_ref", _f32p), ("orc_lengthproj_f64", _f64p)):
+            getattr(L, fn).argtypes = [G, _f32p, C.c_double, C.c_double, C.c_double, out]
+            getattr(L, fn).restype = C.c_int
 
     # -- tables
     def view_tables(self, g: Geom):
@@ -227,6 +230,20 @@ class Oracle:
         return grad
 
 
+    def lengthproj_ref(self, g: Geom, proj, z_cm: float, couch_cm: float, iso_det: float = 57.6):
+        """npf [N,N] float (reference rounding); None when the view window is invalid."""
+        out = np.zeros((g.N, g.N), np.float32)
+        rc = self.lib.orc_lengthproj_ref(C.byref(g.c()), np.ascontiguousarray(proj, np.float32), z_cm, couch_cm,
+                                         iso_det, out)
+        return out if rc == 0 else None
+
+    def lengthproj_f64(self, g: Geom, proj, z_cm: float, couch_cm: float, iso_det: float = 57.6):
+        out = np.zeros((g.N, g.N), np.float64)
+        rc = self.lib.orc_lengthproj_f64(C.byref(g.c()), np.ascontiguousarray(proj, np.float32), z_cm, couch_cm,
+                                         iso_det, out)
+        return out if rc == 0 else None
+
+
 class RefMBFPA:
     """The reference's own MBFPA functions (oracle/_ref/libref_mbfpa.so)."""
 
@@ -242,6 +259,19 @@ class RefMBFPA:
                                             _f64p, _f64p, C.c_double, _f32p, _f32p, C.c_int,
                                             C.c_int, C.c_int, C.c_int]
         L.ref_reprojection_full.restype = C.c_int
+        L.ref_lengthproj.argtypes = [C.c_int, _f32p, _f32p, C.c_double, C.c_int, C.c_int, _f64p, _f64p,
+                                     C.c_double, C.c_double]
+        L.ref_lengthproj.restype = C.c_int
+
+    def lengthproj(self, g: Geom, proj, cosT, sinT, z_cm: float, couch_cm: float):
+        """The reference's lengthproj_TOMO: npf [N,N] (zeros outside the mask)."""
+        npf = np.zeros((g.N, g.N), np.float32)
+        p = np.ascontiguousarray(proj, np.float32).copy()
+        rc = self.lib.ref_lengthproj(g.N, npf, p, g.pix, g.W, g.V, np.ascontiguousarray(cosT, np.float64),
+                                     np.ascontiguousarray(sinT, np.float64), z_cm, couch_cm)
+        if rc != 0:
+            raise MemoryError("reference cache allocation failed")
+        return npf
 
     def attenuation(self, path: str, M: int = 6):
         att = np.zeros(M * 1024, np.float32); pdf = np.zeros(1024, np.float32)

@@ -2,8 +2,8 @@
 //
 // extern "C" entry points onto the reference's own MBFPA functions.  This
 // file is compiled together with the reference sources where they lie
-// (Release_MBFPA_CPU_human/{reprojection_TOMO,prior_weight_production,
-// misc_TOMO,mallocMD}.cpp) into oracle/_ref/libref_mbfpa.so by
+// (Release_MBFPA_CPU_human/{reprojection_TOMO,lengthproj_TOMO,
+// prior_weight_production,misc_TOMO,mallocMD}.cpp) into oracle/_ref/libref_mbfpa.so by
 // oracle/Makefile.  It only allocates the buffers the reference functions
 // expect and forwards the call; no reference code is restated here.
 #include <stdlib.h>
@@ -12,6 +12,8 @@
 #include "physParamsTOMO.h"
 #include "mallocMD.h"
 #include "reprojection_TOMO.h"
+#include "lengthproj_TOMO.h"
+#include "misc_TOMO.h"
 #include "virtual_projection.h"
 
 extern "C" {
@@ -112,6 +114,27 @@ int ref_segments(int N, float* dens, double pix, int W, int H, double* cosT, dou
         }
     }
     free(cacheM); free(cacheV); free(cacheS); free(proj);
+    return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// M/lengthproj_TOMO.cpp:14 (the npf pass of M/main.cpp:407-419): npf [N][N]
+// must be zero on entry (clear_npf, M/main.cpp:330); proj [H][W] is the
+// sinogram window; the mask is the reference's own prepareMaskData.
+// Returns 0, or -1 on allocation failure.
+int ref_lengthproj(int N, float* npf, float* proj, double pix, int W, int H, double* cosT, double* sinT,
+                   double z, double couch)
+{
+    const size_t cache = (size_t)N * N * H;
+    float* sm = (float*)calloc(cache, sizeof(float));
+    float* xd = (float*)calloc(cache, sizeof(float));
+    char* mask = prepareMaskData(N);
+    if (!sm || !xd || !mask) { free(sm); free(xd); free(mask); return -1; }
+    lengthproj_TOMO(0, N, npf, proj, pix, W, H, cosT, sinT, z, couch, mask, sm, xd, 1);
+    free(sm); free(xd); free(mask);
     return 0;
 }
 

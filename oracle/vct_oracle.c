@@ -734,3 +734,134 @@ void orc_adjoint_f64(const orc_geom* g, const float* dens, int M,
     free(priv);
     tables_free(&t);
 }
+
+/* ------------------------------------------------------------ npf (lengthproj) */
+
+/* View window of M/lengthproj_TOMO.cpp:35-49 (UsedReconAngle = 2,
+ * M/physParamsTOMO.h:67): start = (int)(z/pitch - V/2 + 1e-8), V/2*2 views,
+ * view index start + j - (start/V)*V, wrapped once.  Only start >= 0 keeps
+ * every index inside [0, V). */
+static int lp_window(int V, double z_cm, double couch_cm, int* base, int* count)
+{
+    double pitch = (double)(couch_cm / V);
+    double s = z_cm / pitch - V / 2 + 0.00000001;
+    if (!(s > -1.0) || !(s < 2147483647.0 - V)) return -1;
+    int start = (int)s;
+    *base = start - (start / V) * V;
+    *count = V / 2 * 2;
+    return 0;
+}
+
+/* One pixel-view term np * rr of M/lengthproj_TOMO.cpp:72-167 in the
+ * reference's operation order; returns 0 when the reference adds nothing
+ * (a NaN channel position). */
+static int lp_term(const orc_geom* g, double rx, double ry, double rz, double c, double s,
+                   const float* row, double iso_det, double* out)
+{
+    const double pix = g->pix;
+    const double sx = g->sid * s, sy = g->sid * c, sz = rz;
+    const double yslope = fabs(ry - sy), xslope = fabs(rx - sx);
+    double rr;
+    if (xslope > yslope) {
+        double xx0 = rx - 0.5 * pix;
+        double yy0 = (xx0 - sx) / (rx - sx) * (ry - sy) + sy;
+        double zz0 = (xx0 - sx) / (rx - sx) * (rz - sz) + sz;
+        double xx1 = rx + 0.5 * pix;
+        double yy1 = (xx1 - sx) / (rx - sx) * (ry - sy) + sy;
+        double zz1 = (xx1 - sx) / (rx - sx) * (rz - sz) + sz;
+        rr = sqrt((xx0 - xx1) * (xx0 - xx1) + (yy0 - yy1) * (yy0 - yy1) + (zz0 - zz1) * (zz0 - zz1));
+    } else {
+        /* :94-101: the far end reuses yy0 (so xx1 == xx0 and rr = pix) */
+        double yy0 = ry - 0.5 * pix;
+        double xx0 = (yy0 - sy) / (ry - sy) * (rx - sx) + sx;
+        double zz0 = (yy0 - sy) / (ry - sy) * (rz - sz) + sz;
+        double yy1 = ry + 0.5 * pix;
+        double xx1 = (yy0 - sy) / (ry - sy) * (rx - sx) + sx;
+        double zz1 = (yy0 - sy) / (ry - sy) * (rz - sz) + sz;
+        rr = sqrt((xx0 - xx1) * (xx0 - xx1) + (yy0 - yy1) * (yy0 - yy1) + (zz0 - zz1) * (zz0 - zz1));
+    }
+    /* detector hit on the arc of radius R about the source, :109-131 */
+    const double dx = g->det_src_iso * s, dy = g->det_src_iso * c;
+    const double R = g->det_radius;
+    const double aa = (sy - ry) / (sx - rx);
+    const double bb = -sx * (sy - ry) / (sx - rx) + sy;
+    const double A = 1.0 + aa * aa;
+    const double B = 2.0 * aa * bb - 2.0 * aa * dy - 2.0 * dx;
+    const double C = dx * dx + bb * bb + dy * dy - 2.0 * bb * dy - R * R;
+    const double px1 = (-B + sqrt(B * B - 4.0 * A * C)) / (2.0 * A);
+    const double py1 = aa * px1 + bb;
+    const double px2 = (-B - sqrt(B * B - 4.0 * A * C)) / (2.0 * A);
+    const double py2 = aa * px2 + bb;
+    double px, py;
+    if (px1 * px1 + py1 * py1 > px2 * px2 + py2 * py2) { px = px2; py = py2; }
+    else { px = px1; py = py1; }
+    /* channel position, :133-150 */
+    const double ox = -iso_det * s, oy = -iso_det * c;
+    const double dis = (px - ox) * (px - ox) + (py - oy) * (py - oy);
+    const double angle = acos(1.0 - dis / (2.0 * R * R));
+    const double rot_px = px * c - py * s;
+    const double delta = g->max_alpha / ((g->W - 1) / 2);
+    const double dj = rot_px > 0.0 ? g->iso_channel - angle / delta : g->iso_channel + angle / delta;
+    const int last = g->W - 1;
+    double np;
+    if (0 <= dj && dj < last) {   /* :152-158 */
+        int i0 = (int)dj;
+        double w = dj - (double)i0;
+        np = (1.0 - w) * row[i0] + w * row[i0 + 1];
+    } else if (0 > dj) {
+        np = row[0];
+    } else if (dj >= last) {
+        np = row[last];
+    } else {
+        return 0;
+    }
+    *out = np * rr;
+    return 1;
+}
+
+#define LP_BODY(ACC_T, ADD)                                                                 \
+    int base, count;                                                                        \
+    if (lp_window(g->V, z_cm, couch_cm, &base, &count) != 0) return -1;                     \
+    const int N = g->N, V = g->V, W = g->W;                                                 \
+    double* cosT = (double*)malloc(sizeof(double) * V);                                     \
+    double* sinT = (double*)malloc(sizeof(double) * V);                                     \
+    uint8_t* mask = (uint8_t*)malloc((size_t)N * N);                                        \
+    orc_view_tables(g, cosT, sinT);                                                         \
+    orc_mask(N, mask);                                                                      \
+    _Pragma("omp parallel for schedule(dynamic, 4)")                                        \
+    for (int jy = 0; jy < N; jy++) {                                                        \
+        double ry = N * g->pix * 0.5 - (jy + 0.5) * g->pix;                                 \
+        for (int jx = 0; jx < N; jx++) {                                                    \
+            ACC_T acc = 0;                                                                  \
+            if (mask[jy * N + jx] != 0) {                                                   \
+                double rx = -N * g->pix * 0.5 + (jx + 0.5) * g->pix;                        \
+                for (int j = 0; j < count; j++) {                                           \
+                    int jr = base + j;                                                      \
+                    if (jr >= V) jr -= V;                                                   \
+                    double t;                                                               \
+                    if (lp_term(g, rx, ry, z_cm, cosT[jr], sinT[jr], proj + (size_t)j * W,  \
+                                iso_det, &t))                                               \
+                        ADD;                                                                \
+                }                                                                           \
+            }                                                                               \
+            npf[jy * N + jx] = acc;                                                         \
+        }                                                                                   \
+    }                                                                                       \
+    free(cosT);                                                                             \
+    free(sinT);                                                                             \
+    free(mask);                                                                             \
+    return 0;
+
+int orc_lengthproj_ref(const orc_geom* g, const float* proj, double z_cm, double couch_cm,
+                       double iso_det, float* npf)
+{
+    /* M/lengthproj_TOMO.cpp:14-185: float accumulator, views ascending */
+    LP_BODY(float, acc += (float)t)
+}
+
+int orc_lengthproj_f64(const orc_geom* g, const float* proj, double z_cm, double couch_cm,
+                       double iso_det, double* npf)
+{
+    LP_BODY(double, acc += t)
+}
+#undef LP_BODY

@@ -104,6 +104,19 @@ void orc_adjoint_f64(const orc_geom* g, const float* dens, int M,
                      const float* att, const float* pdf, int NE,
                      const double* gI /* [V][W] */, double* grad /* [M][N][N] */);
 
+/* Chord-weighted pixel backprojection of a normalised sinogram, the npf
+ * side output of the MBFPA program.  proj: float [V][W] (row j is the j-th
+ * view of the window that starts at view (int)(z/(couch/V) - V/2 + 1e-8)).
+ * npf: [N][N], masked pixels written, others set to 0.  iso_det is the
+ * isocentre-to-detector distance (57.6 cm).  Returns 0, or -1 when the view
+ * window starts before view 0 (the reference then indexes its tables with
+ * negative views).  _ref accumulates in float as the reference, _f64 in
+ * double. */
+int orc_lengthproj_ref(const orc_geom* g, const float* proj, double z_cm, double couch_cm,
+                       double iso_det, float* npf);
+int orc_lengthproj_f64(const orc_geom* g, const float* proj, double z_cm, double couch_cm,
+                       double iso_det, double* npf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,6 +21,10 @@ Outputs (all data, no code from the reference):
   ref_constmap_rows.npz         rows of prepareConstMap for both filters
   icru110_120kV_sigma0.f32.xz   the reference's noise-free normalised 120 kV sinogram
                                 of the ICRU110 phantom (the KAT-M input with sigma = 0)
+  ref_npf.npz                   npf images of the reference's lengthproj_TOMO (small
+                                geometries, whole; full geometry, sample rows; the
+                                full-image SHA-256 go to kat_hashes.json)
+    (--npf-only regenerates ref_npf.npz and its hashes alone)
 """
 from __future__ import annotations
 
@@ -208,5 +212,48 @@ def bhc_goldens():
         json.dump(hashes, f, indent=1)
 
 
+NPF_SMALL = [("n64", 64, 0.7, 80), ("n128", 128, 0.35, 160)]
+NPF_WINDOWS = [(0.4, 0.8), (1.3, 0.8), (2.0, 1.2)]  # (z cm, couch cm/rotation): window bases 0, 100, 133 at V=800
+
+
+def npf_input(V: int, seed: int) -> np.ndarray:
+    """The small-case npf input: uniform(0.05, 1) float32 [V][609] from PCG64(seed)."""
+    return np.random.default_rng(seed).uniform(0.05, 1.0, (V, 609)).astype(np.float32)
+
+
+def npf_goldens():
+    """The reference's lengthproj_TOMO (oracle/_ref/libref_mbfpa.so):
+    whole npf images for two small geometries with seeded uniform inputs, and
+    for the full geometry on the noise-free 120 kV sinogram the SHA-256 of
+    npf and rows 0, 128, 256, 384 of it, for three view windows."""
+    ref = RefMBFPA()
+    orc = Oracle()
+    out = {}
+    for k, (name, N, pix, V) in enumerate(NPF_SMALL):
+        g = Geom(N=N, pix=pix, V=V)
+        c, s = orc.view_tables(g)
+        proj = npf_input(V, 100 + k)
+        for z, couch in NPF_WINDOWS:
+            out[f"{name}_z{z}_c{couch}"] = ref.lengthproj(g, proj, c, s, z, couch)
+    path = os.path.join(HERE, "icru110_120kV_sigma0.f32.xz")
+    sino = np.frombuffer(lzma.decompress(open(path, "rb").read()), np.float32).reshape(800, 609)
+    g = Geom()
+    c, s = orc.view_tables(g)
+    with open(os.path.join(HERE, "kat_hashes.json")) as f:
+        hashes = json.load(f)
+    for z, couch in NPF_WINDOWS:
+        npf = ref.lengthproj(g, sino, c, s, z, couch)
+        key = f"npf_sigma0_z{z}_c{couch}"
+        hashes[key + "_sha256"] = hashlib.sha256(npf.tobytes()).hexdigest()
+        out[key + "_rows"] = npf[[0, 128, 256, 384]].copy()
+    np.savez_compressed(os.path.join(HERE, "ref_npf.npz"), **out)
+    with open(os.path.join(HERE, "kat_hashes.json"), "w") as f:
+        json.dump(hashes, f, indent=1)
+
+
 if __name__ == "__main__":
-    main()
+    if "--npf-only" in sys.argv:
+        npf_goldens()
+    else:
+        main()
+        npf_goldens()

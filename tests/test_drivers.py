@@ -111,3 +111,28 @@ def test_mbfpa_then_fbp_pipeline(gpu, tmp_path, golden, oracle):
     img = np.fromfile(d / "FBP_virtual_projection_512x512_human.raw", np.float32).reshape(512, 512)
     ref = oracle.fbp_ref(Geom(), golden.sigma0_120kv(), 2)
     assert rel_max(img, ref) < 1e-5
+
+
+@pytest.mark.gpu
+def test_mbfpa_driver_npf_from_sinogram(gpu, tmp_path, golden, oracle):
+    """VCT_NPF_SINOGRAM: npf_float.raw is the chord-weighted backprojection of
+    the given window for slice z = reconstruction start z (here 1.3 cm, couch
+    0.8 cm/rotation, window starting at view 100)."""
+    d = _mbfpa_dir(tmp_path, golden, noise="0.0")
+    txt = (d / "in.txt").read_text()
+    txt = txt.replace("//Couch speed [cm/rotation] // 0.4=fine, 0.8=normal, 1.2=coarse\n0.0",
+                      "//Couch speed [cm/rotation] // 0.4=fine, 0.8=normal, 1.2=coarse\n0.8")
+    txt = txt.replace("// reconstruction start z [cm]\n0.0", "// reconstruction start z [cm]\n1.3")
+    (d / "in.txt").write_text(txt)
+    win = golden.sigma0_120kv()
+    win.tofile(d / "window.raw")
+    r = run("mvct_mbfpa", ["in.txt"], d, env={"VCT_NPF_SINOGRAM": str(d / "window.raw")})
+    assert r.returncode == 0, r.stderr
+    npf = np.fromfile(d / "npf_float.raw", np.float32).reshape(512, 512)
+    ref = oracle.lengthproj_ref(Geom(), win, 1.3, 0.8)
+    assert rel_max(npf, ref) < 1e-5
+    # the shipped input (no couch motion) has no valid window: a clear error
+    d2 = _mbfpa_dir(tmp_path / "b", golden, noise="0.0")
+    win.tofile(d2 / "window.raw")
+    r = run("mvct_mbfpa", ["in.txt"], d2, env={"VCT_NPF_SINOGRAM": str(d2 / "window.raw")})
+    assert r.returncode == 1 and "view window" in r.stderr

@@ -153,3 +153,66 @@ def test_autograd_gradcheck_small(gpu):
     fd = ((fp - fm) / (2 * eps)).item()
     ad = (x.grad.double() * d.double()).sum().item()
     assert ad == pytest.approx(fd, rel=2e-3)
+
+
+@pytest.mark.parametrize("stride", [0, 7, -1])
+def test_adjoint_lane_orders_agree_with_fp64(gpu, golden, oracle, stride):
+    """The atomic adjoint in view-major order (0), channel-major with view
+    stride 7 and the automatic stride changes only the order of the float
+    additions: every order matches the FP64 adjoint."""
+    import torch
+    from virtualct_amd._lib import OPT_ADJ_VIEW_STRIDE
+    ops = _ops()
+    N, pix, V = 128, 0.35, 160
+    ph, tabs = _problem(golden, gpu, N, pix, V)
+    dens = ph.astype(np.float32) / np.float32(1000)
+    att, pdf, _ = tabs[0]
+    plan = ops.Plan(_geom(N, pix, V))  # own plan: the option is per plan
+    plan.set_option(OPT_ADJ_VIEW_STRIDE, stride)
+    rng = np.random.default_rng(21)
+    gI = rng.normal(size=(V, 609))
+    x = _t(dens, gpu).requires_grad_(True)
+    I = ops.mbfpa_intensity(x, plan, _t(att, gpu), _t(pdf, gpu))
+    (I[0] * _t(gI.astype(np.float32), gpu)).sum().backward()
+    ref = oracle.adjoint_f64(Geom(N=N, pix=pix, V=V), dens, att, pdf, gI)
+    got = x.grad.cpu().numpy()
+    assert rel_l2(got, ref) < 1e-4
+    assert rel_max(got, ref) < 1e-4
+
+
+def test_adjoint_view_stride_rejects_common_factor(gpu):
+    from virtualct_amd._lib import OPT_ADJ_VIEW_STRIDE
+    ops = _ops()
+    plan = ops.Plan(_geom(64, 0.7, 80))
+    with pytest.raises(RuntimeError):
+        plan.set_option(OPT_ADJ_VIEW_STRIDE, 10)  # gcd(10, 80) = 10
+    with pytest.raises(RuntimeError):
+        plan.set_option(OPT_ADJ_VIEW_STRIDE, -2)
+
+
+def test_sysmat_adjoint_mode_matches_fp64_and_is_deterministic(gpu, golden, oracle):
+    """adjoint_mode "sysmat": the gradient gathers over the cached voxel-major
+    system matrix; it matches the FP64 adjoint, keeps the sign test of the
+    tracer (negative densities get zero) and repeats bit for bit."""
+    import torch
+    ops = _ops()
+    N, pix, V = 128, 0.35, 160
+    ph, tabs = _problem(golden, gpu, N, pix, V)
+    dens = ph.astype(np.float32) / np.float32(1000)
+    dens[2, 60:64, 60:64] = -0.5  # clamped voxels: zero gradient
+    att, pdf, _ = tabs[1]
+    plan = ops.Plan(_geom(N, pix, V))
+    plan.adjoint_mode = "sysmat"
+    rng = np.random.default_rng(22)
+    gI = rng.normal(size=(V, 609))
+    grads = []
+    for _ in range(2):
+        x = _t(dens, gpu).requires_grad_(True)
+        I = ops.mbfpa_intensity(x, plan, _t(att, gpu), _t(pdf, gpu))
+        (I[0] * _t(gI.astype(np.float32), gpu)).sum().backward()
+        grads.append(x.grad.cpu().numpy())
+    assert np.array_equal(grads[0], grads[1])
+    ref = oracle.adjoint_f64(Geom(N=N, pix=pix, V=V), dens, att, pdf, gI)
+    assert rel_l2(grads[0], ref) < 1e-4
+    assert rel_max(grads[0], ref) < 1e-4
+    assert np.all(grads[0][2, 60:64, 60:64] == 0)

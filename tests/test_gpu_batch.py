@@ -174,3 +174,45 @@ def test_batch_forward_fbp_uneven_sizes(gpu):
         _ = sum(float(i.sum()) for i in eager_imgs)  # eager work between replays
         for a, b in zip(i2, eager_imgs):
             assert torch.allclose(a, b, rtol=1e-5, atol=1e-7)
+
+
+def test_single_phantom_split_gather_equals_lane_gather(gpu, small_setup):
+    """lanes_per_row 1 (rows split over 16 lanes) gives the FP64 sums of the
+    one-lane-per-phantom gather, forward and adjoint."""
+    import torch
+    ops, batch, _ = _mods()
+    plan, sm = small_setup
+    x = batch.interleave(_phantoms(1, 128, 0.35, gpu))
+    L1 = torch.empty((plan.V * plan.W, 1, 8), device=gpu)
+    L8 = torch.empty_like(L1)
+    batch.batch_paths([(sm, x, L1, 1)], 6, 1)
+    batch.batch_paths([(sm, x, L8, 1)], 6, 8)
+    assert rel_max(L1.cpu().numpy(), L8.cpu().numpy()) < 1e-6
+    g = torch.Generator(device=gpu).manual_seed(8)
+    w = torch.zeros((plan.V * plan.W, 1, 8), device=gpu)
+    w[..., :6] = torch.randn((plan.V * plan.W, 1, 6), generator=g, device=gpu)
+    o1 = torch.empty((128 * 128, 1, 8), device=gpu)
+    o8 = torch.empty_like(o1)
+    batch.batch_adjoint([(sm, w, o1, 1)], 6, 1)
+    batch.batch_adjoint([(sm, w, o8, 1)], 6, 8)
+    assert rel_l2(o1.cpu().numpy(), o8.cpu().numpy()) < 1e-6
+    assert torch.all(o1[..., 6:] == 0)
+    with pytest.raises(RuntimeError):
+        batch.batch_paths([(sm, batch.interleave(_phantoms(2, 128, 0.35, gpu)),
+                            torch.empty((plan.V * plan.W, 2, 8), device=gpu), 2)], 6, 1)
+
+
+def test_gather_applies_xfactor(gpu):
+    """The gathers return path sums in the tracer's units (divided by xfactor)."""
+    import torch
+    ops, batch, _ = _mods()
+    g = _geom(64, 0.7, 80)
+    g.xfactor = 4.0
+    plan = ops.Plan(g)
+    sm = batch.SystemMatrix(plan)
+    x = _phantoms(1, 64, 0.7, gpu)
+    L = torch.empty((plan.V * plan.W, 1, 8), device=gpu)
+    batch.batch_paths([(sm, batch.interleave(x), L, 1)], 6)
+    pk, dt = ops.pack_densities(plan, x[0].contiguous())
+    ref = ops.trace_paths(plan, pk, dt, 6).reshape(-1, 8)
+    assert rel_max(L[:, 0].cpu().numpy(), ref.cpu().numpy()) < 2e-6

@@ -209,3 +209,75 @@ def test_beam_hardening_matches_reference_tool(oracle, golden, alpha, beta):
     key = f"bhc_sigma0_a{alpha}_b{beta}"
     assert sha(out) == golden.hashes()[key + "_sha256"]
     assert f"{mn0:f} {mn1:f}" == golden.hashes()[key + "_stdout"]
+
+
+NPF_SMALL = [("n64", 64, 0.7, 80), ("n128", 128, 0.35, 160)]
+NPF_WINDOWS = [(0.4, 0.8), (1.3, 0.8), (2.0, 1.2)]
+
+
+def npf_input(V, seed):
+    # the inputs tests/golden/make_golden.py gave the reference
+    return np.random.default_rng(seed).uniform(0.05, 1.0, (V, 609)).astype(np.float32)
+
+
+def test_lengthproj_matches_reference_small(oracle, golden):
+    t = golden.npz("ref_npf.npz")
+    for k, (name, N, pix, V) in enumerate(NPF_SMALL):
+        g = Geom(N=N, pix=pix, V=V)
+        proj = npf_input(V, 100 + k)
+        for z, couch in NPF_WINDOWS:
+            np.testing.assert_array_equal(oracle.lengthproj_ref(g, proj, z, couch), t[f"{name}_z{z}_c{couch}"])
+
+
+def test_lengthproj_matches_reference_full(oracle, golden):
+    """npf of the noise-free 120 kV sinogram at 512 x 512 / 800 x 609, three
+    view windows: bit-identical to the reference (SHA-256 of the image)."""
+    sino = golden.sigma0_120kv()
+    h = golden.hashes()
+    for z, couch in NPF_WINDOWS:
+        npf = oracle.lengthproj_ref(Geom(), sino, z, couch)
+        assert sha(npf) == h[f"npf_sigma0_z{z}_c{couch}_sha256"]
+
+
+def test_lengthproj_window_must_start_at_view_zero(oracle):
+    g = Geom(N=64, pix=0.7, V=80)
+    proj = npf_input(80, 1)
+    assert oracle.lengthproj_ref(g, proj, 0.0, 0.8) is None   # starts at view -40
+    assert oracle.lengthproj_ref(g, proj, 0.3, 0.0) is None   # no couch motion: 0/0
+    assert oracle.lengthproj_ref(g, proj, 0.4, 0.8) is not None
+
+
+def test_lengthproj_closed_form(oracle):
+    """proj[j][c] = c makes every sample the channel position itself, so
+    npf = sum_views chord * channel with the channel from the ray angle,
+    written here from the geometry alone: theta = atan(|x'| / (sid - y'))
+    in the frame where the source sits on the y' axis, channel = iso -/+
+    theta / (max_alpha / 304) on the side of x'; chord = pix * sec of the
+    ray's angle to the major axis when |dx| > |dy|, else pix (the
+    reference's y-branch reuses the near end, M/lengthproj_TOMO.cpp:98)."""
+    g = Geom(N=64, pix=0.7, V=80)
+    V, N = g.V, g.N
+    proj = np.tile(np.arange(609, dtype=np.float32), (V, 1))
+    z, couch = 1.3, 0.8
+    got = oracle.lengthproj_ref(g, proj, z, couch).astype(np.float64)
+    start = int(z / (couch / V) - V // 2 + 1e-8)
+    views = (start % V + np.arange(V)) % V
+    beta = (np.arange(V) * 360.0 / V) * np.pi / 180.0
+    c, s = np.cos(beta[views]), np.sin(beta[views])
+    jy, jx = np.mgrid[0:N, 0:N]
+    ry = N * g.pix / 2 - (jy + 0.5) * g.pix
+    rx = -N * g.pix / 2 + (jx + 0.5) * g.pix
+    mask = oracle.mask(N).astype(bool)
+    sx, sy = g.sid * s[:, None, None], g.sid * c[:, None, None]
+    ddx, ddy = rx[None] - sx, ry[None] - sy
+    chord = np.where(np.abs(ddx) > np.abs(ddy), g.pix * np.sqrt(1 + (ddy / ddx) ** 2), g.pix)
+    xl = rx[None] * c[:, None, None] - ry[None] * s[:, None, None]
+    yl = rx[None] * s[:, None, None] + ry[None] * c[:, None, None]
+    theta = np.arctan(np.abs(xl) / (g.sid - yl))
+    ch = np.where(xl > 0, 304.5 - theta / (g.max_alpha / 304), 304.5 + theta / (g.max_alpha / 304))
+    expect = (chord * np.clip(ch, 0, 608)).sum(0) * mask
+    assert np.all(got[~mask] == 0)
+    assert rel_max(got, expect) < 2e-6
+    # a uniform sinogram gives the summed chord lengths
+    ones = oracle.lengthproj_f64(g, np.ones((V, 609), np.float32), z, couch)
+    np.testing.assert_allclose(ones, chord.sum(0) * mask, rtol=1e-12, atol=0)

@@ -18,6 +18,7 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of mem
           4: "I/O error", 5: "unsupported", 6: "internal limit"}
 DENS_U16, DENS_F32 = 0, 1
 FILTER_RAMLAK, FILTER_SHEPP_LOGAN = 1, 2
+OPT_EXACT_TRACE, OPT_ADJ_VIEW_STRIDE = 1, 2
 
 
 class VctError(RuntimeError):
@@ -72,6 +73,7 @@ _SIGS = {
                                      C.c_uint64, C.c_uint32, _P, _P, _P]),
     "vct_filter_sinogram": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "vct_backproject": (C.c_int, [_P, _P, C.c_int, _P, _P]),
+    "vct_lengthproj": (C.c_int, [_P, _P, C.c_double, C.c_double, C.c_double, _P, _P]),
     "vct_beam_hardening": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, _P, _P]),
     "vct_filter_sinogram_bhc": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P, _P]),
     "vct_minmax": (C.c_int, [_P, C.c_int, _P, _P]),
@@ -80,6 +82,7 @@ _SIGS = {
     "vct_gaussian_noise": (C.c_int, [_P, _P, C.c_int, C.c_double, C.c_uint64, _P]),
     "vct_filter_sinogram_host": (C.c_int, [_P, C.POINTER(Geometry), _P, C.c_int, C.c_int, _P]),
     "vct_reconstruct_image_host": (C.c_int, [_P, C.POINTER(Geometry), _P, _P]),
+    "vct_lengthproj_host": (C.c_int, [_P, C.POINTER(Geometry), _P, C.c_double, C.c_double, C.c_double, _P]),
 }
 
 

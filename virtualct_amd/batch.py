@@ -73,6 +73,10 @@ def deinterleave(x: torch.Tensor, M: int, N: int) -> torch.Tensor:
 
 
 def lanes_for(B: int) -> int:
+    """Lanes per row of the gather: 1 (one phantom, rows split over 16 lanes)
+    or the power of two in 8..64 that holds B phantoms."""
+    if B == 1:
+        return 1
     lanes = 8
     while lanes < B:
         lanes *= 2
@@ -166,7 +170,7 @@ class BatchedSolver:
         base = lanes_for(self.B)
         cus = torch.cuda.get_device_properties(self.x.device).multi_processor_count
         out = []
-        for lanes in sorted({base, min(64, base * 2)}):
+        for lanes in sorted({base, 8 if base == 1 else min(64, base * 2)}):
             for per_cu in (2, 4, 8, 16):
                 out.append({"lanes": lanes, "blocks": cus * per_cu})
         return out

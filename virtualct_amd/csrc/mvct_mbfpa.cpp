@@ -10,9 +10,14 @@
 //          (int16 [M][N][N], mg/cm^3; M/main.cpp:421-429)
 //   writes reprojection_float.raw (float32 [V][W], max-normalised with
 //          Gaussian detector noise; M/main.cpp:508-530) and npf_float.raw
-//          (zeros when the input sinogram is "*", M/main.cpp:419)
+//          (M/main.cpp:407-419: zeros when the input sinogram is "*")
 // Environment overrides (argv is unchanged): VCT_PHANTOM (density file),
-// VCT_NUM_MATERIAL (6 or 8), VCT_SEED (noise seed), VCT_DEVICE.
+// VCT_NUM_MATERIAL (6 or 8), VCT_SEED (noise seed), VCT_DEVICE, and
+// VCT_NPF_SINOGRAM: a normalised float32 [V/2*2][W] sinogram window from
+// which npf_float.raw is computed for slice z = reconstruction start z with
+// the txt's couch speed (the reference's measured-data loader, which would
+// build that window from raw detector and air-scan files, is not part of
+// this build).
 // The iterative-reconstruction leftovers (ReconImageRaw/, M/main.cpp:558-624)
 // are not produced.  Exit status is 0 on success and on the reference's own
 // soft failures (missing info file, usage), 1 on a library error.
@@ -169,32 +174,6 @@ int main(int argc, char* argv[])
     std::printf("%d \n", H / used_recon_angle * 2);
     std::fprintf(stderr, "Iteration loop, %d step,\t\n", 0);
 
-    // npf side output, M/main.cpp:407-419 (zeros without an input sinogram)
-    const size_t NN = (size_t)in.recon_size * in.recon_size;
-    if (!have_sino) {
-        std::vector<float> zeros(NN, 0.f);
-        if (FILE* fp = std::fopen("npf_float.raw", "wb")) {
-            std::fwrite(zeros.data(), sizeof(float), NN, fp);
-            std::fclose(fp);
-        }
-    } else {
-        std::fprintf(stderr, "WARNING: npf_float.raw is not produced for a measured sinogram by this build\n");
-    }
-
-    // densities, M/main.cpp:421-429 and initial_image_def :93-109
-    std::vector<int16_t> ph((size_t)M * NN, 0);
-    if (FILE* fp = std::fopen(phantom_file.c_str(), "rb")) {
-        size_t got = std::fread(ph.data(), sizeof(int16_t), ph.size(), fp);
-        std::fclose(fp);
-        if (got != ph.size()) std::fprintf(stderr, "WARNING: %s is short (%zu of %zu values)\n", phantom_file.c_str(), got, ph.size());
-    } else {
-        std::fprintf(stderr, "ERROR: cannot open %s\n", phantom_file.c_str());
-        return 1;
-    }
-    std::vector<float> dens(ph.size());
-    for (size_t i = 0; i < ph.size(); i++) dens[i] = (float)ph[i] / 1000;
-    std::fprintf(stderr, "Iteration loop, %d step,\t\n", 1);
-
     vct_geometry g;
     vct_geometry_default(&g);
     g.recon_size = in.recon_size;
@@ -207,6 +186,52 @@ int main(int argc, char* argv[])
     vct_ctx* ctx = nullptr;
     st = vct_init(device, &ctx);
     if (st != VCT_OK) return fail("vct_init", st);
+
+    // npf side output, M/main.cpp:407-419: zeros without an input sinogram;
+    // from the normalised window rows named by VCT_NPF_SINOGRAM otherwise
+    const size_t NN = (size_t)in.recon_size * in.recon_size;
+    const char* env_npf = std::getenv("VCT_NPF_SINOGRAM");
+    std::vector<float> npf(NN, 0.f);
+    if (env_npf) {
+        const int rows = H / used_recon_angle * 2;
+        std::vector<float> win((size_t)rows * W);
+        FILE* fp = std::fopen(env_npf, "rb");
+        const size_t got = fp ? std::fread(win.data(), sizeof(float), win.size(), fp) : 0;
+        if (fp) std::fclose(fp);
+        if (got != win.size()) {
+            std::fprintf(stderr, "ERROR: %s must hold float32 [%d][%d]\n", env_npf, rows, W);
+            vct_terminate(ctx);
+            return 1;
+        }
+        st = vct_lengthproj_host(ctx, &g, win.data(), zstart, in.couch_speed, 57.6, npf.data());
+        if (st != VCT_OK) {
+            vct_terminate(ctx);
+            return fail("lengthproj", st);
+        }
+    } else if (have_sino) {
+        std::fprintf(stderr, "WARNING: the measured-data loader is not part of this build; npf_float.raw is "
+                             "written as zeros (set VCT_NPF_SINOGRAM to a normalised float32 sinogram)\n");
+    }
+    if (FILE* fp = std::fopen("npf_float.raw", "wb")) {
+        std::fwrite(npf.data(), sizeof(float), NN, fp);
+        std::fclose(fp);
+    }
+
+    // densities, M/main.cpp:421-429 and initial_image_def :93-109
+    std::vector<int16_t> ph((size_t)M * NN, 0);
+    if (FILE* fp = std::fopen(phantom_file.c_str(), "rb")) {
+        size_t got = std::fread(ph.data(), sizeof(int16_t), ph.size(), fp);
+        std::fclose(fp);
+        if (got != ph.size()) std::fprintf(stderr, "WARNING: %s is short (%zu of %zu values)\n", phantom_file.c_str(), got, ph.size());
+    } else {
+        std::fprintf(stderr, "ERROR: cannot open %s\n", phantom_file.c_str());
+        vct_terminate(ctx);
+        return 1;
+    }
+    std::vector<float> dens(ph.size());
+    for (size_t i = 0; i < ph.size(); i++) dens[i] = (float)ph[i] / 1000;
+    std::fprintf(stderr, "Iteration loop, %d step,\t\n", 1);
+
     std::vector<float> sino((size_t)W * H, 0.f);
     st = vct_reprojection(ctx, &g, dens.data(), M, att.data(), pdf.data(), NE, sino.data());
     if (st != VCT_OK) return fail("vct_reprojection", st);

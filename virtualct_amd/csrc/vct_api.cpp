@@ -2,11 +2,13 @@
 // spectrum/attenuation and filter-matrix tables, and the C ABI entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -44,6 +46,15 @@ static vct_status check_geometry(const vct_geometry* g)
     return VCT_OK;
 }
 
+// Smallest stride >= ceil(V/64) coprime with V: the 64 lanes of a wave then
+// take views spread over the whole turn.
+static int auto_view_stride(int V)
+{
+    int P = std::max(1, (V + 63) / 64);
+    while (std::gcd(P, V) != 1) P++;
+    return P;
+}
+
 TraceArgs make_trace_args(const vct_plan* p)
 {
     TraceArgs a{};
@@ -73,6 +84,7 @@ TraceArgs make_trace_args(const vct_plan* p)
     a.ray_offset = 0;
     a.n_rays = g.n_views * g.n_channels;
     a.exact = p->exact_trace;
+    a.adj_vstride = 0;
     return a;
 }
 
@@ -316,6 +328,12 @@ vct_status vct_plan_set_option(vct_plan* p, int option, int value)
     if (!p) return set_error(VCT_ERR_ARG, "plan is NULL");
     switch (option) {
     case VCT_OPT_EXACT_TRACE: p->exact_trace = value ? 1 : 0; return VCT_OK;
+    case VCT_OPT_ADJ_VIEW_STRIDE:
+        if (value < -1 || (value > 0 && std::gcd(value, p->g.n_views) != 1))
+            return set_error(VCT_ERR_ARG, "view stride %d must be -1, 0 or coprime with %d views", value,
+                             p->g.n_views);
+        p->adj_vstride = value;
+        return VCT_OK;
     default: return set_error(VCT_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -625,6 +643,44 @@ vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* 
     return launch_backproject(a, n_sino, pick_stream(p, stream));
 }
 
+vct_status vct_lengthproj(vct_plan* p, const float* d_proj, double z_cm, double couch_speed_cm, double iso_det_cm,
+                          float* d_npf, void* stream)
+{
+    clear_error();
+    if (!p || !d_proj || !d_npf) return set_error(VCT_ERR_ARG, "NULL argument");
+    const vct_geometry& g = p->g;
+    const int V = g.n_views;
+    // view window of M/lengthproj_TOMO.cpp:35-49 (UsedReconAngle = 2)
+    const double pitch = (double)(couch_speed_cm / V);
+    const double s = z_cm / pitch - V / 2 + 0.00000001;
+    if (!(s > -1.0) || !(s < 2147483647.0 - V))
+        return set_error(VCT_ERR_ARG,
+                         "view window starts at %g (z %g cm, couch %g cm/rotation): it must not precede view 0", s,
+                         z_cm, couch_speed_cm);
+    if (!(iso_det_cm > 0.0) || !std::isfinite(iso_det_cm))
+        return set_error(VCT_ERR_ARG, "isocentre-to-detector distance must be > 0");
+    const int start = (int)s;
+    LengthprojArgs a{};
+    a.proj = d_proj;
+    a.cosT = p->d_cos;
+    a.sinT = p->d_sin;
+    a.npf = d_npf;
+    a.N = g.recon_size;
+    a.W = g.n_channels;
+    a.V = V;
+    a.base = start - (start / V) * V;
+    a.count = V / 2 * 2;
+    a.pix = g.pixel_cm;
+    a.rz = z_cm;
+    a.sid = g.sid_cm;
+    a.det_src_iso = g.det_src_iso_cm;
+    a.det_radius = g.det_radius_cm;
+    a.iso_det = iso_det_cm;
+    a.iso_channel = g.iso_channel;
+    a.delta = g.max_alpha / p->half;
+    return launch_lengthproj(a, pick_stream(p, stream));
+}
+
 vct_status vct_trace_adjoint(vct_plan* p, const void* d_packed, int dens_type, int M, const float* d_dIdL,
                              const float* d_g, float* d_grad, void* stream)
 {
@@ -638,6 +694,7 @@ vct_status vct_trace_adjoint(vct_plan* p, const void* d_packed, int dens_type, i
     a.g = d_g;
     a.grad = d_grad;
     a.out_scale = (dens_type == VCT_DENS_U16 ? 1.0 / 1000.0 : 1.0) / p->g.xfactor;
+    a.adj_vstride = p->adj_vstride >= 0 ? p->adj_vstride : auto_view_stride(p->g.n_views);
     return launch_trace(a, VCT_DENS_F32, M, true, pick_stream(p, stream));
 }
 
@@ -777,6 +834,33 @@ vct_status vct_reconstruct_image_host(vct_ctx* ctx, const vct_geometry* g, const
     if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
     if (d_f) (void)hipFree(d_f);
     if (d_i) (void)hipFree(d_i);
+    vct_plan_destroy(p);
+    return st;
+}
+
+vct_status vct_lengthproj_host(vct_ctx* ctx, const vct_geometry* g, const float* proj, double z_cm,
+                               double couch_speed_cm, double iso_det_cm, float* npf)
+{
+    clear_error();
+    if (!ctx || !g || !proj || !npf) return set_error(VCT_ERR_ARG, "NULL argument");
+    vct_plan* p = nullptr;
+    VCT_CHECK(vct_plan_create(ctx, g, &p));
+    const size_t nr = (size_t)(g->n_views / 2 * 2) * g->n_channels;
+    const size_t NN = (size_t)g->recon_size * g->recon_size;
+    hipStream_t s = ctx->stream;
+    float *d_p = nullptr, *d_n = nullptr;
+    vct_status st = VCT_OK;
+    auto hip = [&](hipError_t e, const char* what) {
+        if (st == VCT_OK && e != hipSuccess) st = set_error(VCT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    };
+    hip(hipMalloc(&d_p, nr * sizeof(float)), "hipMalloc");
+    hip(hipMalloc(&d_n, NN * sizeof(float)), "hipMalloc");
+    if (st == VCT_OK) hip(hipMemcpyAsync(d_p, proj, nr * sizeof(float), hipMemcpyHostToDevice, s), "copy");
+    if (st == VCT_OK) st = vct_lengthproj(p, d_p, z_cm, couch_speed_cm, iso_det_cm, d_n, s);
+    if (st == VCT_OK) hip(hipMemcpyAsync(npf, d_n, NN * sizeof(float), hipMemcpyDeviceToHost, s), "copy");
+    if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
+    if (d_p) (void)hipFree(d_p);
+    if (d_n) (void)hipFree(d_n);
     vct_plan_destroy(p);
     return st;
 }

@@ -26,6 +26,7 @@ struct vct_sysmat {
     int64_t nseg = 0;
     int nrays = 0;
     int NN = 0;
+    float scale = 1.f;            // 1/xfactor: the gathers return path sums in the tracer's units
     uint32_t* row_ptr = nullptr;  // [nrays+1]
     uint2* csr = nullptr;         // [nseg] (voxel, float bits of length)
     uint32_t* col_ptr = nullptr;  // [NN+1]
@@ -107,7 +108,7 @@ struct GatherGroup {
     int B;
     int rows;
     int item_begin;       // first work item of this group
-    int pad;
+    float scale;          // applied to every output of the group (1/xfactor)
 };
 
 struct GatherArgs {
@@ -115,8 +116,16 @@ struct GatherArgs {
     int n_groups;
     int n_items;
     int rows_per_item;
-    float scale;          // applied to every output
 };
+
+__device__ __forceinline__ int group_of(const GatherArgs& a, int item)
+{
+    int gi = 0;
+#pragma unroll 1
+    for (int k = 1; k < a.n_groups; k++)
+        if (item >= a.g[k].item_begin) gi = k;
+    return gi;
+}
 
 // out[row][b][m] = scale * sum_{(idx,val) in row} in[idx][b][m] * val
 // LPR lanes per row (one per phantom, B <= LPR), 256/LPR rows per item.
@@ -126,11 +135,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a)
     const int lane_row = threadIdx.x / LPR;
     const int b = threadIdx.x % LPR;
     for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
-        int gi = 0;
-#pragma unroll 1
-        for (int k = 1; k < a.n_groups; k++)
-            if (item >= a.g[k].item_begin) gi = k;
-        const GatherGroup& G = a.g[gi];
+        const GatherGroup& G = a.g[group_of(a, item)];
         const int row = (item - G.item_begin) * a.rows_per_item + lane_row;
         if (row >= G.rows || b >= G.B) continue;
         const uint32_t j0 = G.ptr[row], j1 = G.ptr[row + 1];
@@ -167,10 +172,57 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a)
         }
         float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int m = 0; m < M; m++) o[m] = (float)(acc[m] * a.scale);
+        for (int m = 0; m < M; m++) o[m] = (float)(acc[m] * G.scale);
         float4* dst = reinterpret_cast<float4*>(G.out) + ((size_t)row * G.B + b) * 2;
         dst[0] = make_float4(o[0], o[1], o[2], o[3]);
         dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+    }
+}
+
+// One phantom per group (B = 1): the SPL lanes of a row split its entries
+// (lane k takes entries j0+k, j0+k+SPL, ...: the row is read in contiguous
+// SPL*8-byte runs) and combine their FP64 partial sums by a butterfly over
+// the SPL lanes; 256/SPL rows per item.
+template <int M, int SPL>
+__global__ __launch_bounds__(256) void k_gather_split(GatherArgs a)
+{
+    static_assert(SPL >= 2 && SPL <= 64 && (SPL & (SPL - 1)) == 0, "SPL is a power of two <= 64");
+    const int lane_row = threadIdx.x / SPL;
+    const int k = threadIdx.x % SPL;
+    for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
+        const GatherGroup& G = a.g[group_of(a, item)];
+        const int row = (item - G.item_begin) * a.rows_per_item + lane_row;
+        // every lane of a row takes the same branch, and the SPL lanes of a
+        // row lie in one wave, so the shuffles below see only active lanes
+        if (row >= G.rows) continue;
+        const uint32_t j0 = G.ptr[row], j1 = G.ptr[row + 1];
+        double acc[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[m] = 0.0;
+        const float4* in4 = reinterpret_cast<const float4*>(G.in);
+        for (uint32_t j = j0 + k; j < j1; j += SPL) {
+            const uint2 e = G.ent[j];
+            const float4 x0 = in4[(size_t)e.x * 2];
+            float4 x1 = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (M > 4) x1 = in4[(size_t)e.x * 2 + 1];
+            const double v = __uint_as_float(e.y);
+            const float x[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+            for (int m = 0; m < M; m++) acc[m] = __builtin_fma((double)x[m], v, acc[m]);
+        }
+#pragma unroll
+        for (int off = SPL / 2; off > 0; off >>= 1) {
+#pragma unroll
+            for (int m = 0; m < M; m++) acc[m] += __shfl_xor(acc[m], off, SPL);
+        }
+        if (k == 0) {
+            float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int m = 0; m < M; m++) o[m] = (float)(acc[m] * G.scale);
+            float4* dst = reinterpret_cast<float4*>(G.out) + (size_t)row * 2;
+            dst[0] = make_float4(o[0], o[1], o[2], o[3]);
+            dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+        }
     }
 }
 
@@ -342,6 +394,7 @@ vct_status vct_sysmat_build(vct_plan* p, int transpose, vct_sysmat** out, void* 
     const int NN = g.recon_size * g.recon_size;
     vct_sysmat* m = new vct_sysmat();
     m->plan = p;
+    m->scale = (float)(1.0 / p->g.xfactor);
     m->nrays = nrays;
     m->NN = NN;
     uint32_t* counts = nullptr;
@@ -448,12 +501,13 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
     if (!groups || n_groups < 1 || n_groups > kMaxGroups)
         return set_error(VCT_ERR_ARG, "n_groups %d outside 1..%d", n_groups, kMaxGroups);
     if (M < 1 || M > 8) return set_error(VCT_ERR_ARG, "M=%d outside 1..8", M);
-    if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
-        return set_error(VCT_ERR_ARG, "lanes_per_row %d not 8/16/32/64", lanes);
+    if (lanes != 1 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+        return set_error(VCT_ERR_ARG, "lanes_per_row %d not 1/8/16/32/64", lanes);
+    // lanes_per_row 1: one phantom per group, each row split over kSplit lanes
+    constexpr int kSplit = 16;
     GatherArgs a{};
     a.n_groups = n_groups;
-    a.rows_per_item = 256 / lanes;
-    a.scale = 1.0f;
+    a.rows_per_item = 256 / (lanes == 1 ? kSplit : lanes);
     int items = 0;
     for (int k = 0; k < n_groups; k++) {
         const vct_batch_group& G = groups[k];
@@ -469,6 +523,7 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
         d.in = G.d_in;
         d.out = G.d_out;
         d.B = G.n_phantoms;
+        d.scale = m->scale;
         d.item_begin = items;
         items += (d.rows + a.rows_per_item - 1) / a.rows_per_item;
     }
@@ -478,7 +533,9 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
     hipStream_t s = (hipStream_t)stream;
 #define VCT_GATHER_CASE(MM, LL) \
     if (M == MM && lanes == LL) hipLaunchKernelGGL((k_gather<MM, LL>), dim3(blocks), dim3(256), 0, s, a);
-#define VCT_GATHER_M(MM) VCT_GATHER_CASE(MM, 8) VCT_GATHER_CASE(MM, 16) VCT_GATHER_CASE(MM, 32) VCT_GATHER_CASE(MM, 64)
+#define VCT_GATHER_M(MM)                                                                                   \
+    if (M == MM && lanes == 1) hipLaunchKernelGGL((k_gather_split<MM, kSplit>), dim3(blocks), dim3(256), 0, s, a); \
+    VCT_GATHER_CASE(MM, 8) VCT_GATHER_CASE(MM, 16) VCT_GATHER_CASE(MM, 32) VCT_GATHER_CASE(MM, 64)
     VCT_GATHER_M(1) VCT_GATHER_M(2) VCT_GATHER_M(3) VCT_GATHER_M(4)
     VCT_GATHER_M(5) VCT_GATHER_M(6) VCT_GATHER_M(7) VCT_GATHER_M(8)
 #undef VCT_GATHER_M

@@ -52,6 +52,7 @@ struct TraceArgs {
     int max_steps;        // hard bound on marching steps per ray
     int rcheck;           // 1 when the reference's r >= rmax test can fire
     int exact;            // 1 forces the exact-division march
+    int adj_vstride;      // adjoint lane order: 0 view-major, P > 0 channel-major with views j*P mod V
     int ray_offset;       // first ray handled by this launch
     int n_rays;           // rays handled by this launch
     double D;             // pixel pitch
@@ -107,6 +108,18 @@ struct BackprojArgs {
     double pix, sid, dbeta_ch, inv_dbeta_ch, dbeta_view, xfactor, center, mask_r2, mask_r;
 };
 
+struct LengthprojArgs {
+    const float* proj;    // [count][W] sinogram window
+    const double* cosT;   // [V]
+    const double* sinT;   // [V]
+    float* npf;           // [N][N]
+    int N, W, V;
+    int base;             // view of window row 0
+    int count;            // window rows (V/2*2)
+    double pix, rz, sid, det_src_iso, det_radius, iso_det, iso_channel;
+    double delta;         // channel pitch in radians, max_alpha / ((W-1)/2)
+};
+
 }  // namespace vct
 
 struct vct_ctx {
@@ -135,6 +148,7 @@ struct vct_plan {
     double* d_coeff = nullptr;
     float* d_bpk[3] = {nullptr, nullptr, nullptr};  // packed filter matrix per type
     int exact_trace = 0;  // VCT_OPT_EXACT_TRACE
+    int adj_vstride = -1; // VCT_OPT_ADJ_VIEW_STRIDE (-1: automatic)
 };
 
 namespace vct {
@@ -149,6 +163,7 @@ vct_status launch_pack_f32(const float* src, int M, int N, float* dst, hipStream
 vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipStream_t s);
 vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s);
 vct_status launch_beam_hardening(const float* in, float* out, int n, double alpha, double beta, hipStream_t s);
+vct_status launch_lengthproj(const LengthprojArgs& a, hipStream_t s);
 vct_status plan_filter_matrix(vct_plan* plan, int filter, float** d_bpk);
 TraceArgs make_trace_args(const vct_plan* plan);
 }  // namespace vct

@@ -53,8 +53,26 @@ class Plan:
             call("vct_plan_create", self.ctx, C.byref(geom), C.byref(self.handle))
         g = geom
         self.N, self.W, self.V = g.recon_size, g.n_channels, g.n_views
+        # "atomic": the adjoint re-traces every ray and adds with atomics;
+        # "sysmat": it gathers over a cached voxel-major system matrix
+        # (built on first use, ~8 bytes per segment twice), deterministic.
+        self.adjoint_mode = "atomic"
+        self._sysmat = None
+
+    def set_option(self, option: int, value: int):
+        call("vct_plan_set_option", self.handle, option, value)
+
+    def system_matrix(self):
+        """The cached system matrix of this geometry (ray- and voxel-major)."""
+        if self._sysmat is None:
+            from .batch import SystemMatrix
+            self._sysmat = SystemMatrix(self, transpose=True)
+        return self._sysmat
 
     def close(self):
+        if self._sysmat is not None:
+            self._sysmat.close()
+            self._sysmat = None
         if self.handle:
             _lib.lib().vct_plan_destroy(self.handle)
             self.handle = C.c_void_p()
@@ -202,6 +220,23 @@ def backproject(plan: Plan, fltd: torch.Tensor):
     return img
 
 
+ISO_TO_DETECTOR_CM = 57.6
+
+
+def lengthproj(plan: Plan, proj: torch.Tensor, z_cm: float, couch_speed_cm: float,
+               iso_det_cm: float = ISO_TO_DETECTOR_CM) -> torch.Tensor:
+    """npf [N,N]: chord-weighted pixel backprojection of the sinogram window
+    proj [V,W] for slice z (the MBFPA program's npf_float.raw)."""
+    _require_gpu(proj)
+    proj = proj.contiguous()
+    if proj.dtype != torch.float32 or proj.shape != (plan.V // 2 * 2, plan.W):
+        raise ValueError(f"proj must be float32 [{plan.V // 2 * 2}, {plan.W}]")
+    npf = torch.empty((plan.N, plan.N), dtype=torch.float32, device=proj.device)
+    call("vct_lengthproj", plan.handle, _ptr(proj), float(z_cm), float(couch_speed_cm), float(iso_det_cm),
+         _ptr(npf), _stream(plan.device))
+    return npf
+
+
 def fbp(plan: Plan, proj: torch.Tensor, filt: int = FILTER_SHEPP_LOGAN, in_mode: bool = False,
         bhc: tuple[float, float] | None = None):
     """FBP; bhc=(alpha, beta) fuses the beam-hardening correction into the filter."""
@@ -255,6 +290,21 @@ def trace_adjoint(plan: Plan, weights: torch.Tensor, M: int, g: torch.Tensor | N
     return grad
 
 
+def sysmat_adjoint(plan: Plan, weights: torch.Tensor, M: int,
+                   dens_packed_f32: torch.Tensor | None = None) -> torch.Tensor:
+    """The adjoint of ``trace_adjoint`` (g = None) as an atomic-free gather
+    over the plan's cached voxel-major system matrix: grad [N,N,8].  Voxels
+    whose packed density is negative get zero, as the tracer's sign test."""
+    from .batch import batch_adjoint
+    sm = plan.system_matrix()
+    grad = torch.empty((plan.N, plan.N, 8), dtype=torch.float32, device=_dev(plan))
+    w = weights.contiguous().view(plan.V * plan.W, 1, 8)
+    batch_adjoint([(sm, w, grad.view(plan.N * plan.N, 1, 8), 1)], M, 1)
+    if dens_packed_f32 is not None:
+        grad.mul_((dens_packed_f32.view(plan.N, plan.N, 8) >= 0).to(grad.dtype))
+    return grad
+
+
 class MBFPAIntensity(torch.autograd.Function):
     """I[s,v,w] = sum_E pdf_s(E) exp(-sum_m mu_s,m(E) L_m(v,w)) of float densities
     [M,N,N] (g/cm^3).  Backward runs the hand-written adjoint once for all
@@ -276,7 +326,10 @@ class MBFPAIntensity(torch.autograd.Function):
         pk, dIdL = ctx.saved_tensors
         plan, M = ctx.plan, ctx.M
         w = (dIdL * grad_raw.contiguous().unsqueeze(-1)).sum(0)  # [V,W,8]
-        grad = trace_adjoint(plan, w, M, None, pk)
+        if plan.adjoint_mode == "sysmat":
+            grad = sysmat_adjoint(plan, w, M, pk)
+        else:
+            grad = trace_adjoint(plan, w, M, None, pk)
         return grad[..., :M].permute(2, 0, 1).contiguous(), None, None, None
 
 
