@@ -105,14 +105,8 @@ vct_status vct_plan_destroy(vct_plan* plan);
 /* Plan options (no counterpart in the reference).
  * VCT_OPT_EXACT_TRACE = 1: march every ray with correctly rounded divisions
  * (the reference's operation sequence) instead of the default guarded
- * reciprocal march; both give the same segments, the default is faster.
- * VCT_OPT_ADJ_VIEW_STRIDE = 2: lane order of the atomic adjoint
- * (vct_trace_adjoint).  0 gives consecutive channels of one view to
- * consecutive lanes; P > 0 gives lanes of one channel the views
- * j*P mod V, so that simultaneous atomic adds of a wave land on different
- * voxels.  P must be coprime with n_views; -1 (the default) picks
- * P automatically.  Only the order of the float additions changes. */
-enum { VCT_OPT_EXACT_TRACE = 1, VCT_OPT_ADJ_VIEW_STRIDE = 2 };
+ * reciprocal march; both give the same segments, the default is faster. */
+enum { VCT_OPT_EXACT_TRACE = 1 };
 vct_status vct_plan_set_option(vct_plan* plan, int option, int value);
 
 /* ------------------------------------------------------- host-side tables */

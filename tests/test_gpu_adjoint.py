@@ -155,20 +155,17 @@ def test_autograd_gradcheck_small(gpu):
     assert ad == pytest.approx(fd, rel=2e-3)
 
 
-@pytest.mark.parametrize("stride", [0, 7, -1])
-def test_adjoint_lane_orders_agree_with_fp64(gpu, golden, oracle, stride):
-    """The atomic adjoint in view-major order (0), channel-major with view
-    stride 7 and the automatic stride changes only the order of the float
-    additions: every order matches the FP64 adjoint."""
-    import torch
-    from virtualct_amd._lib import OPT_ADJ_VIEW_STRIDE
+@pytest.mark.parametrize("mode", ["atomic", "sysmat"])
+def test_both_adjoint_modes_match_fp64(gpu, golden, oracle, mode):
+    """The atomic re-tracing adjoint and the gather over the cached system
+    matrix both match the FP64 adjoint."""
     ops = _ops()
     N, pix, V = 128, 0.35, 160
     ph, tabs = _problem(golden, gpu, N, pix, V)
     dens = ph.astype(np.float32) / np.float32(1000)
     att, pdf, _ = tabs[0]
-    plan = ops.Plan(_geom(N, pix, V))  # own plan: the option is per plan
-    plan.set_option(OPT_ADJ_VIEW_STRIDE, stride)
+    plan = ops.Plan(_geom(N, pix, V))  # own plan: the mode is per plan
+    plan.adjoint_mode = mode
     rng = np.random.default_rng(21)
     gI = rng.normal(size=(V, 609))
     x = _t(dens, gpu).requires_grad_(True)
@@ -178,16 +175,6 @@ def test_adjoint_lane_orders_agree_with_fp64(gpu, golden, oracle, stride):
     got = x.grad.cpu().numpy()
     assert rel_l2(got, ref) < 1e-4
     assert rel_max(got, ref) < 1e-4
-
-
-def test_adjoint_view_stride_rejects_common_factor(gpu):
-    from virtualct_amd._lib import OPT_ADJ_VIEW_STRIDE
-    ops = _ops()
-    plan = ops.Plan(_geom(64, 0.7, 80))
-    with pytest.raises(RuntimeError):
-        plan.set_option(OPT_ADJ_VIEW_STRIDE, 10)  # gcd(10, 80) = 10
-    with pytest.raises(RuntimeError):
-        plan.set_option(OPT_ADJ_VIEW_STRIDE, -2)
 
 
 def test_sysmat_adjoint_mode_matches_fp64_and_is_deterministic(gpu, golden, oracle):

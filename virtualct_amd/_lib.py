@@ -18,7 +18,7 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of mem
           4: "I/O error", 5: "unsupported", 6: "internal limit"}
 DENS_U16, DENS_F32 = 0, 1
 FILTER_RAMLAK, FILTER_SHEPP_LOGAN = 1, 2
-OPT_EXACT_TRACE, OPT_ADJ_VIEW_STRIDE = 1, 2
+OPT_EXACT_TRACE = 1
 
 
 class VctError(RuntimeError):

@@ -205,8 +205,9 @@ int main(int argc, char* argv[])
         }
         st = vct_lengthproj_host(ctx, &g, win.data(), zstart, in.couch_speed, 57.6, npf.data());
         if (st != VCT_OK) {
+            const int rc = fail("lengthproj", st);  // before vct_terminate clears the message
             vct_terminate(ctx);
-            return fail("lengthproj", st);
+            return rc;
         }
     } else if (have_sino) {
         std::fprintf(stderr, "WARNING: the measured-data loader is not part of this build; npf_float.raw is "

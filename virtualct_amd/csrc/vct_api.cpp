@@ -8,7 +8,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <numeric>
 #include <string>
 #include <vector>
 
@@ -46,15 +45,6 @@ static vct_status check_geometry(const vct_geometry* g)
     return VCT_OK;
 }
 
-// Smallest stride >= ceil(V/64) coprime with V: the 64 lanes of a wave then
-// take views spread over the whole turn.
-static int auto_view_stride(int V)
-{
-    int P = std::max(1, (V + 63) / 64);
-    while (std::gcd(P, V) != 1) P++;
-    return P;
-}
-
 TraceArgs make_trace_args(const vct_plan* p)
 {
     TraceArgs a{};
@@ -84,7 +74,6 @@ TraceArgs make_trace_args(const vct_plan* p)
     a.ray_offset = 0;
     a.n_rays = g.n_views * g.n_channels;
     a.exact = p->exact_trace;
-    a.adj_vstride = 0;
     return a;
 }
 
@@ -328,12 +317,6 @@ vct_status vct_plan_set_option(vct_plan* p, int option, int value)
     if (!p) return set_error(VCT_ERR_ARG, "plan is NULL");
     switch (option) {
     case VCT_OPT_EXACT_TRACE: p->exact_trace = value ? 1 : 0; return VCT_OK;
-    case VCT_OPT_ADJ_VIEW_STRIDE:
-        if (value < -1 || (value > 0 && std::gcd(value, p->g.n_views) != 1))
-            return set_error(VCT_ERR_ARG, "view stride %d must be -1, 0 or coprime with %d views", value,
-                             p->g.n_views);
-        p->adj_vstride = value;
-        return VCT_OK;
     default: return set_error(VCT_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -694,7 +677,6 @@ vct_status vct_trace_adjoint(vct_plan* p, const void* d_packed, int dens_type, i
     a.g = d_g;
     a.grad = d_grad;
     a.out_scale = (dens_type == VCT_DENS_U16 ? 1.0 / 1000.0 : 1.0) / p->g.xfactor;
-    a.adj_vstride = p->adj_vstride >= 0 ? p->adj_vstride : auto_view_stride(p->g.n_views);
     return launch_trace(a, VCT_DENS_F32, M, true, pick_stream(p, stream));
 }
 

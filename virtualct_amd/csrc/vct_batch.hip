@@ -127,6 +127,32 @@ __device__ __forceinline__ int group_of(const GatherArgs& a, int item)
     return gi;
 }
 
+// acc[m] += in[idx][b][m] * val for the U entries j, j+stride, ..., in that
+// order: all U entry loads are issued first, then all U gathers, so a lane
+// keeps U independent loads in flight.
+template <int M, int U>
+__device__ __forceinline__ void gather_step(const uint2* __restrict__ ent, const float4* __restrict__ in4, int B, int b,
+                                            uint32_t j, uint32_t stride, double (&acc)[M])
+{
+    uint2 e[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) e[u] = ent[j + u * stride];
+    float4 lo[U], hi[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const size_t i = ((size_t)e[u].x * B + b) * 2;
+        lo[u] = in4[i];
+        hi[u] = M > 4 ? in4[i + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const double v = __uint_as_float(e[u].y);
+        const float x[8] = {lo[u].x, lo[u].y, lo[u].z, lo[u].w, hi[u].x, hi[u].y, hi[u].z, hi[u].w};
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[m] = __builtin_fma((double)x[m], v, acc[m]);
+    }
+}
+
 // out[row][b][m] = scale * sum_{(idx,val) in row} in[idx][b][m] * val
 // LPR lanes per row (one per phantom, B <= LPR), 256/LPR rows per item.
 template <int M, int LPR>
@@ -144,32 +170,8 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a)
         for (int m = 0; m < M; m++) acc[m] = 0.0;
         const float4* in4 = reinterpret_cast<const float4*>(G.in);
         uint32_t j = j0;
-        for (; j + 1 < j1; j += 2) {
-            const uint2 e0 = G.ent[j], e1 = G.ent[j + 1];
-            const size_t i0 = ((size_t)e0.x * G.B + b) * 2, i1 = ((size_t)e1.x * G.B + b) * 2;
-            const float4 a0 = in4[i0], a1 = in4[i1];
-            float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
-            if (M > 4) { c0 = in4[i0 + 1]; c1 = in4[i1 + 1]; }
-            const double v0 = __uint_as_float(e0.y), v1 = __uint_as_float(e1.y);
-            const float x0[8] = {a0.x, a0.y, a0.z, a0.w, c0.x, c0.y, c0.z, c0.w};
-            const float x1[8] = {a1.x, a1.y, a1.z, a1.w, c1.x, c1.y, c1.z, c1.w};
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                acc[m] = __builtin_fma((double)x0[m], v0, acc[m]);
-                acc[m] = __builtin_fma((double)x1[m], v1, acc[m]);
-            }
-        }
-        if (j < j1) {
-            const uint2 e0 = G.ent[j];
-            const size_t i0 = ((size_t)e0.x * G.B + b) * 2;
-            const float4 a0 = in4[i0];
-            float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (M > 4) c0 = in4[i0 + 1];
-            const double v0 = __uint_as_float(e0.y);
-            const float x0[8] = {a0.x, a0.y, a0.z, a0.w, c0.x, c0.y, c0.z, c0.w};
-#pragma unroll
-            for (int m = 0; m < M; m++) acc[m] = __builtin_fma((double)x0[m], v0, acc[m]);
-        }
+        for (; j + 3 < j1; j += 4) gather_step<M, 4>(G.ent, in4, G.B, b, j, 1, acc);
+        for (; j < j1; j++) gather_step<M, 1>(G.ent, in4, G.B, b, j, 1, acc);
         float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int m = 0; m < M; m++) o[m] = (float)(acc[m] * G.scale);
@@ -200,16 +202,9 @@ __global__ __launch_bounds__(256) void k_gather_split(GatherArgs a)
 #pragma unroll
         for (int m = 0; m < M; m++) acc[m] = 0.0;
         const float4* in4 = reinterpret_cast<const float4*>(G.in);
-        for (uint32_t j = j0 + k; j < j1; j += SPL) {
-            const uint2 e = G.ent[j];
-            const float4 x0 = in4[(size_t)e.x * 2];
-            float4 x1 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (M > 4) x1 = in4[(size_t)e.x * 2 + 1];
-            const double v = __uint_as_float(e.y);
-            const float x[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-#pragma unroll
-            for (int m = 0; m < M; m++) acc[m] = __builtin_fma((double)x[m], v, acc[m]);
-        }
+        uint32_t j = j0 + k;
+        for (; j + 3 * SPL < j1; j += 4 * SPL) gather_step<M, 4>(G.ent, in4, 1, 0, j, SPL, acc);
+        for (; j < j1; j += SPL) gather_step<M, 1>(G.ent, in4, 1, 0, j, SPL, acc);
 #pragma unroll
         for (int off = SPL / 2; off > 0; off >>= 1) {
 #pragma unroll

@@ -52,7 +52,6 @@ struct TraceArgs {
     int max_steps;        // hard bound on marching steps per ray
     int rcheck;           // 1 when the reference's r >= rmax test can fire
     int exact;            // 1 forces the exact-division march
-    int adj_vstride;      // adjoint lane order: 0 view-major, P > 0 channel-major with views j*P mod V
     int ray_offset;       // first ray handled by this launch
     int n_rays;           // rays handled by this launch
     double D;             // pixel pitch
@@ -148,7 +147,6 @@ struct vct_plan {
     double* d_coeff = nullptr;
     float* d_bpk[3] = {nullptr, nullptr, nullptr};  // packed filter matrix per type
     int exact_trace = 0;  // VCT_OPT_EXACT_TRACE
-    int adj_vstride = -1; // VCT_OPT_ADJ_VIEW_STRIDE (-1: automatic)
 };
 
 namespace vct {

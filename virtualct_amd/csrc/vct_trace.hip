@@ -145,20 +145,11 @@ __global__ __launch_bounds__(256) void k_trace_adj(TraceArgs a)
 {
     const int local = blockIdx.x * blockDim.x + threadIdx.x;
     if (local >= a.n_rays) return;
-    int view, ch;
-    if (a.adj_vstride > 0) {
-        // channel-major: lanes of a wave take views j*P mod V of one channel
-        // (a bijection on the views because gcd(P, V) = 1; the host checks it
-        // and launches all V*W rays from offset 0).
-        ch = local / a.V;
-        const int j = local - ch * a.V;
-        view = (int)(((long long)j * a.adj_vstride) % a.V);
-    } else {
-        const int r = a.ray_offset + local;
-        view = r / a.W;
-        ch = r - view * a.W;
-    }
-    const int ray = view * a.W + ch;
+    // view-major lanes (adjacent channels of one view): the atomics of a wave
+    // land on neighbouring voxels; lanes spread over views measured slower
+    const int ray = a.ray_offset + local;
+    const int view = ray / a.W;
+    const int ch = ray - view * a.W;
     if (view >= a.VT) return;
     SinkAdj<M> sink;
     sink.grad = a.grad;

@@ -53,10 +53,12 @@ class Plan:
             call("vct_plan_create", self.ctx, C.byref(geom), C.byref(self.handle))
         g = geom
         self.N, self.W, self.V = g.recon_size, g.n_channels, g.n_views
-        # "atomic": the adjoint re-traces every ray and adds with atomics;
-        # "sysmat": it gathers over a cached voxel-major system matrix
-        # (built on first use, ~8 bytes per segment twice), deterministic.
-        self.adjoint_mode = "atomic"
+        # "sysmat" (default): the autograd adjoint gathers over a cached
+        # voxel-major system matrix, built on first use (8 bytes per segment
+        # in each direction, 4.5 GB at 512^2 / 800 x 609; ~40 ms), atomic-free
+        # and deterministic, ~50x faster per call than "atomic", which
+        # re-traces every ray and scatters with float atomics (no memory).
+        self.adjoint_mode = "sysmat"
         self._sysmat = None
 
     def set_option(self, option: int, value: int):
