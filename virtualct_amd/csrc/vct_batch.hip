@@ -38,11 +38,13 @@ namespace vct {
 constexpr int kMaxGroups = 16;
 
 struct CountSink {
+    static constexpr bool kAdd2 = false;
     int n = 0;
     __device__ __forceinline__ void add(int, double) { n++; }
 };
 
 struct FillSink {
+    static constexpr bool kAdd2 = false;
     uint2* out;
     uint32_t* keys;   // voxel of each entry (optional)
     uint32_t* vals;   // entry index (optional)
@@ -118,6 +120,20 @@ struct GatherArgs {
     int rows_per_item;
 };
 
+// Workgroups are dispatched round-robin over the XCDs, each with its own L2.
+// Renumber them so that the workgroups resident on one XCD take consecutive
+// work items: adjacent rays (or voxels) share most of their gathered data,
+// which then stays in that XCD's L2 (a bijection when the grid is a multiple
+// of kXcds; other grids keep the hardware order).
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_block()
+{
+    if (gridDim.x % kXcds != 0) return blockIdx.x;
+    const int per = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
 __device__ __forceinline__ int group_of(const GatherArgs& a, int item)
 {
     int gi = 0;
@@ -160,7 +176,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a)
 {
     const int lane_row = threadIdx.x / LPR;
     const int b = threadIdx.x % LPR;
-    for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
+    for (int item = xcd_block(); item < a.n_items; item += gridDim.x) {
         const GatherGroup& G = a.g[group_of(a, item)];
         const int row = (item - G.item_begin) * a.rows_per_item + lane_row;
         if (row >= G.rows || b >= G.B) continue;
@@ -191,7 +207,7 @@ __global__ __launch_bounds__(256) void k_gather_split(GatherArgs a)
     static_assert(SPL >= 2 && SPL <= 64 && (SPL & (SPL - 1)) == 0, "SPL is a power of two <= 64");
     const int lane_row = threadIdx.x / SPL;
     const int k = threadIdx.x % SPL;
-    for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
+    for (int item = xcd_block(); item < a.n_items; item += gridDim.x) {
         const GatherGroup& G = a.g[group_of(a, item)];
         const int row = (item - G.item_begin) * a.rows_per_item + lane_row;
         // every lane of a row takes the same branch, and the SPL lanes of a
@@ -525,6 +541,7 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
     a.n_items = items;
     if (blocks <= 0) blocks = 2048;
     blocks = blocks < items ? blocks : items;
+    if (blocks >= kXcds) blocks -= blocks % kXcds;
     hipStream_t s = (hipStream_t)stream;
 #define VCT_GATHER_CASE(MM, LL) \
     if (M == MM && lanes == LL) hipLaunchKernelGGL((k_gather<MM, LL>), dim3(blocks), dim3(256), 0, s, a);

@@ -19,7 +19,13 @@
 // reference's exact expression and the decision is taken on those, so every
 // decision equals the reference's.  Segment lengths become |da| * sec(ray),
 // which differs from the reference's sqrt by rounding only and never enters a
-// decision.
+// decision.  Each step (plane point, minor cell, crossing, the voxels of its
+// one or two segments) is first evaluated from the fast values with all
+// margin tests folded into one flag and committed without branches when the
+// flag holds; only a step that fails a test runs the general step, which
+// makes the points involved exact.  The general step, given the same fast
+// values and no failed test, takes the same decisions, so both routes give
+// the same segments.
 #pragma once
 
 #include <climits>
@@ -47,23 +53,26 @@ template <bool FAST>
 __device__ __forceinline__ int index_of(double h, double D, double invD, double c)
 {
     if (FAST) {
-        const double q = h * invD + c;
+        const double q = __builtin_fma(h, invD, c);
         const double t = trunc(q);
         const double f = fabs(q - t);
-        if (f > kMarchMargin && f < 1.0 - kMarchMargin && fabs(q) < 1.0e6) return (int)t;
+        if (fabs(f - 0.5) < 0.5 - kMarchMargin && fabs(q) < 1.0e6) return (int)t;
     }
     return trunc_i32(h / D + c);
 }
 
 // Index of a value known only to ~1e-13: returns false (undecided) when the
 // value is within the margin of an integer boundary.
+// The value is only trusted away from the boundaries, so the fused
+// multiply-add (one rounding fewer) is as good as the separate operations,
+// and |f - 0.5| < 0.5 - margin is the two-sided test f in (margin, 1 - margin).
 __device__ __forceinline__ bool index_checked(double h, double invD, double c, int& out)
 {
-    const double q = h * invD + c;
+    const double q = __builtin_fma(h, invD, c);
     const double t = trunc(q);
     const double f = fabs(q - t);
     out = (int)t;
-    return f > kMarchMargin && f < 1.0 - kMarchMargin && fabs(q) < 1.0e6;
+    return fabs(f - 0.5) < 0.5 - kMarchMargin && fabs(q) < 1.0e6;
 }
 
 // (u/fabs(u) != v/fabs(v)) of M/reprojection_TOMO.cpp:134,221 for finite
@@ -133,33 +142,38 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
     const bool b_neg = (bf - bi) < 0;
 
     // one candidate segment p0 -> p1, :141-166 / :168-189
+    // The midpoint voxel (:146-149) in (major, minor) form: the x index
+    // takes +x_mid = (x0 + x1) * 0.5, the y index (-y0 - y1) * 0.5, which is
+    // exactly -(y0 + y1) * 0.5 (negation and halving are exact; only the sign
+    // of a zero can differ, which no index sees).  So the major index takes
+    // sA * a_mid and the minor one -sA * b_mid with sA = +1 when x is major,
+    // and voxel = ia * stA + ib * stB; no per-segment selects between x and y.
+    const double sA = xmajor ? 1.0 : -1.0;
+    const int stA = xmajor ? 1 : a.N, stB = xmajor ? a.N : 1;
     auto emit = [&](MarchPoint& p0, MarchPoint& p1) {
-        int xmc, ymc;
+        int ia, ib;
         bool decided = false;
         if (FAST) {
-            const double xb0 = xmajor ? p0.a : p0.b, yb0 = xmajor ? p0.b : p0.a;
-            const double xb1 = xmajor ? p1.a : p1.b, yb1 = xmajor ? p1.b : p1.a;
-            decided = index_checked((xb0 + xb1) * 0.5, a.invD, (double)a.nxm, xmc) &&
-                      index_checked((-yb0 - yb1) * 0.5, a.invD, (double)a.nxm, ymc);
+            decided = index_checked(sA * ((p0.a + p1.a) * 0.5), a.invD, (double)a.nxm, ia) &&
+                      index_checked(-sA * ((p0.b + p1.b) * 0.5), a.invD, (double)a.nxm, ib);
             if (!decided) {
                 make_exact(p0);
                 make_exact(p1);
             }
         }
-        const double xb0 = xmajor ? p0.a : p0.b, yb0 = xmajor ? p0.b : p0.a;
-        const double xb1 = xmajor ? p1.a : p1.b, yb1 = xmajor ? p1.b : p1.a;
         if (!decided) {
-            xmc = index_of<FAST>((xb0 + xb1) * 0.5, D, a.invD, (double)a.nxm);
-            ymc = index_of<FAST>((-yb0 - yb1) * 0.5, D, a.invD, (double)a.nxm);
+            ia = index_of<FAST>(sA * ((p0.a + p1.a) * 0.5), D, a.invD, (double)a.nxm);
+            ib = index_of<FAST>(-sA * ((p0.b + p1.b) * 0.5), D, a.invD, (double)a.nxm);
         }
-        if (xmc >= 0 && ymc >= 0 && xmc <= a.N - 1 && ymc <= a.N - 1) {
+        if (ia >= 0 && ib >= 0 && ia <= a.N - 1 && ib <= a.N - 1) {
             if (a.rcheck) {
-                const double r = sqrt((xb0 + xb1) * (xb0 + xb1) + (-yb0 - yb1) * (-yb0 - yb1)) * 0.5;
+                // x^2 + y^2 of the reference with the terms in either order
+                const double r = sqrt((p0.a + p1.a) * (p0.a + p1.a) + (p0.b + p1.b) * (p0.b + p1.b)) * 0.5;
                 if (r >= a.rmax) return;
             }
             const double rr = FAST ? fabs(p1.a - p0.a) * sec
-                                   : sqrt((xb0 - xb1) * (xb0 - xb1) + (yb0 - yb1) * (yb0 - yb1));
-            sink.add(xmc + ymc * a.N, rr);
+                                   : sqrt((p0.a - p1.a) * (p0.a - p1.a) + (p0.b - p1.b) * (p0.b - p1.b));
+            sink.add(ia * stA + ib * stB, rr);
         }
     };
 
@@ -170,11 +184,58 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
     int bma = 0;
     int iks = 0;
 
+    const double nxm = (double)a.nxm;
+    auto in_image = [&](int ia, int ib) { return ia >= 0 && ib >= 0 && ia <= a.N - 1 && ib <= a.N - 1; };
+
     for (int ik = 0;; ik++) {
         if (!(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) break;
         if (ik >= a.max_steps) { ovf = true; break; }
         const int bmb = bma;
         const double bj = pc.b;
+        if (FAST) {
+            // The whole step from the fast values with every margin test
+            // folded into one flag: when all of them pass, the general step
+            // below would make no point exact and take exactly these
+            // decisions with exactly these values, so the step is committed
+            // here without its branches; otherwise the general step runs.
+            const double na = ain + pha * ik * D;
+            const double nb = b_calc(na);
+            const double nab = fabs(nb);
+            int nbma;
+            bool sure = index_checked(nb, a.invD, 0.0, nbma);
+            sure &= (nab >= kMarchMargin) & (fabs(nab - Lb) >= kMarchMargin);
+            const bool cand = (ik != 0) & ((nbma != bmb) | sign_split(nb, bj));
+            const double xbb = (double)(nbma * D);
+            const double xb = ((nb < 0 && b_neg) || (nb > 0 && b_pos)) ? xbb : xbb - bdir * D;
+            const double xa = a_calc(xb);
+            const double dxa = na - xa, dxb = nb - xb;
+            sure &= !(cand & (fabs(dxa) < kMarchMargin) & (fabs(dxb) < kMarchMargin));
+            const bool cross = cand & ((dxa * dxa + dxb * dxb) != 0);
+            int ia1, ib1, ia2, ib2;
+            const bool ok1 = index_checked(sA * ((p0.a + xa) * 0.5), a.invD, nxm, ia1) &
+                             index_checked(-sA * ((p0.b + xb) * 0.5), a.invD, nxm, ib1);
+            const double s_a = cross ? xa : p0.a, s_b = cross ? xb : p0.b;
+            const bool ok2 = index_checked(sA * ((s_a + na) * 0.5), a.invD, nxm, ia2) &
+                             index_checked(-sA * ((s_b + nb) * 0.5), a.invD, nxm, ib2);
+            sure &= (ok1 | !cross) & ok2;
+            if (sure) {
+                const bool v1 = cross && in_image(ia1, ib1);
+                const bool v2 = in_image(ia2, ib2);
+                iks += cross ? 2 : 1;
+                if constexpr (Sink::kAdd2) {
+                    // empty segments: length 0 at voxel 0, which adds nothing
+                    sink.add2(v1 ? ia1 * stA + ib1 * stB : 0, v1 ? fabs(xa - p0.a) * sec : 0.0,
+                              v2 ? ia2 * stA + ib2 * stB : 0, v2 ? fabs(na - s_a) * sec : 0.0);
+                } else {
+                    if (v1) sink.add(ia1 * stA + ib1 * stB, fabs(xa - p0.a) * sec);
+                    if (v2) sink.add(ia2 * stA + ib2 * stB, fabs(na - s_a) * sec);
+                }
+                bma = nbma;
+                pc = MarchPoint{na, nb, 1, false};
+                p0 = pc;
+                continue;
+            }
+        }
         pc.a = ain + pha * ik * D;  // :126
         pc.b = b_calc(pc.a);        // :129
         pc.exact = !FAST;

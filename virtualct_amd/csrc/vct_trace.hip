@@ -29,8 +29,13 @@ namespace vct {
 
 // ------------------------------------------------------------------ sinks
 
+// Forward sinks accumulate L_m += rho_m(voxel) * length in FP64.  add2
+// takes the two candidate segments of one march step and issues both gathers
+// before using either; a segment of length 0 at voxel 0 adds exactly nothing
+// (fma(rho, +0, acc) == acc), so the march calls it without branching.
 template <int M>
 struct SinkU16 {
+    static constexpr bool kAdd2 = true;
     const uint4* __restrict__ vox;
     double acc[M];
     __device__ __forceinline__ void init(const void* p)
@@ -39,9 +44,8 @@ struct SinkU16 {
 #pragma unroll
         for (int m = 0; m < M; m++) acc[m] = 0.0;
     }
-    __device__ __forceinline__ void add(int v, double rr)
+    __device__ __forceinline__ void accumulate(const uint4& q, double rr)
     {
-        const uint4 q = vox[v];
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
         for (int m = 0; m < M; m++) {
@@ -49,26 +53,39 @@ struct SinkU16 {
             acc[m] = __builtin_fma((double)u, rr, acc[m]);
         }
     }
+    __device__ __forceinline__ void add(int v, double rr) { accumulate(vox[v], rr); }
+    __device__ __forceinline__ void add2(int v1, double r1, int v2, double r2)
+    {
+        const uint4 q1 = vox[v1], q2 = vox[v2];
+        accumulate(q1, r1);
+        accumulate(q2, r2);
+    }
 };
 
 template <int M>
 struct SinkF32 {
+    static constexpr bool kAdd2 = true;
     const float4* __restrict__ vox;
     double acc[M];
+    struct Vox {
+        float4 lo, hi;
+    };
     __device__ __forceinline__ void init(const void* p)
     {
         vox = reinterpret_cast<const float4*>(p);
 #pragma unroll
         for (int m = 0; m < M; m++) acc[m] = 0.0;
     }
-    __device__ __forceinline__ void add(int v, double rr)
+    __device__ __forceinline__ Vox load(int v) const
     {
-        const float4 a = vox[2 * v];
-        float r[8] = {a.x, a.y, a.z, a.w, 0.f, 0.f, 0.f, 0.f};
-        if (M > 4) {
-            const float4 b = vox[2 * v + 1];
-            r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
-        }
+        Vox q;
+        q.lo = vox[2 * v];
+        q.hi = M > 4 ? vox[2 * v + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        return q;
+    }
+    __device__ __forceinline__ void accumulate(const Vox& q, double rr)
+    {
+        const float r[8] = {q.lo.x, q.lo.y, q.lo.z, q.lo.w, q.hi.x, q.hi.y, q.hi.z, q.hi.w};
 #pragma unroll
         for (int m = 0; m < M; m++) {
             // a negative density contributes nothing (reference ph = 0, :155)
@@ -76,11 +93,19 @@ struct SinkF32 {
             acc[m] = __builtin_fma(rho, rr, acc[m]);
         }
     }
+    __device__ __forceinline__ void add(int v, double rr) { accumulate(load(v), rr); }
+    __device__ __forceinline__ void add2(int v1, double r1, int v2, double r2)
+    {
+        const Vox q1 = load(v1), q2 = load(v2);
+        accumulate(q1, r1);
+        accumulate(q2, r2);
+    }
 };
 
 // Adjoint sink: grad[v][m] += w[m] * rr for every in-image segment.
 template <int M>
 struct SinkAdj {
+    static constexpr bool kAdd2 = false;
     float* __restrict__ grad;
     const float4* __restrict__ dens;  // f32 densities for the sign test, or null
     float w[M];
