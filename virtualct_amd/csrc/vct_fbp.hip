@@ -145,9 +145,15 @@ __device__ __forceinline__ double rcp_f64(double x)
     return __builtin_fma(r, e, r);
 }
 
+// WIDE: some masked pixel may see |tan(gamma)| > 0.5, beyond the polynomial;
+// those take the library atan.  The host picks the variant from the bound
+// |tan(gamma)| <= r / sqrt(sid^2 - r^2) over the mask radius r; without the
+// fallback the view loop has no branch and the compiler overlaps the views.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void k_backproject(BackprojArgs a)
 {
-    extern __shared__ float sv[];  // [kBpViews][W]
+    extern __shared__ float sv[];  // [kBpViews][W] filtered views, then [kBpViews][2] cos/sin (doubles)
+    double* trig = reinterpret_cast<double*>(sv + kBpViews * a.W + ((kBpViews * a.W) & 1));
     const int tx = threadIdx.x & (kBpTile - 1), ty = threadIdx.x / kBpTile;
     const int x = blockIdx.x * kBpTile + tx;
     const int y = blockIdx.y * kBpTile + ty;
@@ -165,26 +171,36 @@ __global__ __launch_bounds__(256) void k_backproject(BackprojArgs a)
         __syncthreads();
         for (int i = threadIdx.x; i < nv * a.W; i += blockDim.x)
             sv[i] = fl[(size_t)th0 * a.W + i];
+        if (threadIdx.x < kBpViews) {
+            const int th = th0 + min((int)threadIdx.x, nv - 1);
+            trig[2 * threadIdx.x] = a.cosT[th];
+            trig[2 * threadIdx.x + 1] = a.sinT[th];
+        }
         __syncthreads();
         if (masked) {
-            for (int t = 0; t < nv; t++) {
-                const int th = th0 + t;
-                const double c = a.cosT[th], sn = a.sinT[th];
+            // The views' terms are independent FP64 chains the compiler can
+            // overlap; each is added to acc in view order (the reference's
+            // float accumulation order) and only when the reference adds it.
+#pragma unroll
+            for (int u = 0; u < kBpViews; u++) {
+                const int tu = u < nv ? u : nv - 1;  // a clamped view is computed, not added
+                const double c = trig[2 * tu], sn = trig[2 * tu + 1];
                 const double X = xx * c + yy * sn;
                 const double Y = -xx * sn + yy * c;
                 const double den = a.sid - Y;
                 const double L2 = den * den + X * X;
                 const double tg = X * rcp_f64(den);
-                const double gam = fabs(tg) <= 0.5 ? atan_small(tg) : atan(X / den);
+                double gam = atan_small(tg);
+                if (WIDE && fabs(tg) > 0.5) gam = atan(X / den);
                 const double fGAM = gam * a.inv_dbeta_ch + a.cent;
                 const int GAM = trunc_i32_bp(fGAM);
                 const double w = fGAM - GAM;
-                if (GAM >= 0 && GAM < a.W - 1) {
-                    const float* row = sv + t * a.W;
-                    const double sum1 = (1 - w) * row[GAM] + w * row[GAM + 1];
-                    const double sum = sum1 + 0.0;
-                    acc += (float)(rcp_f64(L2) * sum * a.dbeta_view * a.xfactor);
-                }
+                const bool ok = u < nv && GAM >= 0 && GAM < a.W - 1;
+                const float* row = sv + tu * a.W + (ok ? GAM : 0);
+                const double sum1 = (1 - w) * row[0] + w * row[1];
+                const double sum = sum1 + 0.0;
+                const float term = (float)(rcp_f64(L2) * sum * a.dbeta_view * a.xfactor);
+                acc = ok ? acc + term : acc;
             }
         }
     }
@@ -227,9 +243,16 @@ vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipSt
 vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s)
 {
     const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, S);
-    const size_t lds = sizeof(float) * (size_t)kBpViews * a.W;
+    const size_t lds = sizeof(float) * ((size_t)kBpViews * a.W + 1) + sizeof(double) * 2 * kBpViews;
     if (lds > 64 * 1024) return set_error(VCT_ERR_UNSUPPORTED, "W=%d too large for the LDS view chunk", a.W);
-    hipLaunchKernelGGL(k_backproject, grid, dim3(kBpTile * kBpTile), lds, s, a);
+    // largest |tan(gamma)| of a masked pixel: the tangent from the source to
+    // the mask circle of radius r (pixel centres lie within mask_r pitches)
+    const double r = a.mask_r * a.pix;
+    const bool wide = !(r < a.sid) || r / std::sqrt(a.sid * a.sid - r * r) > 0.49;
+    if (wide)
+        hipLaunchKernelGGL(k_backproject<true>, grid, dim3(kBpTile * kBpTile), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_backproject<false>, grid, dim3(kBpTile * kBpTile), lds, s, a);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
