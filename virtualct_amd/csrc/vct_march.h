@@ -66,13 +66,16 @@ __device__ __forceinline__ int index_of(double h, double D, double invD, double 
 // The value is only trusted away from the boundaries, so the fused
 // multiply-add (one rounding fewer) is as good as the separate operations,
 // and |f - 0.5| < 0.5 - margin is the two-sided test f in (margin, 1 - margin).
+// A NaN or infinite q fails that test (f is NaN), and every index whose value
+// is used comes from a point within a few image widths of the image, far
+// inside the int range, so no separate range test is needed.
 __device__ __forceinline__ bool index_checked(double h, double invD, double c, int& out)
 {
     const double q = __builtin_fma(h, invD, c);
     const double t = trunc(q);
     const double f = fabs(q - t);
     out = (int)t;
-    return fabs(f - 0.5) < 0.5 - kMarchMargin && fabs(q) < 1.0e6;
+    return fabs(f - 0.5) < 0.5 - kMarchMargin;
 }
 
 // (u/fabs(u) != v/fabs(v)) of M/reprojection_TOMO.cpp:134,221 for finite
