@@ -150,3 +150,24 @@ def test_bhc_driver(gpu, golden, oracle, tmp_path):
     ref, mn0, mn1 = oracle.beam_hardening(p, 0.01, 2.0)
     assert rel_max(out, ref) < 1e-6
     assert r.stdout.strip() == f"{mn0:f} {mn1:f}"
+
+
+def test_wide_field_backprojection_takes_library_atan(gpu, oracle):
+    """A 76.8 cm field (N=128, 0.6 cm) puts masked pixels at |tan(gamma)| up to
+    0.53, beyond the polynomial: the kernel variant with the library atan
+    fallback, against the oracle."""
+    ops = _ops()
+    N, pix, V = 128, 0.6, 160
+    g = Geom(N=N, pix=pix, V=V)
+    r = (N - 1) / 2.0 * pix
+    assert r / np.sqrt(85.0 ** 2 - r ** 2) > 0.5
+    rng = np.random.default_rng(31)
+    fl = rng.normal(size=(V, 609)).astype(np.float32) * 1e3
+    plan = ops.plan_for(_geom(N, pix, V))
+    img = ops.backproject(plan, _t(fl, gpu))[0].cpu().numpy()
+    ref = oracle.backproject_ref(g, fl)
+    f64 = oracle.backproject_f64(g, fl.astype(np.float64))
+    assert rel_max(img, ref) < 1e-5
+    assert rel_max(img, f64) < 1e-5
+    mask = oracle.mask(N).astype(bool)
+    assert np.all(img[~mask] == 0)

@@ -145,16 +145,28 @@ __device__ __forceinline__ double rcp_f64(double x)
     return __builtin_fma(r, e, r);
 }
 
+// Two lanes per pixel (the even and the odd lane of a pair), each forming
+// half of a chunk's view terms: 512 threads for a 16x16 tile, so a 512x512
+// image gives 8 waves per SIMD to hide the FP64 and LDS latencies.  The float
+// accumulation keeps the reference's view order exactly: the even lane adds
+// views 0..7 of the chunk to the running sum, hands it to the odd lane, which
+// adds views 8..15, and the result returns to both lanes.
+//
 // WIDE: some masked pixel may see |tan(gamma)| > 0.5, beyond the polynomial;
 // those take the library atan.  The host picks the variant from the bound
 // |tan(gamma)| <= r / sqrt(sid^2 - r^2) over the mask radius r; without the
-// fallback the view loop has no branch and the compiler overlaps the views.
+// fallback the view loop has no branch.
+constexpr int kBpHalf = kBpViews / 2;
+
 template <bool WIDE>
-__global__ __launch_bounds__(256) void k_backproject(BackprojArgs a)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 8))) void k_backproject(BackprojArgs a)
 {
     extern __shared__ float sv[];  // [kBpViews][W] filtered views, then [kBpViews][2] cos/sin (doubles)
     double* trig = reinterpret_cast<double*>(sv + kBpViews * a.W + ((kBpViews * a.W) & 1));
-    const int tx = threadIdx.x & (kBpTile - 1), ty = threadIdx.x / kBpTile;
+    const int half = threadIdx.x & 1;
+    const int pix = threadIdx.x >> 1;
+    const int lane = threadIdx.x & 63;
+    const int tx = pix & (kBpTile - 1), ty = pix / kBpTile;
     const int x = blockIdx.x * kBpTile + tx;
     const int y = blockIdx.y * kBpTile + ty;
     const int s = blockIdx.z;
@@ -169,20 +181,33 @@ __global__ __launch_bounds__(256) void k_backproject(BackprojArgs a)
     for (int th0 = 0; th0 < a.use_views; th0 += kBpViews) {
         const int nv = min(kBpViews, a.use_views - th0);
         __syncthreads();
-        for (int i = threadIdx.x; i < nv * a.W; i += blockDim.x)
-            sv[i] = fl[(size_t)th0 * a.W + i];
+        const float* src = fl + (size_t)th0 * a.W;
+        const int n = nv * a.W;
+        if (a.vec4) {  // 16-byte aligned chunk (host-checked): a quarter of the load instructions
+            const float4* s4 = reinterpret_cast<const float4*>(src);
+            float4* d4 = reinterpret_cast<float4*>(sv);
+            for (int i = threadIdx.x; i < n / 4; i += blockDim.x) d4[i] = s4[i];
+            for (int i = n / 4 * 4 + threadIdx.x; i < n; i += blockDim.x) sv[i] = src[i];
+        } else {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) sv[i] = src[i];
+        }
         if (threadIdx.x < kBpViews) {
             const int th = th0 + min((int)threadIdx.x, nv - 1);
             trig[2 * threadIdx.x] = a.cosT[th];
             trig[2 * threadIdx.x + 1] = a.sinT[th];
         }
         __syncthreads();
-        if (masked) {
-            // The views' terms are independent FP64 chains the compiler can
-            // overlap; each is added to acc in view order (the reference's
-            // float accumulation order) and only when the reference adds it.
+        float term[kBpHalf];
+        bool ok[kBpHalf];
 #pragma unroll
-            for (int u = 0; u < kBpViews; u++) {
+        for (int v = 0; v < kBpHalf; v++) {
+            term[v] = 0.f;
+            ok[v] = false;
+        }
+        if (masked) {
+#pragma unroll
+            for (int v = 0; v < kBpHalf; v++) {
+                const int u = half * kBpHalf + v;
                 const int tu = u < nv ? u : nv - 1;  // a clamped view is computed, not added
                 const double c = trig[2 * tu], sn = trig[2 * tu + 1];
                 const double X = xx * c + yy * sn;
@@ -195,16 +220,23 @@ __global__ __launch_bounds__(256) void k_backproject(BackprojArgs a)
                 const double fGAM = gam * a.inv_dbeta_ch + a.cent;
                 const int GAM = trunc_i32_bp(fGAM);
                 const double w = fGAM - GAM;
-                const bool ok = u < nv && GAM >= 0 && GAM < a.W - 1;
-                const float* row = sv + tu * a.W + (ok ? GAM : 0);
+                ok[v] = u < nv && GAM >= 0 && GAM < a.W - 1;
+                const float* row = sv + tu * a.W + (ok[v] ? GAM : 0);
                 const double sum1 = (1 - w) * row[0] + w * row[1];
                 const double sum = sum1 + 0.0;
-                const float term = (float)(rcp_f64(L2) * sum * a.dbeta_view * a.xfactor);
-                acc = ok ? acc + term : acc;
+                term[v] = (float)(rcp_f64(L2) * sum * a.dbeta_view * a.xfactor);
             }
         }
+        // ordered merge, all lanes converged: even lane's views, then the odd lane's
+        float se = acc;
+#pragma unroll
+        for (int v = 0; v < kBpHalf; v++) se = ok[v] ? se + term[v] : se;
+        float so = __shfl(se, lane & ~1);
+#pragma unroll
+        for (int v = 0; v < kBpHalf; v++) so = ok[v] ? so + term[v] : so;
+        acc = __shfl(so, lane | 1);
     }
-    if (in_img) a.img[((size_t)s * a.N + y) * a.N + x] = acc;
+    if (in_img && half == 0) a.img[((size_t)s * a.N + y) * a.N + x] = acc;
 }
 
 // The beam-hardening tool M/beam_hardening_correction.cpp:47-60 elementwise:
@@ -240,8 +272,11 @@ vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipSt
     return VCT_OK;
 }
 
-vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s)
+vct_status launch_backproject(const BackprojArgs& args, int S, hipStream_t s)
 {
+    BackprojArgs a = args;
+    // every chunk (and every sinogram of the stack) starts on a 16-byte boundary
+    a.vec4 = ((uintptr_t)a.fltd % 16 == 0) && ((size_t)a.V * a.W % 4 == 0) && ((size_t)kBpViews * a.W % 4 == 0);
     const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, S);
     const size_t lds = sizeof(float) * ((size_t)kBpViews * a.W + 1) + sizeof(double) * 2 * kBpViews;
     if (lds > 64 * 1024) return set_error(VCT_ERR_UNSUPPORTED, "W=%d too large for the LDS view chunk", a.W);
@@ -250,9 +285,9 @@ vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s)
     const double r = a.mask_r * a.pix;
     const bool wide = !(r < a.sid) || r / std::sqrt(a.sid * a.sid - r * r) > 0.49;
     if (wide)
-        hipLaunchKernelGGL(k_backproject<true>, grid, dim3(kBpTile * kBpTile), lds, s, a);
+        hipLaunchKernelGGL(k_backproject<true>, grid, dim3(2 * kBpTile * kBpTile), lds, s, a);
     else
-        hipLaunchKernelGGL(k_backproject<false>, grid, dim3(kBpTile * kBpTile), lds, s, a);
+        hipLaunchKernelGGL(k_backproject<false>, grid, dim3(2 * kBpTile * kBpTile), lds, s, a);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
