@@ -171,3 +171,16 @@ def test_wide_field_backprojection_takes_library_atan(gpu, oracle):
     assert rel_max(img, f64) < 1e-5
     mask = oracle.mask(N).astype(bool)
     assert np.all(img[~mask] == 0)
+
+
+def test_backprojection_more_views_than_one_trig_block(gpu, oracle):
+    """2100 views: the view cos/sin are staged in blocks of 2048 views."""
+    ops = _ops()
+    N, pix, V = 64, 0.7, 2100
+    g = Geom(N=N, pix=pix, V=V)
+    fl = np.random.default_rng(32).normal(size=(V, 609)).astype(np.float32)
+    plan = ops.plan_for(_geom(N, pix, V))
+    img = ops.backproject(plan, _t(fl, gpu))[0].cpu().numpy()
+    ref = oracle.backproject_ref(g, fl)
+    assert rel_max(img, ref) < 1e-5
+    assert np.count_nonzero(img != ref) < 0.05 * img.size

@@ -14,7 +14,7 @@
 //                  load per four MFMAs; the two halves meet in LDS.
 //   k_backproject  pixel-driven fan backprojection
 //                  (F/reconstructImageTOMO.cpp:46-100): a 16x16 pixel tile per
-//                  workgroup, chunks of filtered views staged in LDS, FP64
+//                  workgroup, two lanes per pixel, the view cos/sin in LDS, FP64
 //                  geometry and the reference's float accumulation order.
 #include "vct_internal.h"
 
@@ -146,23 +146,25 @@ __device__ __forceinline__ double rcp_f64(double x)
 }
 
 // Two lanes per pixel (the even and the odd lane of a pair), each forming
-// half of a chunk's view terms: 512 threads for a 16x16 tile, so a 512x512
-// image gives 8 waves per SIMD to hide the FP64 and LDS latencies.  The float
-// accumulation keeps the reference's view order exactly: the even lane adds
-// views 0..7 of the chunk to the running sum, hands it to the odd lane, which
-// adds views 8..15, and the result returns to both lanes.
+// half of each group of 16 views' terms: 512 threads for a 16x16 tile, so a
+// 512x512 image gives 8 waves per SIMD to hide the FP64 and memory latencies.
+// The float accumulation keeps the reference's view order exactly: the even
+// lane adds views 0..7 of the group to the running sum, hands it to the odd
+// lane, which adds views 8..15, and the result returns to both lanes.  The
+// filtered sinogram (1.95 MB at 800 x 609) is read through L2 without
+// staging; the view cos/sin go to LDS once per block of kBpTrig views.
 //
 // WIDE: some masked pixel may see |tan(gamma)| > 0.5, beyond the polynomial;
 // those take the library atan.  The host picks the variant from the bound
 // |tan(gamma)| <= r / sqrt(sid^2 - r^2) over the mask radius r; without the
 // fallback the view loop has no branch.
 constexpr int kBpHalf = kBpViews / 2;
+constexpr int kBpTrig = 2048;  // views whose cos/sin sit in LDS at a time (32 KB)
 
 template <bool WIDE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 8))) void k_backproject(BackprojArgs a)
 {
-    extern __shared__ float sv[];  // [kBpViews][W] filtered views, then [kBpViews][2] cos/sin (doubles)
-    double* trig = reinterpret_cast<double*>(sv + kBpViews * a.W + ((kBpViews * a.W) & 1));
+    __shared__ double trig[2 * kBpTrig];  // [view][cos, sin]
     const int half = threadIdx.x & 1;
     const int pix = threadIdx.x >> 1;
     const int lane = threadIdx.x & 63;
@@ -178,25 +180,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 
     const double xx = -(x - a.center) * a.pix;  // :52-53
     const double yy = -(y - a.center) * a.pix;
     float acc = 0.f;
-    for (int th0 = 0; th0 < a.use_views; th0 += kBpViews) {
-        const int nv = min(kBpViews, a.use_views - th0);
-        __syncthreads();
-        const float* src = fl + (size_t)th0 * a.W;
-        const int n = nv * a.W;
-        if (a.vec4) {  // 16-byte aligned chunk (host-checked): a quarter of the load instructions
-            const float4* s4 = reinterpret_cast<const float4*>(src);
-            float4* d4 = reinterpret_cast<float4*>(sv);
-            for (int i = threadIdx.x; i < n / 4; i += blockDim.x) d4[i] = s4[i];
-            for (int i = n / 4 * 4 + threadIdx.x; i < n; i += blockDim.x) sv[i] = src[i];
-        } else {
-            for (int i = threadIdx.x; i < n; i += blockDim.x) sv[i] = src[i];
+    for (int g0 = 0; g0 < a.use_views; g0 += kBpViews) {
+        const int gt = g0 % kBpTrig;  // groups never straddle a cos/sin block
+        if (gt == 0) {
+            __syncthreads();
+            const int nb = min(kBpTrig, a.use_views - g0);
+            for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+                trig[2 * i] = a.cosT[g0 + i];
+                trig[2 * i + 1] = a.sinT[g0 + i];
+            }
+            __syncthreads();
         }
-        if (threadIdx.x < kBpViews) {
-            const int th = th0 + min((int)threadIdx.x, nv - 1);
-            trig[2 * threadIdx.x] = a.cosT[th];
-            trig[2 * threadIdx.x + 1] = a.sinT[th];
-        }
-        __syncthreads();
+        const int nv = min(kBpViews, a.use_views - g0);
         float term[kBpHalf];
         bool ok[kBpHalf];
 #pragma unroll
@@ -209,7 +204,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 
             for (int v = 0; v < kBpHalf; v++) {
                 const int u = half * kBpHalf + v;
                 const int tu = u < nv ? u : nv - 1;  // a clamped view is computed, not added
-                const double c = trig[2 * tu], sn = trig[2 * tu + 1];
+                const double c = trig[2 * (gt + tu)], sn = trig[2 * (gt + tu) + 1];
                 const double X = xx * c + yy * sn;
                 const double Y = -xx * sn + yy * c;
                 const double den = a.sid - Y;
@@ -221,13 +216,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 
                 const int GAM = trunc_i32_bp(fGAM);
                 const double w = fGAM - GAM;
                 ok[v] = u < nv && GAM >= 0 && GAM < a.W - 1;
-                const float* row = sv + tu * a.W + (ok[v] ? GAM : 0);
+                const float* row = fl + (size_t)(g0 + tu) * a.W + (ok[v] ? GAM : 0);
                 const double sum1 = (1 - w) * row[0] + w * row[1];
                 const double sum = sum1 + 0.0;
                 term[v] = (float)(rcp_f64(L2) * sum * a.dbeta_view * a.xfactor);
             }
         }
-        // ordered merge, all lanes converged: even lane's views, then the odd lane's
+        // ordered merge, all lanes converged: the even lane's views, then the odd lane's
         float se = acc;
 #pragma unroll
         for (int v = 0; v < kBpHalf; v++) se = ok[v] ? se + term[v] : se;
@@ -272,22 +267,17 @@ vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipSt
     return VCT_OK;
 }
 
-vct_status launch_backproject(const BackprojArgs& args, int S, hipStream_t s)
+vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s)
 {
-    BackprojArgs a = args;
-    // every chunk (and every sinogram of the stack) starts on a 16-byte boundary
-    a.vec4 = ((uintptr_t)a.fltd % 16 == 0) && ((size_t)a.V * a.W % 4 == 0) && ((size_t)kBpViews * a.W % 4 == 0);
     const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, S);
-    const size_t lds = sizeof(float) * ((size_t)kBpViews * a.W + 1) + sizeof(double) * 2 * kBpViews;
-    if (lds > 64 * 1024) return set_error(VCT_ERR_UNSUPPORTED, "W=%d too large for the LDS view chunk", a.W);
     // largest |tan(gamma)| of a masked pixel: the tangent from the source to
     // the mask circle of radius r (pixel centres lie within mask_r pitches)
     const double r = a.mask_r * a.pix;
     const bool wide = !(r < a.sid) || r / std::sqrt(a.sid * a.sid - r * r) > 0.49;
     if (wide)
-        hipLaunchKernelGGL(k_backproject<true>, grid, dim3(2 * kBpTile * kBpTile), lds, s, a);
+        hipLaunchKernelGGL(k_backproject<true>, grid, dim3(2 * kBpTile * kBpTile), 0, s, a);
     else
-        hipLaunchKernelGGL(k_backproject<false>, grid, dim3(2 * kBpTile * kBpTile), lds, s, a);
+        hipLaunchKernelGGL(k_backproject<false>, grid, dim3(2 * kBpTile * kBpTile), 0, s, a);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }

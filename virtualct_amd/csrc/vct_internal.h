@@ -104,7 +104,6 @@ struct BackprojArgs {
     const double* sinT;   // [V]
     float* img;           // [S][N][N]
     int N, W, V, use_views, cent, half;
-    int vec4;             // view chunks start 16-byte aligned (set by the launcher)
     double pix, sid, dbeta_ch, inv_dbeta_ch, dbeta_view, xfactor, center, mask_r2, mask_r;
 };
 
