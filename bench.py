@@ -120,15 +120,23 @@ def bench_decomposition(B=16, iters=50, plain_iters=1):
     sm = batch.SystemMatrix(plan, transpose=True)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
-    # plain loop: per phantom, per spectrum forward + adjoint
-    plain = decompose.PlainLoop(plan, targets, att, pdf, init)
-    plain.step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(plain_iters):
+    # plain loops: per phantom, per spectrum forward + adjoint through autograd,
+    # with the ray-driven atomic adjoint (the plain path) and with the gather
+    # over the plan's cached system matrix
+    plain_ms = {}
+    mode0 = plan.adjoint_mode
+    for mode in ("atomic", "sysmat"):
+        plan.adjoint_mode = mode
+        plain = decompose.PlainLoop(plan, targets, att, pdf, init)
         plain.step()
-    torch.cuda.synchronize()
-    plain_step = (time.perf_counter() - t0) / plain_iters
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(plain_iters):
+            plain.step()
+        torch.cuda.synchronize()
+        plain_ms[mode] = (time.perf_counter() - t0) / plain_iters
+    plan.adjoint_mode = mode0
+    plain_step = plain_ms["atomic"]
     # batched, graph-captured
     solver = batch.BatchedSolver(sm, targets, att, pdf, init)
     err0 = [decompose.reprojection_error(plan, init[b], targets[b], att, pdf) for b in range(2)]
@@ -146,13 +154,16 @@ def bench_decomposition(B=16, iters=50, plain_iters=1):
         "batched_steps_per_s": 1.0 / batched_step, "plain_steps_per_s": 1.0 / plain_step,
         "batched_vs_plain": plain_step / batched_step,
         "batched_ms_per_step": batched_step * 1e3, "plain_ms_per_step": plain_step * 1e3,
+        "plain_sysmat_steps_per_s": 1.0 / plain_ms["sysmat"],
+        "batched_vs_plain_sysmat": plain_ms["sysmat"] / batched_step,
         "system_matrix_build_s": build_s, "system_matrix_segments": sm.segments,
         "system_matrix_bytes": sm.bytes,
         "reprojection_rel_rms_before": err0, "reprojection_rel_rms_after": err1,
         "definition": "one step = forward of both spectra, log-domain loss, adjoint and projected BB update "
-                      "for all phantoms; plain = per phantom and per spectrum on-the-fly trace + spectral sum "
-                      "+ adjoint trace; batched = cached system matrix, one gather launch per direction, "
-                      "HIP-graph replay",
+                      "for all phantoms; plain = per phantom and per spectrum autograd calls: on-the-fly trace "
+                      "+ spectral sum, adjoint by the ray-driven transposed march with atomics; plain_sysmat = "
+                      "the same with the adjoint gathered over the plan's cached system matrix; batched = cached "
+                      "system matrix, one gather launch per direction for all phantoms, HIP-graph replay",
     }
 
 

@@ -107,8 +107,12 @@ struct RayConst {
 // The march from the entry point (M/reprojection_TOMO.cpp:113-283, either
 // branch) for one ray.  FAST selects reciprocal products with guarded
 // decisions (see the file comment) over exact divisions.
-template <bool FAST, class Sink>
-__device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r, Sink& sink, bool& ovf)
+// SYNC: the loop is wave-uniform and the segments of a committed fast step
+// go to Sink::add2_sync, which every lane calls once per step (see
+// march_ray_sync); alive_in = false makes a lane march nothing.
+template <bool FAST, bool SYNC = false, class Sink>
+__device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r, Sink& sink, bool& ovf,
+                                          bool alive_in = true)
 {
     const bool xmajor = r.xmajor;
     const double ai = r.ai, af = r.af, bi = r.bi, bf = r.bf;
@@ -190,100 +194,121 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
     const double nxm = (double)a.nxm;
     auto in_image = [&](int ia, int ib) { return ia >= 0 && ib >= 0 && ia <= a.N - 1 && ib <= a.N - 1; };
 
+    bool alive = alive_in;
     for (int ik = 0;; ik++) {
-        if (!(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) break;
-        if (ik >= a.max_steps) { ovf = true; break; }
-        const int bmb = bma;
-        const double bj = pc.b;
-        if (FAST) {
-            // The whole step from the fast values with every margin test
-            // folded into one flag: when all of them pass, the general step
-            // below would make no point exact and take exactly these
-            // decisions with exactly these values, so the step is committed
-            // here without its branches; otherwise the general step runs.
-            const double na = ain + pha * ik * D;
-            const double nb = b_calc(na);
-            const double nab = fabs(nb);
-            int nbma;
-            bool sure = index_checked(nb, a.invD, 0.0, nbma);
-            sure &= (nab >= kMarchMargin) & (fabs(nab - Lb) >= kMarchMargin);
-            const bool cand = (ik != 0) & ((nbma != bmb) | sign_split(nb, bj));
-            const double xbb = (double)(nbma * D);
-            const double xb = ((nb < 0 && b_neg) || (nb > 0 && b_pos)) ? xbb : xbb - bdir * D;
-            const double xa = a_calc(xb);
-            const double dxa = na - xa, dxb = nb - xb;
-            sure &= !(cand & (fabs(dxa) < kMarchMargin) & (fabs(dxb) < kMarchMargin));
-            const bool cross = cand & ((dxa * dxa + dxb * dxb) != 0);
-            int ia1, ib1, ia2, ib2;
-            const bool ok1 = index_checked(sA * ((p0.a + xa) * 0.5), a.invD, nxm, ia1) &
-                             index_checked(-sA * ((p0.b + xb) * 0.5), a.invD, nxm, ib1);
-            const double s_a = cross ? xa : p0.a, s_b = cross ? xb : p0.b;
-            const bool ok2 = index_checked(sA * ((s_a + na) * 0.5), a.invD, nxm, ia2) &
-                             index_checked(-sA * ((s_b + nb) * 0.5), a.invD, nxm, ib2);
-            sure &= (ok1 | !cross) & ok2;
-            if (sure) {
-                const bool v1 = cross && in_image(ia1, ib1);
-                const bool v2 = in_image(ia2, ib2);
-                iks += cross ? 2 : 1;
-                if constexpr (Sink::kAdd2) {
-                    // empty segments: length 0 at voxel 0, which adds nothing
-                    sink.add2(v1 ? ia1 * stA + ib1 * stB : 0, v1 ? fabs(xa - p0.a) * sec : 0.0,
-                              v2 ? ia2 * stA + ib2 * stB : 0, v2 ? fabs(na - s_a) * sec : 0.0);
-                } else {
-                    if (v1) sink.add(ia1 * stA + ib1 * stB, fabs(xa - p0.a) * sec);
-                    if (v2) sink.add(ia2 * stA + ib2 * stB, fabs(na - s_a) * sec);
-                }
-                bma = nbma;
-                pc = MarchPoint{na, nb, 1, false};
-                p0 = pc;
-                continue;
+        if constexpr (SYNC) {
+            // wave-uniform loop: finished lanes idle until every lane is done
+            if (alive && !(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) alive = false;
+            if (alive && ik >= a.max_steps) {
+                ovf = true;
+                alive = false;
             }
-        }
-        pc.a = ain + pha * ik * D;  // :126
-        pc.b = b_calc(pc.a);        // :129
-        pc.exact = !FAST;
-        if (FAST) {
-            const double ab = fabs(pc.b);
-            if (ab < kMarchMargin || fabs(ab - Lb) < kMarchMargin) make_exact(pc);
-            if (!index_checked(pc.b, a.invD, 0.0, bma)) {
-                make_exact(pc);
-                bma = trunc_i32(pc.b / D);  // :130
-            }
+            if (!__any(alive)) break;
         } else {
-            bma = index_of<false>(pc.b, D, a.invD, 0.0);
+            if (!(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) break;
+            if (ik >= a.max_steps) { ovf = true; break; }
         }
-        const double bii = pc.b;
-        if (ik != 0 && (bma != bmb || sign_split(bii, bj))) {  // :134
-            MarchPoint px{0.0, 0.0, 2, !FAST};
-            if ((bii < 0 && b_neg) || (bii > 0 && b_pos))
-                px.b = (double)(bma * D);
-            else
-                px.b = (double)(bma * D) - bdir * D;
-            px.a = a_calc(px.b);  // :137
-            if (FAST && fabs(pc.a - px.a) < kMarchMargin && fabs(pc.b - px.b) < kMarchMargin) {
-                make_exact(pc);
-                make_exact(px);
+        int ev1 = -1, ev2 = -1;  // SYNC: this step's committed segments (-1: none)
+        double er1 = 0.0, er2 = 0.0;
+        if (alive) {
+            const int bmb = bma;
+            const double bj = pc.b;
+            bool committed = false;
+            if (FAST) {
+                // The whole step from the fast values with every margin test
+                // folded into one flag: when all of them pass, the general step
+                // below would make no point exact and take exactly these
+                // decisions with exactly these values, so the step is committed
+                // here without its branches; otherwise the general step runs.
+                const double na = ain + pha * ik * D;
+                const double nb = b_calc(na);
+                const double nab = fabs(nb);
+                int nbma;
+                bool sure = index_checked(nb, a.invD, 0.0, nbma);
+                sure &= (nab >= kMarchMargin) & (fabs(nab - Lb) >= kMarchMargin);
+                const bool cand = (ik != 0) & ((nbma != bmb) | sign_split(nb, bj));
+                const double xbb = (double)(nbma * D);
+                const double xb = ((nb < 0 && b_neg) || (nb > 0 && b_pos)) ? xbb : xbb - bdir * D;
+                const double xa = a_calc(xb);
+                const double dxa = na - xa, dxb = nb - xb;
+                sure &= !(cand & (fabs(dxa) < kMarchMargin) & (fabs(dxb) < kMarchMargin));
+                const bool cross = cand & ((dxa * dxa + dxb * dxb) != 0);
+                int ia1, ib1, ia2, ib2;
+                const bool ok1 = index_checked(sA * ((p0.a + xa) * 0.5), a.invD, nxm, ia1) &
+                                 index_checked(-sA * ((p0.b + xb) * 0.5), a.invD, nxm, ib1);
+                const double s_a = cross ? xa : p0.a, s_b = cross ? xb : p0.b;
+                const bool ok2 = index_checked(sA * ((s_a + na) * 0.5), a.invD, nxm, ia2) &
+                                 index_checked(-sA * ((s_b + nb) * 0.5), a.invD, nxm, ib2);
+                sure &= (ok1 | !cross) & ok2;
+                if (sure) {
+                    const bool v1 = cross && in_image(ia1, ib1);
+                    const bool v2 = in_image(ia2, ib2);
+                    iks += cross ? 2 : 1;
+                    if constexpr (SYNC) {
+                        ev1 = v1 ? ia1 * stA + ib1 * stB : -1;
+                        er1 = fabs(xa - p0.a) * sec;
+                        ev2 = v2 ? ia2 * stA + ib2 * stB : -1;
+                        er2 = fabs(na - s_a) * sec;
+                    } else if constexpr (Sink::kAdd2) {
+                        // empty segments: length 0 at voxel 0, which adds nothing
+                        sink.add2(v1 ? ia1 * stA + ib1 * stB : 0, v1 ? fabs(xa - p0.a) * sec : 0.0,
+                                  v2 ? ia2 * stA + ib2 * stB : 0, v2 ? fabs(na - s_a) * sec : 0.0);
+                    } else {
+                        if (v1) sink.add(ia1 * stA + ib1 * stB, fabs(xa - p0.a) * sec);
+                        if (v2) sink.add(ia2 * stA + ib2 * stB, fabs(na - s_a) * sec);
+                    }
+                    bma = nbma;
+                    pc = MarchPoint{na, nb, 1, false};
+                    p0 = pc;
+                    committed = true;
+                }
             }
-            const double chy = (pc.a - px.a) * (pc.a - px.a) + (pc.b - px.b) * (pc.b - px.b);
-            if (chy != 0) {  // :141
+            if (!committed) {
+                pc.a = ain + pha * ik * D;  // :126
+                pc.b = b_calc(pc.a);        // :129
+                pc.exact = !FAST;
+                if (FAST) {
+                    const double ab = fabs(pc.b);
+                    if (ab < kMarchMargin || fabs(ab - Lb) < kMarchMargin) make_exact(pc);
+                    if (!index_checked(pc.b, a.invD, 0.0, bma)) {
+                        make_exact(pc);
+                        bma = trunc_i32(pc.b / D);  // :130
+                    }
+                } else {
+                    bma = index_of<false>(pc.b, D, a.invD, 0.0);
+                }
+                const double bii = pc.b;
+                if (ik != 0 && (bma != bmb || sign_split(bii, bj))) {  // :134
+                    MarchPoint px{0.0, 0.0, 2, !FAST};
+                    if ((bii < 0 && b_neg) || (bii > 0 && b_pos))
+                        px.b = (double)(bma * D);
+                    else
+                        px.b = (double)(bma * D) - bdir * D;
+                    px.a = a_calc(px.b);  // :137
+                    if (FAST && fabs(pc.a - px.a) < kMarchMargin && fabs(pc.b - px.b) < kMarchMargin) {
+                        make_exact(pc);
+                        make_exact(px);
+                    }
+                    const double chy = (pc.a - px.a) * (pc.a - px.a) + (pc.b - px.b) * (pc.b - px.b);
+                    if (chy != 0) {  // :141
+                        iks++;
+                        emit(p0, px);
+                        p0 = px;
+                    }
+                }
                 iks++;
-                emit(p0, px);
-                p0 = px;
+                emit(p0, pc);
+                p0 = pc;
             }
         }
-        iks++;
-        emit(p0, pc);
-        p0 = pc;
+        if constexpr (SYNC) sink.add2_sync(ev1, er1, ev2, er2);  // every lane of the wave
     }
     return iks;
 }
 
-// Trace ray (view, ch).  Calls sink.add(voxel, length) for every segment
-// whose midpoint voxel lies in the image, in the reference's order, and
-// returns the reference's per-ray segment counter (iks).  ovf is set when
-// the step bound is hit (never for valid geometry).
-template <bool FAST_IN, class Sink>
-__device__ __forceinline__ int march_ray(const TraceArgs& a, int view, int ch, Sink& sink, bool& ovf)
+// Entry, exit and march constants of ray (view, ch) (M/reprojection_TOMO.cpp:53-113);
+// false when the ray misses the image.
+__device__ __forceinline__ bool setup_ray(const TraceArgs& a, int view, int ch, RayConst& r)
 {
     // source and detector point, M/reprojection_TOMO.cpp:53-74
     const double c = a.cosT[view], s = a.sinT[view];
@@ -328,7 +353,7 @@ __device__ __forceinline__ int march_ray(const TraceArgs& a, int view, int ch, S
             m++;
         }
     }
-    if (m < 2) return 0;  // the ray misses the image: no segment, transmission 1
+    if (m < 2) return false;  // the ray misses the image: no segment, transmission 1
 
     // entry = the intersection closer to the source, :103-108
     const double d0 = (h0x - sx) * (h0x - sx) + (h0y - sy) * (h0y - sy);
@@ -345,14 +370,42 @@ __device__ __forceinline__ int march_ray(const TraceArgs& a, int view, int ch, S
     const double sa = xmajor ? sx : sy, sb = xmajor ? sy : sx;
     const double pa = xmajor ? px : py, pb = xmajor ? py : px;
 
-    RayConst r;
     r.xmajor = xmajor;
     r.ai = ai; r.af = af; r.bi = bi; r.bf = bf;
     r.sa = sa; r.sb = sb; r.dA = pa - sa; r.dB = pb - sb;
     r.Lb = Lb;
+    return true;
+}
+
+// Trace ray (view, ch).  Calls sink.add(voxel, length) for every segment
+// whose midpoint voxel lies in the image, in the reference's order, and
+// returns the reference's per-ray segment counter (iks).  ovf is set when
+// the step bound is hit (never for valid geometry).
+template <bool FAST_IN, class Sink>
+__device__ __forceinline__ int march_ray(const TraceArgs& a, int view, int ch, Sink& sink, bool& ovf)
+{
+    RayConst r;
+    if (!setup_ray(a, view, ch, r)) return 0;
     // the fast path needs finite reciprocals and no r >= rmax test
     if (FAST_IN && r.dB != 0.0 && !a.rcheck && !a.exact) return march_loop<true>(a, r, sink, ovf);
     return march_loop<false>(a, r, sink, ovf);
+}
+
+// The same march for a sink that combines the segments of the lanes of a
+// wave before writing (Sink::add2_sync uses cross-lane shuffles): every lane
+// of the wave must call it.  Lanes with active == false, rays that miss the
+// image and rays that finish early stay in the wave-uniform fast loop as
+// lanes without segments; rays that need the exact march run it first,
+// adding through Sink::add, and then idle in the same way.
+template <class Sink>
+__device__ __forceinline__ void march_ray_sync(const TraceArgs& a, bool active, int view, int ch, Sink& sink,
+                                               bool& ovf)
+{
+    RayConst r{};
+    const bool hit = active && setup_ray(a, view, ch, r);
+    const bool fast = hit && r.dB != 0.0 && !a.rcheck && !a.exact;
+    if (hit && !fast) march_loop<false>(a, r, sink, ovf);
+    march_loop<true, true>(a, r, sink, ovf, fast);
 }
 
 }  // namespace vct

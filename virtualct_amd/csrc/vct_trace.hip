@@ -102,31 +102,81 @@ struct SinkF32 {
     }
 };
 
-// Adjoint sink: grad[v][m] += w[m] * rr for every in-image segment.
+// Adjoint sink: grad[v][m] += w[m] * rr for every in-image segment.  The
+// segments the lanes of a wave emit in one march step (add2_sync, called by
+// every lane) are first combined over runs of consecutive lanes with the same
+// voxel -- adjacent channels of one view often cross the same voxel at the
+// same step -- by a segmented suffix sum over the wave; the first lane of each
+// run then issues the atomics for the whole run.  Segments of the general
+// (exact-fallback) step go straight to add.
 template <int M>
 struct SinkAdj {
     static constexpr bool kAdd2 = false;
     float* __restrict__ grad;
     const float4* __restrict__ dens;  // f32 densities for the sign test, or null
     float w[M];
-    __device__ __forceinline__ void add(int v, double rr)
+    __device__ __forceinline__ void contributions(int v, float r, float (&c)[M]) const
     {
-        const float r = (float)rr;
         float keep[8] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
         if (dens) {
-            const float4 a = dens[2 * v];
-            keep[0] = a.x >= 0.f; keep[1] = a.y >= 0.f; keep[2] = a.z >= 0.f; keep[3] = a.w >= 0.f;
+            const float4 lo = dens[2 * v];
+            keep[0] = lo.x >= 0.f; keep[1] = lo.y >= 0.f; keep[2] = lo.z >= 0.f; keep[3] = lo.w >= 0.f;
             if (M > 4) {
-                const float4 b = dens[2 * v + 1];
-                keep[4] = b.x >= 0.f; keep[5] = b.y >= 0.f; keep[6] = b.z >= 0.f; keep[7] = b.w >= 0.f;
+                const float4 hi = dens[2 * v + 1];
+                keep[4] = hi.x >= 0.f; keep[5] = hi.y >= 0.f; keep[6] = hi.z >= 0.f; keep[7] = hi.w >= 0.f;
             }
         }
+#pragma unroll
+        for (int m = 0; m < M; m++) c[m] = w[m] * r * keep[m];
+    }
+    __device__ __forceinline__ void scatter(int v, const float (&c)[M])
+    {
         float* dst = grad + (size_t)v * 8;
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            const float val = w[m] * r * keep[m];
-            if (val != 0.f) unsafeAtomicAdd(dst + m, val);
+        for (int m = 0; m < M; m++)
+            if (c[m] != 0.f) unsafeAtomicAdd(dst + m, c[m]);
+    }
+    __device__ __forceinline__ void add(int v, double rr)
+    {
+        float c[M];
+        contributions(v, (float)rr, c);
+        scatter(v, c);
+    }
+    // v < 0: this lane has no segment in this slot
+    __device__ __forceinline__ void reduced(int v, double rr)
+    {
+        const int lane = __lane_id();
+        const int key = v >= 0 ? v : -1 - lane;  // lanes without a segment match nobody
+        float c[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) c[m] = 0.f;
+        if (v >= 0) contributions(v, (float)rr, c);
+        // Every shuffle runs on all lanes (a shuffle behind a short-circuit
+        // would leave its source lane out).  stop: the run goes no further
+        // right; c becomes the sum over the rest of the run (a lane l + d >= 64
+        // is never read: lane 63 stops).
+        const int right = __shfl_down(key, 1);
+        const int left = __shfl_up(key, 1);
+        int stop = (lane == 63) | (right != key);
+        for (int d = 1; d < 64; d <<= 1) {
+            if (__all(stop)) break;
+            float o[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) o[m] = __shfl_down(c[m], d);
+            const int ostop = __shfl_down(stop, d);
+            if (!stop) {
+#pragma unroll
+                for (int m = 0; m < M; m++) c[m] += o[m];
+                stop = ostop;
+            }
         }
+        const bool head = (lane == 0) | (left != key);
+        if (head && v >= 0) scatter(v, c);
+    }
+    __device__ __forceinline__ void add2_sync(int v1, double r1, int v2, double r2)
+    {
+        reduced(v1, r1);
+        reduced(v2, r2);
     }
 };
 
@@ -168,30 +218,40 @@ __global__ __launch_bounds__(256) void k_trace_fwd(TraceArgs a)
 template <int M, bool FAST>
 __global__ __launch_bounds__(256) void k_trace_adj(TraceArgs a)
 {
+    // no early return: the lanes of a wave combine their segments with
+    // cross-lane shuffles, so every lane stays in the march (idle lanes emit
+    // nothing).  View-major lanes (adjacent channels of one view) give the
+    // runs of equal voxels that the combination removes.
     const int local = blockIdx.x * blockDim.x + threadIdx.x;
-    if (local >= a.n_rays) return;
-    // view-major lanes (adjacent channels of one view): the atomics of a wave
-    // land on neighbouring voxels; lanes spread over views measured slower
-    const int ray = a.ray_offset + local;
+    bool active = local < a.n_rays;
+    const int ray = active ? a.ray_offset + local : 0;
     const int view = ray / a.W;
     const int ch = ray - view * a.W;
-    if (view >= a.VT) return;
+    active = active && view < a.VT;
     SinkAdj<M> sink;
     sink.grad = a.grad;
     sink.dens = reinterpret_cast<const float4*>(a.dens);
-    const float gscale = (a.g ? a.g[ray] : 1.f) * (float)a.out_scale;
-    const float4* wv = reinterpret_cast<const float4*>(a.wts) + (size_t)ray * 2;
-    const float4 w0 = wv[0], w1 = wv[1];
-    const float wa[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    float wa[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float gscale = 0.f;
+    if (active) {
+        gscale = (a.g ? a.g[ray] : 1.f) * (float)a.out_scale;
+        const float4* wv = reinterpret_cast<const float4*>(a.wts) + (size_t)ray * 2;
+        const float4 w0 = wv[0], w1 = wv[1];
+        wa[0] = w0.x; wa[1] = w0.y; wa[2] = w0.z; wa[3] = w0.w;
+        wa[4] = w1.x; wa[5] = w1.y; wa[6] = w1.z; wa[7] = w1.w;
+    }
     bool any = false;
 #pragma unroll
     for (int m = 0; m < M; m++) {
         sink.w[m] = wa[m] * gscale;
         any = any || (sink.w[m] != 0.f);
     }
-    if (!any) return;
+    active = active && any;
     bool ovf = false;
-    march_ray<FAST>(a, view, ch, sink, ovf);
+    if (FAST)
+        march_ray_sync(a, active, view, ch, sink, ovf);
+    else if (active)
+        march_ray<false>(a, view, ch, sink, ovf);
     if (ovf) atomicOr(a.err, 1);
 }
 
