@@ -117,7 +117,7 @@ def bench_decomposition(B=16, iters=50, plain_iters=1):
     init = decompose.water_init(mu, 6, mu_w)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sm = batch.SystemMatrix(plan, transpose=True)
+    sm = batch.SystemMatrix(plan, transpose=True, blocked=True)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     # plain loops: per phantom, per spectrum forward + adjoint through autograd,

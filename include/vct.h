@@ -233,13 +233,25 @@ vct_status vct_trace_adjoint(vct_plan* plan, const void* d_packed, int dens_type
  * (reprojection_systemMatrixfloat / reprojection_vnum / reprojection_sum,
  * M/reprojection_TOMO.cpp:160-163, :186-188, :195) and replays them in
  * reprojection_TOMO_2 (:291-352); here they are built once on the device by
- * the exact tracer, stored ray-major (CSR), and with transpose != 0 also
- * voxel-major (CSC, rays in increasing order within each voxel). */
+ * the exact tracer and stored ray-major (CSR).  Flags:
+ *   VCT_SYSMAT_TRANSPOSE  also voxel-major (CSC, rays in increasing order
+ *                         within each voxel), needed by vct_batch_adjoint;
+ *   VCT_SYSMAT_BLOCKED    also the blocked two-stage form of each direction
+ *                         built: entries regrouped by 64x64 image tile
+ *                         (forward) and by 50-view block x tile (adjoint),
+ *                         so that each block's reads stay in one L2; used by
+ *                         the batched gathers of groups that pass a
+ *                         workspace.  No counterpart in the reference. */
+enum { VCT_SYSMAT_TRANSPOSE = 1, VCT_SYSMAT_BLOCKED = 2 };
 typedef struct vct_sysmat vct_sysmat;
-vct_status vct_sysmat_build(vct_plan* plan, int transpose, vct_sysmat** out, void* stream);
+vct_status vct_sysmat_build(vct_plan* plan, int flags, vct_sysmat** out, void* stream);
 vct_status vct_sysmat_destroy(vct_sysmat* m);
 int64_t vct_sysmat_segments(const vct_sysmat* m);
 size_t vct_sysmat_bytes(const vct_sysmat* m);
+/* Bytes of the partial-sum workspace a batched gather over this system
+ * matrix needs for n_phantoms phantoms in the blocked form (0 if it has
+ * none); one workspace serves both directions. */
+size_t vct_sysmat_workspace(const vct_sysmat* m, int n_phantoms);
 
 /* One group of phantoms sharing a system matrix.  Batched arrays are
  * phantom-interleaved: densities [N*N][B][8], path integrals and adjoint
@@ -250,13 +262,17 @@ typedef struct vct_batch_group {
     float* d_out;        /* paths: path integrals; adjoint: gradient (overwritten) */
     int32_t n_phantoms;  /* B <= lanes_per_row                           */
     int32_t reserved;
+    float* d_work;       /* NULL, or vct_sysmat_workspace(sysmat, n_phantoms)
+                            bytes: gather in the blocked form (partial sums are
+                            rounded to float once); launches that share a
+                            workspace must not overlap */
 } vct_batch_group;
 
 /* Batched replay of the cached segments, replacing reprojection_TOMO_2
  * (M/reprojection_TOMO.cpp:291-352) for many phantoms and geometries in one
- * persistent launch (up to 16 groups; lanes_per_row in {8,16,32,64}, or 1
- * when every group holds one phantom: each row is then split over 16 lanes;
- * blocks <= 0 picks the persistent grid).
+ * persistent launch (two with blocked groups; up to 16 groups; lanes_per_row
+ * in {8,16,32,64}, or 1 when every group holds one phantom: each row is then
+ * split over 16 lanes; blocks <= 0 picks the persistent grid).
  * out[ray][b][m] = sum_seg in[vox][b][m]*len / xfactor (the tracer's units). */
 vct_status vct_batch_paths(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row,
                            int blocks, void* stream);

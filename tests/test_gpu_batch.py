@@ -216,3 +216,126 @@ def test_gather_applies_xfactor(gpu):
     pk, dt = ops.pack_densities(plan, x[0].contiguous())
     ref = ops.trace_paths(plan, pk, dt, 6).reshape(-1, 8)
     assert rel_max(L[:, 0].cpu().numpy(), ref.cpu().numpy()) < 2e-6
+
+
+@pytest.fixture(scope="module")
+def blocked_setup(gpu, small_setup):
+    ops, batch, _ = _mods()
+    plan, sm = small_setup
+    smb = batch.SystemMatrix(plan, transpose=True, blocked=True)
+    return plan, sm, smb
+
+
+def _random_weights(rays, B, dev, seed):
+    import torch
+    g = torch.Generator(device=dev).manual_seed(seed)
+    w = torch.zeros((rays, B, 8), device=dev)
+    w[..., :6] = torch.randn((rays, B, 6), generator=g, device=dev)
+    return w
+
+
+@pytest.mark.parametrize("B", [1, 3, 16])
+def test_blocked_gathers_equal_direct(gpu, blocked_setup, B):
+    """The blocked two-stage gathers (64x64 image tiles; 50-view blocks x
+    tiles for the adjoint; float partial sums) give the direct gathers' sums
+    in both directions, with identical bits from launch to launch."""
+    import torch
+    ops, batch, _ = _mods()
+    plan, sm, smb = blocked_setup
+    assert smb.segments == sm.segments and smb.bytes > sm.bytes
+    rays = plan.V * plan.W
+    x = batch.interleave(_phantoms(B, 128, 0.35, gpu, seed=3300))
+    Ld, Lb, Lb2 = (torch.empty((rays, B, 8), device=gpu) for _ in range(3))
+    batch.batch_paths([(sm, x, Ld, B)], 6)
+    batch.batch_paths([(smb, x, Lb, B)], 6)
+    batch.batch_paths([(smb, x, Lb2, B)], 6)
+    assert rel_max(Lb.cpu().numpy(), Ld.cpu().numpy()) < 1e-6
+    assert torch.equal(Lb, Lb2)
+    w = _random_weights(rays, B, gpu, 11)
+    od, ob, ob2 = (torch.empty((128 * 128, B, 8), device=gpu) for _ in range(3))
+    batch.batch_adjoint([(sm, w, od, B)], 6)
+    batch.batch_adjoint([(smb, w, ob, B)], 6)
+    batch.batch_adjoint([(smb, w, ob2, B)], 6)
+    assert rel_l2(ob.cpu().numpy(), od.cpu().numpy()) < 1e-6
+    assert torch.equal(ob, ob2)
+    assert torch.all(ob[..., 6:] == 0)
+
+
+def test_blocked_gathers_mixed_groups(gpu, blocked_setup):
+    """One launch holding two blocked groups on the same matrix (separate
+    workspaces), a blocked group of another size and a direct group."""
+    import torch
+    ops, batch, _ = _mods()
+    plan, sm, smb = blocked_setup
+    plan2 = ops.plan_for(_geom(96, 0.45, 160))
+    sm2 = batch.SystemMatrix(plan2, transpose=True, blocked=True)
+    sm2d = batch.SystemMatrix(plan2, transpose=True)
+    rays = plan.V * plan.W
+    xa = batch.interleave(_phantoms(2, 128, 0.35, gpu, seed=3400))
+    xb = batch.interleave(_phantoms(2, 128, 0.35, gpu, seed=3500))
+    xc = batch.interleave(_phantoms(3, 96, 0.45, gpu, seed=3600))
+    outs = [torch.empty((rays, 2, 8), device=gpu), torch.empty((rays, 2, 8), device=gpu),
+            torch.empty((plan2.V * plan2.W, 3, 8), device=gpu), torch.empty((rays, 2, 8), device=gpu)]
+    batch.batch_paths([(smb, xa, outs[0], 2), (smb, xb, outs[1], 2), (sm2, xc, outs[2], 3), (sm, xb, outs[3], 2)], 6)
+    for (m, x, B), o in zip([(sm, xa, 2), (sm, xb, 2), (sm2d, xc, 3), (sm, xb, 2)], outs):
+        ref = torch.empty_like(o)
+        batch.batch_paths([(m, x, ref, B)], 6)
+        assert rel_max(o.cpu().numpy(), ref.cpu().numpy()) < 1e-6
+    wa, wc = _random_weights(rays, 2, gpu, 12), _random_weights(plan2.V * plan2.W, 3, gpu, 13)
+    ga, gc = torch.empty((128 * 128, 2, 8), device=gpu), torch.empty((96 * 96, 3, 8), device=gpu)
+    batch.batch_adjoint([(smb, wa, ga, 2), (sm2, wc, gc, 3)], 6)
+    ra, rc = torch.empty_like(ga), torch.empty_like(gc)
+    batch.batch_adjoint([(sm, wa, ra, 2)], 6)
+    batch.batch_adjoint([(sm2d, wc, rc, 3)], 6)
+    assert rel_l2(ga.cpu().numpy(), ra.cpu().numpy()) < 1e-6
+    assert rel_l2(gc.cpu().numpy(), rc.cpu().numpy()) < 1e-6
+
+
+def test_blocked_gathers_full_geometry(gpu):
+    """512^2, 800 views x 609 channels (8x8 tiles, 16 view blocks)."""
+    import torch
+    ops, batch, _ = _mods()
+    plan = ops.plan_for(_geom())
+    sm = plan.system_matrix()
+    smb = batch.SystemMatrix(plan, transpose=True, blocked=True)
+    rays, B = plan.V * plan.W, 2
+    x = batch.interleave(_phantoms(B, 512, 0.1, gpu, seed=3700))
+    Ld, Lb = torch.empty((rays, B, 8), device=gpu), torch.empty((rays, B, 8), device=gpu)
+    batch.batch_paths([(sm, x, Ld, B)], 6)
+    batch.batch_paths([(smb, x, Lb, B)], 6)
+    assert rel_max(Lb.cpu().numpy(), Ld.cpu().numpy()) < 1e-6
+    w = _random_weights(rays, B, gpu, 14)
+    od, ob = torch.empty((512 * 512, B, 8), device=gpu), torch.empty((512 * 512, B, 8), device=gpu)
+    batch.batch_adjoint([(sm, w, od, B)], 6)
+    batch.batch_adjoint([(smb, w, ob, B)], 6)
+    assert rel_l2(ob.cpu().numpy(), od.cpu().numpy()) < 1e-6
+    smb.close()
+
+
+def test_blocked_solver_graph_replay(gpu, blocked_setup):
+    """The solver on the blocked matrix follows the direct one, and its graph
+    replays (workspace inside the graph) equal its eager steps."""
+    import torch
+    ops, batch, _ = _mods()
+    plan, sm, smb = blocked_setup
+    att, pdf = _spectra(gpu)
+    truth = _phantoms(2, 128, 0.35, gpu, seed=3800)
+    tgt = torch.stack([ops.mbfpa_forward(plan, truth[b].contiguous(), att, pdf, sigma=5.37e-5, stream_id=5 * b)
+                       for b in range(2)])
+    init = truth * 0.9
+    direct = batch.BatchedSolver(sm, tgt, att, pdf, init)
+    eager = batch.BatchedSolver(smb, tgt, att, pdf, init)
+    graphed = batch.BatchedSolver(smb, tgt, att, pdf, init)
+    direct.step()
+    eager.step()
+    graphed.capture()
+    for it in range(4):
+        ld = direct.step().clone()
+        le = eager.step().clone()
+        lg = graphed.step().clone()
+        torch.cuda.synchronize()
+        _ = (truth * 2).sum().item()  # unrelated eager work between replays
+        # the loss is summed with atomics: equal to rounding, not in bits
+        assert torch.allclose(lg, le, rtol=1e-6, atol=0), it
+        assert rel_l2(graphed.x.cpu().numpy(), eager.x.cpu().numpy()) < 1e-6, it
+        assert rel_max(le.cpu().numpy(), ld.cpu().numpy()) < 1e-4, it

@@ -19,6 +19,8 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of mem
 DENS_U16, DENS_F32 = 0, 1
 FILTER_RAMLAK, FILTER_SHEPP_LOGAN = 1, 2
 OPT_EXACT_TRACE = 1
+SYSMAT_TRANSPOSE = 1
+SYSMAT_BLOCKED = 2
 
 
 class VctError(RuntimeError):
@@ -90,7 +92,7 @@ class BatchGroup(C.Structure):
     """Mirror of ``vct_batch_group``."""
 
     _fields_ = [("sysmat", C.c_void_p), ("d_in", C.c_void_p), ("d_out", C.c_void_p),
-                ("n_phantoms", C.c_int32), ("reserved", C.c_int32)]
+                ("n_phantoms", C.c_int32), ("reserved", C.c_int32), ("d_work", C.c_void_p)]
 
 
 _SIGS.update({
@@ -98,6 +100,7 @@ _SIGS.update({
     "vct_sysmat_destroy": (C.c_int, [_P]),
     "vct_sysmat_segments": (C.c_int64, [_P]),
     "vct_sysmat_bytes": (C.c_size_t, [_P]),
+    "vct_sysmat_workspace": (C.c_size_t, [_P, C.c_int]),
     "vct_batch_paths": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "vct_batch_adjoint": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "vct_batch_spectral": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P]),

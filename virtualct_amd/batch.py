@@ -1,7 +1,9 @@
 """Batched execution: many phantoms (and geometries) x spectra per launch.
 
 * ``SystemMatrix`` -- the cached segment lists of one geometry
-  (``vct_sysmat_build``), ray-major and optionally voxel-major.
+  (``vct_sysmat_build``), ray-major and optionally voxel-major and blocked
+  (regrouped by image tile / view block for L2-resident two-stage gathers,
+  with a partial-sum workspace per phantom count).
 * ``interleave`` / ``deinterleave`` -- [B,M,N,N] <-> [N*N,B,8] (the
   phantom-interleaved layout of the batched kernels).
 * ``BatchedSolver`` -- projected-gradient dual-energy decomposition of B
@@ -35,18 +37,35 @@ def _p(t: torch.Tensor | None) -> C.c_void_p:
 
 
 class SystemMatrix:
-    def __init__(self, plan: ops.Plan, transpose: bool = True):
+    def __init__(self, plan: ops.Plan, transpose: bool = True, blocked: bool = False):
         self.plan = plan
         self.handle = C.c_void_p()
         dev = torch.device("cuda", plan.device)
+        flags = (_lib.SYSMAT_TRANSPOSE if transpose else 0) | (_lib.SYSMAT_BLOCKED if blocked else 0)
         with torch.cuda.device(plan.device):
-            call("vct_sysmat_build", plan.handle, int(transpose), C.byref(self.handle), _stream(dev))
+            call("vct_sysmat_build", plan.handle, flags, C.byref(self.handle), _stream(dev))
         L = _lib.lib()
         self.segments = int(L.vct_sysmat_segments(self.handle))
         self.bytes = int(L.vct_sysmat_bytes(self.handle))
         self.transpose = transpose
+        self.blocked = blocked
+        self._work: dict[tuple[int, int], torch.Tensor] = {}
+
+    def workspace(self, B: int, slot: int = 0) -> torch.Tensor | None:
+        """Partial-sum workspace of the blocked gathers for B phantoms (one
+        per slot: gathers that may overlap use different slots); None when
+        the matrix has no blocked form."""
+        if not self.blocked:
+            return None
+        key = (B, slot)
+        if key not in self._work:
+            n = int(_lib.lib().vct_sysmat_workspace(self.handle, B))
+            self._work[key] = torch.empty(max(n // 4, 1), dtype=torch.float32,
+                                          device=torch.device("cuda", self.plan.device))
+        return self._work[key]
 
     def close(self):
+        self._work.clear()
         if self.handle:
             _lib.lib().vct_sysmat_destroy(self.handle)
             self.handle = C.c_void_p()
@@ -87,12 +106,18 @@ def lanes_for(B: int) -> int:
 
 def _groups(items):
     arr = (BatchGroup * len(items))()
+    seen: dict[tuple[int, int], int] = {}
     for k, (sm, d_in, d_out, B) in enumerate(items):
         arr[k].sysmat = sm.handle.value
         arr[k].d_in = d_in.data_ptr()
         arr[k].d_out = d_out.data_ptr()
         arr[k].n_phantoms = B
         arr[k].reserved = 0
+        # groups of one launch that share a matrix get separate workspaces
+        slot = seen.get((id(sm), B), 0)
+        seen[(id(sm), B)] = slot + 1
+        work = sm.workspace(B, slot)
+        arr[k].d_work = work.data_ptr() if work is not None else None
     return arr
 
 
@@ -159,7 +184,8 @@ class BatchedSolver:
         self.prev_g = torch.zeros_like(self.x)
         self.alpha = torch.full((self.B,), alpha0, device=dev)
         self.have_prev = False
-        key = ("gather", self.N, self.B, self.M)
+        self.sm.workspace(self.B)  # allocated here, never under graph capture
+        key = ("gather", self.N, self.B, self.M, "blocked" if self.sm.blocked else "direct")
         cfg = tuner().best(key, lambda: self._candidates(), self._time_config) if lanes is None else \
             {"lanes": lanes, "blocks": blocks}
         self.lanes, self.blocks = cfg["lanes"], cfg["blocks"]
@@ -171,7 +197,7 @@ class BatchedSolver:
         cus = torch.cuda.get_device_properties(self.x.device).multi_processor_count
         out = []
         for lanes in sorted({base, 8 if base == 1 else min(64, base * 2)}):
-            for per_cu in (2, 4, 8, 16):
+            for per_cu in (2, 3, 4, 8):
                 out.append({"lanes": lanes, "blocks": cus * per_cu})
         return out
 
@@ -260,8 +286,9 @@ class BatchForwardFBP:
     """MBFPA (S spectra, noise) + FBP for phantoms of several sizes."""
 
     def __init__(self, groups, att: torch.Tensor, pdf: torch.Tensor, sigma: float = 0.0, seed: int = 0x5EED,
-                 filt: int = 2, lanes: int | None = None, blocks: int = 0):
-        """groups: [(Geometry, densities [B,M,N,N] int16 or float32 tensor)]."""
+                 filt: int = 2, lanes: int | None = None, blocks: int = 0, blocked: bool = True):
+        """groups: [(Geometry, densities [B,M,N,N] int16 or float32 tensor)];
+        blocked: gather through the blocked (tile-regrouped) system matrix."""
         self.att = att.contiguous()
         self.pdf = pdf.contiguous()
         self.S = att.shape[0]
@@ -270,9 +297,10 @@ class BatchForwardFBP:
         self.groups: list[Group] = []
         for geom, dens in groups:
             plan = ops.plan_for(geom)
-            sm = SystemMatrix(plan, transpose=False)
+            sm = SystemMatrix(plan, transpose=False, blocked=blocked)
             d = dens.float() / 1000.0 if dens.dtype == torch.int16 else dens.float()
             x = interleave(d)
+            sm.workspace(x.shape[1])
             L = torch.empty((plan.V * plan.W, x.shape[1], 8), device=x.device)
             self.groups.append(Group(plan, sm, x, L, x.shape[1]))
         V = {g.plan.V for g in self.groups}

@@ -13,6 +13,20 @@
 // read one contiguous B*32-byte block per entry.  One persistent kernel
 // (k_gather) serves both directions over up to kMaxGroups groups of
 // phantoms (one group per geometry), pulling work items round-robin.
+//
+// Blocked two-stage form (VCT_SYSMAT_BLOCKED).  A ray crosses the whole
+// image, so the rays in flight on one XCD sweep all N*N*B*32 bytes of the
+// densities past its 4 MB L2 once per view; the voxels in flight in the
+// adjoint start at different times and so read the rays of every view.
+// The blocked form regroups the entries of each direction by a block key,
+//   forward: the kBlockTile x kBlockTile image tile of the voxel,
+//   adjoint: (view / kAdjViews, tile of the voxel),
+// keeping row order inside a block (stable radix sort).  Stage 1 gathers
+// one row per (block, row) run into a partial-sum workspace, block after
+// block, so what a block reads (one tile of densities, or the rays of
+// kAdjViews views that cross one tile) stays in the L2 of the XCD that owns
+// it; stage 2 sums each row's partials in block order.  Both stages are
+// ordinary k_gather launches and the result is deterministic.
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -20,6 +34,15 @@
 
 #include "vct_internal.h"
 #include "vct_march.h"
+
+// Two-stage form of one direction (see above).
+struct vct_blocked {
+    uint32_t rows1 = 0;           // stage-1 rows: (block, row) runs, block-major
+    uint32_t* ptr1 = nullptr;     // [rows1+1]
+    uint2* ent1 = nullptr;        // [nseg] the direction's entries, regrouped
+    uint32_t* ptr2 = nullptr;     // [rows+1]
+    uint2* ent2 = nullptr;        // [rows1] (stage-1 row, 1.0f), block order per row
+};
 
 struct vct_sysmat {
     vct_plan* plan = nullptr;
@@ -31,6 +54,7 @@ struct vct_sysmat {
     uint2* csr = nullptr;         // [nseg] (voxel, float bits of length)
     uint32_t* col_ptr = nullptr;  // [NN+1]
     uint2* csc = nullptr;         // [nseg] (ray, float bits of length)
+    vct_blocked fwd, adj;         // blocked forms (rows1 == 0: not built)
 };
 
 namespace vct {
@@ -55,10 +79,10 @@ struct FillSink {
     {
         const uint32_t e = base + k++;
         out[e] = make_uint2((uint32_t)v, __float_as_uint((float)rr));
+        if (eray) eray[e] = ray;
         if (keys) {
             keys[e] = (uint32_t)v;
             vals[e] = e;
-            eray[e] = ray;
             atomicAdd(colcnt + v, 1u);
         }
     }
@@ -100,6 +124,95 @@ __global__ __launch_bounds__(256) void k_csc_gather(const uint32_t* __restrict__
     csc[i] = make_uint2(eray[e], csr[e].y);
 }
 
+// ------------------------------------------------------ blocked form build
+
+constexpr int kBlockTile = 64;  // image tile edge (pixels) of both blocked directions
+constexpr int kAdjViews = 50;   // views per adjoint block
+
+struct TileGrid {
+    int N, ntx, ntiles;
+    __host__ __device__ __forceinline__ uint32_t tile(uint32_t v) const
+    {
+        const uint32_t iy = v / (uint32_t)N, ix = v - iy * (uint32_t)N;
+        return (iy / kBlockTile) * (uint32_t)ntx + ix / kBlockTile;
+    }
+};
+
+static TileGrid tile_grid(int N)
+{
+    TileGrid t;
+    t.N = N;
+    t.ntx = (N + kBlockTile - 1) / kBlockTile;
+    t.ntiles = t.ntx * t.ntx;
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+
+// forward block key: the tile of the entry's voxel
+__global__ __launch_bounds__(256) void k_keys_fwd(const uint2* __restrict__ ent, int64_t n, TileGrid tg,
+                                                  uint32_t* __restrict__ key)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) key[i] = tg.tile(ent[i].x);
+}
+
+// adjoint block key: (view block of the entry's ray, tile of the row's voxel)
+__global__ __launch_bounds__(256) void k_keys_adj(const uint2* __restrict__ ent, const uint32_t* __restrict__ vox,
+                                                  int64_t n, TileGrid tg, int W, uint32_t* __restrict__ key)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) key[i] = (ent[i].x / (uint32_t)W / kAdjViews) * (uint32_t)tg.ntiles + tg.tile(vox[i]);
+}
+
+// head[i] = 1 where a new (block, row) run starts in sorted order; head[n] = 0
+__global__ __launch_bounds__(256) void k_run_heads(const uint32_t* __restrict__ key_s, const uint32_t* __restrict__ order,
+                                                   const uint32_t* __restrict__ row_of, int64_t n,
+                                                   uint32_t* __restrict__ head)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { head[n] = 0u; return; }
+    head[i] = (i == 0 || key_s[i] != key_s[i - 1] || row_of[order[i]] != row_of[order[i - 1]]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_stage1_fill(const uint32_t* __restrict__ order, const uint32_t* __restrict__ head,
+                                                     const uint32_t* __restrict__ run, const uint2* __restrict__ ent,
+                                                     const uint32_t* __restrict__ row_of, int64_t n,
+                                                     uint2* __restrict__ ent1, uint32_t* __restrict__ ptr1,
+                                                     uint32_t* __restrict__ row1)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t e = order[i];
+    ent1[i] = ent[e];
+    if (head[i]) {
+        const uint32_t p = run[i];
+        ptr1[p] = (uint32_t)i;
+        row1[p] = row_of[e];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_count_rows(const uint32_t* __restrict__ row1, int64_t n,
+                                                    uint32_t* __restrict__ cnt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(cnt + row1[i], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_stage2_fill(const uint32_t* __restrict__ order2, int64_t n,
+                                                     uint2* __restrict__ ent2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ent2[i] = make_uint2(order2[i], 0x3f800000u);  // weight 1.0f
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
 // ------------------------------------------------------------ gather SpMM
 
 struct GatherGroup {
@@ -118,6 +231,7 @@ struct GatherArgs {
     int n_groups;
     int n_items;
     int rows_per_item;
+    int chunked;          // 1: each XCD takes one contiguous eighth of the items
 };
 
 // Workgroups are dispatched round-robin over the XCDs, each with its own L2.
@@ -132,6 +246,25 @@ __device__ __forceinline__ int xcd_block()
     if (gridDim.x % kXcds != 0) return blockIdx.x;
     const int per = gridDim.x / kXcds;
     return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+// Items [first, end) with stride step for this workgroup.  Default: rounds of
+// gridDim.x items, each XCD taking a consecutive run of every round.
+// chunked: XCD x takes items [x*n/8, (x+1)*n/8) for the whole launch, so a
+// block of the blocked form (one tile's stage-1 rows) is read into one L2.
+struct ItemRange {
+    int first, end, step;
+};
+
+__device__ __forceinline__ ItemRange item_range(const GatherArgs& a)
+{
+    if (a.chunked && gridDim.x % kXcds == 0) {
+        const int x = blockIdx.x % kXcds, per = gridDim.x / kXcds;
+        const int lo = (int)((int64_t)a.n_items * x / kXcds);
+        const int hi = (int)((int64_t)a.n_items * (x + 1) / kXcds);
+        return {lo + (int)blockIdx.x / kXcds, hi, per};
+    }
+    return {xcd_block(), a.n_items, (int)gridDim.x};
 }
 
 __device__ __forceinline__ int group_of(const GatherArgs& a, int item)
@@ -169,25 +302,128 @@ __device__ __forceinline__ void gather_step(const uint2* __restrict__ ent, const
     }
 }
 
+// Lane K of a row's LPR lanes -> every lane of the row (all of them active):
+// a DPP row broadcast when a row is one 16-lane DPP row, a lane read when it
+// is the whole wave, a cross-lane permute otherwise.
+template <int LPR, int K>
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v)
+{
+    if constexpr (LPR == 16)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x150 + K, 0xF, 0xF, false);  // row_newbcast:K
+    else if constexpr (LPR == 64)
+        return (uint32_t)__builtin_amdgcn_readlane((int)v, K);
+    else
+        return (uint32_t)__shfl((int)v, (int)(__lane_id() & ~(LPR - 1)) + K, 64);
+}
+
+template <int LPR, int K>
+__device__ __forceinline__ uint2 entry_bcast(uint2 mine)
+{
+    return make_uint2(row_bcast<LPR, K>(mine.x), row_bcast<LPR, K>(mine.y));
+}
+
+// x[0..M) = in[idx][b][0..M): one 16-byte load, plus 8 or 16 bytes for M > 4
+template <int M>
+__device__ __forceinline__ void load_slot(const float* __restrict__ in, size_t slot, float (&x)[8])
+{
+    const float4 lo = *reinterpret_cast<const float4*>(in + slot * 8);
+    x[0] = lo.x; x[1] = lo.y; x[2] = lo.z; x[3] = lo.w;
+    x[4] = x[5] = x[6] = x[7] = 0.f;
+    if constexpr (M > 6) {
+        const float4 hi = *reinterpret_cast<const float4*>(in + slot * 8 + 4);
+        x[4] = hi.x; x[5] = hi.y; x[6] = hi.z; x[7] = hi.w;
+    } else if constexpr (M > 4) {
+        const float2 hi = *reinterpret_cast<const float2*>(in + slot * 8 + 4);
+        x[4] = hi.x; x[5] = hi.y;
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void fma_slot(const float (&x)[8], uint32_t vbits, double (&acc)[M])
+{
+    const double v = __uint_as_float(vbits);
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m] = __builtin_fma((double)x[m], v, acc[m]);
+}
+
+template <int M, int LPR, int K>
+__device__ __forceinline__ void gather_one(uint2 mine, int n, const float* __restrict__ in, int B, int b, bool act,
+                                           double (&acc)[M])
+{
+    const uint2 e = entry_bcast<LPR, K>(mine);  // every lane of the row takes part
+    if (act && K < n) {
+        float x[8];
+        load_slot<M>(in, (size_t)e.x * B + b, x);
+        fma_slot<M>(x, e.y, acc);
+    }
+}
+
+// Entries U0..U0+3 of a chunk of n (the entry of chunk position k sits in
+// lane k of the row, `mine`): broadcast by every lane of the row, then the
+// lanes of real phantoms (act) issue the four loads before the FMAs, in
+// entry order.  A group cut by the end of the row takes its entries one by
+// one.
+template <int M, int LPR, int U0>
+__device__ __forceinline__ void gather_group(uint2 mine, int n, const float* __restrict__ in, int B, int b, bool act,
+                                             double (&acc)[M])
+{
+    if (U0 + 4 <= n) {
+        const uint2 e0 = entry_bcast<LPR, U0>(mine), e1 = entry_bcast<LPR, U0 + 1>(mine);
+        const uint2 e2 = entry_bcast<LPR, U0 + 2>(mine), e3 = entry_bcast<LPR, U0 + 3>(mine);
+        if (!act) return;
+        float x0[8], x1[8], x2[8], x3[8];
+        load_slot<M>(in, (size_t)e0.x * B + b, x0);
+        load_slot<M>(in, (size_t)e1.x * B + b, x1);
+        load_slot<M>(in, (size_t)e2.x * B + b, x2);
+        load_slot<M>(in, (size_t)e3.x * B + b, x3);
+        fma_slot<M>(x0, e0.y, acc);
+        fma_slot<M>(x1, e1.y, acc);
+        fma_slot<M>(x2, e2.y, acc);
+        fma_slot<M>(x3, e3.y, acc);
+    } else {
+        // per-entry tail; the broadcasts stay outside any lane-dependent branch
+        gather_one<M, LPR, U0>(mine, n, in, B, b, act, acc);
+        gather_one<M, LPR, U0 + 1>(mine, n, in, B, b, act, acc);
+        gather_one<M, LPR, U0 + 2>(mine, n, in, B, b, act, acc);
+    }
+}
+
 // out[row][b][m] = scale * sum_{(idx,val) in row} in[idx][b][m] * val
-// LPR lanes per row (one per phantom, B <= LPR), 256/LPR rows per item.
+// LPR lanes per row (one per phantom, B <= LPR), 256/LPR rows per item.  A
+// row's entries are read in chunks of CH: lane k loads entry k of the chunk
+// (one coalesced load, issued one chunk ahead) and the row broadcasts it, so
+// the vector memory path carries each entry once instead of once per lane.
 template <int M, int LPR>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a)
 {
+    constexpr int CH = LPR < 16 ? LPR : 16;
     const int lane_row = threadIdx.x / LPR;
     const int b = threadIdx.x % LPR;
-    for (int item = xcd_block(); item < a.n_items; item += gridDim.x) {
+    const ItemRange r = item_range(a);
+    for (int item = r.first; item < r.end; item += r.step) {
         const GatherGroup& G = a.g[group_of(a, item)];
         const int row = (item - G.item_begin) * a.rows_per_item + lane_row;
-        if (row >= G.rows || b >= G.B) continue;
+        // all LPR lanes of a row (one wave) take this branch together
+        if (row >= G.rows) continue;
+        const bool act = b < G.B;
         const uint32_t j0 = G.ptr[row], j1 = G.ptr[row + 1];
         double acc[M];
 #pragma unroll
         for (int m = 0; m < M; m++) acc[m] = 0.0;
-        const float4* in4 = reinterpret_cast<const float4*>(G.in);
-        uint32_t j = j0;
-        for (; j + 3 < j1; j += 4) gather_step<M, 4>(G.ent, in4, G.B, b, j, 1, acc);
-        for (; j < j1; j++) gather_step<M, 1>(G.ent, in4, G.B, b, j, 1, acc);
+        uint2 mine = j0 + b < j1 && b < CH ? G.ent[j0 + b] : make_uint2(0u, 0u);
+        for (uint32_t jb = j0; jb < j1; jb += CH) {
+            const int n = (int)min(j1 - jb, (uint32_t)CH);
+            const uint32_t jn = jb + CH;
+            const uint2 next = jn + b < j1 && b < CH ? G.ent[jn + b] : make_uint2(0u, 0u);
+            gather_group<M, LPR, 0>(mine, n, G.in, G.B, b, act, acc);
+            if (n > 4) gather_group<M, LPR, 4>(mine, n, G.in, G.B, b, act, acc);
+            if constexpr (CH > 8) {
+                if (n > 8) gather_group<M, LPR, 8>(mine, n, G.in, G.B, b, act, acc);
+                if (n > 12) gather_group<M, LPR, 12>(mine, n, G.in, G.B, b, act, acc);
+            }
+            mine = next;
+        }
+        if (!act) continue;
         float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int m = 0; m < M; m++) o[m] = (float)(acc[m] * G.scale);
@@ -207,7 +443,8 @@ __global__ __launch_bounds__(256) void k_gather_split(GatherArgs a)
     static_assert(SPL >= 2 && SPL <= 64 && (SPL & (SPL - 1)) == 0, "SPL is a power of two <= 64");
     const int lane_row = threadIdx.x / SPL;
     const int k = threadIdx.x % SPL;
-    for (int item = xcd_block(); item < a.n_items; item += gridDim.x) {
+    const ItemRange r = item_range(a);
+    for (int item = r.first; item < r.end; item += r.step) {
         const GatherGroup& G = a.g[group_of(a, item)];
         const int row = (item - G.item_begin) * a.rows_per_item + lane_row;
         // every lane of a row takes the same branch, and the SPL lanes of a
@@ -392,12 +629,123 @@ using namespace vct;
 
 // ---------------------------------------------------------------- host API
 
+static unsigned key_bits(uint64_t n_keys)
+{
+    unsigned b = 1;
+    while ((1ull << b) < n_keys) b++;
+    return b;
+}
+
+static void free_blocked(vct_blocked& f)
+{
+    void* bufs[] = {f.ptr1, f.ent1, f.ptr2, f.ent2};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    f = vct_blocked{};
+}
+
+// Regroup the n entries ent (row of entry i: row_of[i], entries in row order)
+// by key[i] < n_keys into the two-stage form f over `rows` rows.  key is
+// overwritten.  Stage-1 rows are the (key, row) runs in key-major order;
+// stage 2 lists each row's runs in key order.
+static vct_status build_blocked(const uint2* ent, const uint32_t* row_of, uint32_t* key, int64_t n, uint32_t n_keys,
+                                int rows, vct_blocked& f, hipStream_t s)
+{
+    vct_status st = VCT_OK;
+    auto hip = [&](hipError_t e, const char* what) {
+        if (st == VCT_OK && e != hipSuccess) st = set_error(VCT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    };
+    const size_t nz = (size_t)n;
+    const dim3 g1((unsigned)((n + 255) / 256)), g1p((unsigned)((n + 256) / 256)), blk(256);
+    uint32_t *vals = nullptr, *key_s = nullptr, *order = nullptr, *head = nullptr, *run = nullptr, *row1 = nullptr;
+    uint32_t *cnt = nullptr, *iota2 = nullptr, *row1_s = nullptr, *order2 = nullptr;
+    void* tmp = nullptr;
+    size_t tb = 0;
+    f = vct_blocked{};
+    // 1. stable sort of the entries by block key
+    hip(hipMalloc(&vals, 4 * nz + 16), "hipMalloc vals");
+    hip(hipMalloc(&key_s, 4 * nz + 16), "hipMalloc key_s");
+    hip(hipMalloc(&order, 4 * nz + 16), "hipMalloc order");
+    if (st == VCT_OK) hipLaunchKernelGGL(k_iota, g1, blk, 0, s, vals, n);
+    const unsigned kb = key_bits(n_keys);
+    hip(rocprim::radix_sort_pairs(nullptr, tb, key, key_s, vals, order, nz, 0u, kb, s), "sort size");
+    hip(hipMalloc(&tmp, tb + 16), "hipMalloc sort");
+    hip(rocprim::radix_sort_pairs(tmp, tb, key, key_s, vals, order, nz, 0u, kb, s), "sort");
+    if (tmp) { (void)hipStreamSynchronize(s); (void)hipFree(tmp); tmp = nullptr; }
+    if (vals) { (void)hipFree(vals); vals = nullptr; }
+    // 2. run heads and their exclusive scan = stage-1 row of each run
+    hip(hipMalloc(&head, 4 * nz + 16), "hipMalloc head");
+    hip(hipMalloc(&run, 4 * nz + 16), "hipMalloc run");
+    if (st == VCT_OK) hipLaunchKernelGGL(k_run_heads, g1p, blk, 0, s, key_s, order, row_of, n, head);
+    tb = 0;
+    hip(rocprim::exclusive_scan(nullptr, tb, head, run, 0u, nz + 1, rocprim::plus<uint32_t>(), s), "scan size");
+    hip(hipMalloc(&tmp, tb + 16), "hipMalloc scan");
+    hip(rocprim::exclusive_scan(tmp, tb, head, run, 0u, nz + 1, rocprim::plus<uint32_t>(), s), "scan");
+    uint32_t rows1 = 0;
+    if (st == VCT_OK) hip(hipMemcpyAsync(&rows1, run + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "copy");
+    if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
+    if (tmp) { (void)hipFree(tmp); tmp = nullptr; }
+    // 3. stage 1: regrouped entries, run starts, row of each run
+    hip(hipMalloc(&f.ent1, 8 * nz + 16), "hipMalloc ent1");
+    hip(hipMalloc(&f.ptr1, 4 * ((size_t)rows1 + 1)), "hipMalloc ptr1");
+    hip(hipMalloc(&row1, 4 * ((size_t)rows1 + 1)), "hipMalloc row1");
+    if (st == VCT_OK) {
+        hipLaunchKernelGGL(k_stage1_fill, g1, blk, 0, s, order, head, run, ent, row_of, n, f.ent1, f.ptr1, row1);
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, f.ptr1 + rows1, (uint32_t)n);
+        hip(hipGetLastError(), "k_stage1_fill");
+    }
+    void* done1[] = {key_s, order, head, run};
+    for (void* b : done1)
+        if (b) (void)hipFree(b);
+    // 4. stage 2: the runs of each row in key order (stable sort by row)
+    const dim3 g2((rows1 + 255) / 256);
+    hip(hipMalloc(&cnt, 4 * ((size_t)rows + 1)), "hipMalloc cnt");
+    hip(hipMalloc(&iota2, 4 * ((size_t)rows1 + 1)), "hipMalloc iota2");
+    hip(hipMalloc(&row1_s, 4 * ((size_t)rows1 + 1)), "hipMalloc row1_s");
+    hip(hipMalloc(&order2, 4 * ((size_t)rows1 + 1)), "hipMalloc order2");
+    hip(hipMalloc(&f.ptr2, 4 * ((size_t)rows + 1)), "hipMalloc ptr2");
+    hip(hipMalloc(&f.ent2, 8 * ((size_t)rows1 + 1)), "hipMalloc ent2");
+    if (st == VCT_OK) {
+        hipLaunchKernelGGL(k_zero_f32, dim3((rows + 256) / 256), blk, 0, s, (float*)cnt, rows + 1);
+        if (rows1) {
+            hipLaunchKernelGGL(k_count_rows, g2, blk, 0, s, row1, (int64_t)rows1, cnt);
+            hipLaunchKernelGGL(k_iota, g2, blk, 0, s, iota2, (int64_t)rows1);
+        }
+        hip(hipGetLastError(), "k_count_rows");
+    }
+    tb = 0;
+    const unsigned rb = key_bits((uint64_t)rows);
+    hip(rocprim::radix_sort_pairs(nullptr, tb, row1, row1_s, iota2, order2, (size_t)rows1, 0u, rb, s), "sort size");
+    hip(hipMalloc(&tmp, tb + 16), "hipMalloc sort");
+    hip(rocprim::radix_sort_pairs(tmp, tb, row1, row1_s, iota2, order2, (size_t)rows1, 0u, rb, s), "sort");
+    if (tmp) { (void)hipStreamSynchronize(s); (void)hipFree(tmp); tmp = nullptr; }
+    tb = 0;
+    hip(rocprim::exclusive_scan(nullptr, tb, cnt, f.ptr2, 0u, (size_t)rows + 1, rocprim::plus<uint32_t>(), s),
+        "scan size");
+    hip(hipMalloc(&tmp, tb + 16), "hipMalloc scan");
+    hip(rocprim::exclusive_scan(tmp, tb, cnt, f.ptr2, 0u, (size_t)rows + 1, rocprim::plus<uint32_t>(), s), "scan");
+    if (st == VCT_OK && rows1) {
+        hipLaunchKernelGGL(k_stage2_fill, g2, blk, 0, s, order2, (int64_t)rows1, f.ent2);
+        hip(hipGetLastError(), "k_stage2_fill");
+    }
+    if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
+    void* done2[] = {tmp, row1, cnt, iota2, row1_s, order2};
+    for (void* b : done2)
+        if (b) (void)hipFree(b);
+    f.rows1 = rows1;
+    if (st != VCT_OK) free_blocked(f);
+    return st;
+}
+
 extern "C" {
 
-vct_status vct_sysmat_build(vct_plan* p, int transpose, vct_sysmat** out, void* stream)
+vct_status vct_sysmat_build(vct_plan* p, int flags, vct_sysmat** out, void* stream)
 {
     clear_error();
     if (!p || !out) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (flags & ~(VCT_SYSMAT_TRANSPOSE | VCT_SYSMAT_BLOCKED)) return set_error(VCT_ERR_ARG, "unknown flags %#x", flags);
+    const bool transpose = (flags & VCT_SYSMAT_TRANSPOSE) != 0;
+    const bool blocked = (flags & VCT_SYSMAT_BLOCKED) != 0;
     *out = nullptr;
     hipStream_t s = (hipStream_t)stream;
     const vct_geometry& g = p->g;
@@ -435,10 +783,11 @@ vct_status vct_sysmat_build(vct_plan* p, int transpose, vct_sysmat** out, void* 
     if (tmp) { (void)hipFree(tmp); tmp = nullptr; }
     m->nseg = total;
     if (st == VCT_OK) hip(hipMalloc(&m->csr, sizeof(uint2) * (size_t)total + 8), "hipMalloc csr");
+    if (st == VCT_OK && (transpose || blocked))
+        hip(hipMalloc(&eray, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc eray");
     if (st == VCT_OK && transpose) {
         hip(hipMalloc(&keys, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc keys");
         hip(hipMalloc(&vals, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc vals");
-        hip(hipMalloc(&eray, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc eray");
         hip(hipMalloc(&colcnt, sizeof(uint32_t) * (NN + 1)), "hipMalloc colcnt");
         if (st == VCT_OK)
             hipLaunchKernelGGL(k_zero_f32, dim3((NN + 256) / 256), dim3(256), 0, s, (float*)colcnt, NN + 1);
@@ -471,6 +820,27 @@ vct_status vct_sysmat_build(vct_plan* p, int transpose, vct_sysmat** out, void* 
             hip(hipGetLastError(), "k_csc_gather");
         }
     }
+    if (st == VCT_OK && blocked && total > 0) {
+        // keys (the per-entry voxel of the ray-major order) is free again and
+        // holds the block keys; keys2 holds the voxel of each CSC entry
+        const TileGrid tg = tile_grid(g.recon_size);
+        const dim3 ge((unsigned)(((int64_t)total + 255) / 256));
+        uint32_t* bkey = keys;
+        if (!bkey) hip(hipMalloc(&bkey, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc block keys");
+        if (st == VCT_OK) {
+            hipLaunchKernelGGL(k_keys_fwd, ge, dim3(256), 0, s, m->csr, (int64_t)total, tg, bkey);
+            hip(hipGetLastError(), "k_keys_fwd");
+        }
+        if (st == VCT_OK) st = build_blocked(m->csr, eray, bkey, total, (uint32_t)tg.ntiles, nrays, m->fwd, s);
+        if (st == VCT_OK && transpose) {
+            const int nvb = (g.n_views + kAdjViews - 1) / kAdjViews;
+            hipLaunchKernelGGL(k_keys_adj, ge, dim3(256), 0, s, m->csc, keys2, (int64_t)total, tg, g.n_channels, bkey);
+            hip(hipGetLastError(), "k_keys_adj");
+            if (st == VCT_OK)
+                st = build_blocked(m->csc, keys2, bkey, total, (uint32_t)(nvb * tg.ntiles), NN, m->adj, s);
+        }
+        if (!keys && bkey) (void)hipFree(bkey);
+    }
     int herr = 0;
     if (st == VCT_OK) hip(hipMemcpyAsync(&herr, p->ctx->d_err, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
     if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
@@ -492,6 +862,8 @@ vct_status vct_sysmat_destroy(vct_sysmat* m)
     void* bufs[] = {m->row_ptr, m->csr, m->col_ptr, m->csc};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    free_blocked(m->fwd);
+    free_blocked(m->adj);
     delete m;
     return VCT_OK;
 }
@@ -503,46 +875,47 @@ size_t vct_sysmat_bytes(const vct_sysmat* m)
     if (!m) return 0;
     size_t b = sizeof(uint32_t) * (m->nrays + 1) + sizeof(uint2) * (size_t)m->nseg;
     if (m->csc) b += sizeof(uint32_t) * (m->NN + 1) + sizeof(uint2) * (size_t)m->nseg;
+    const vct_blocked* fs[] = {&m->fwd, &m->adj};
+    const int rows[] = {m->nrays, m->NN};
+    for (int k = 0; k < 2; k++)
+        if (fs[k]->rows1)
+            b += sizeof(uint2) * (size_t)m->nseg + sizeof(uint32_t) * ((size_t)fs[k]->rows1 + 1) +
+                 sizeof(uint32_t) * ((size_t)rows[k] + 1) + sizeof(uint2) * ((size_t)fs[k]->rows1 + 1);
     return b;
 }
 
-static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int M, int lanes, int blocks,
-                                bool adjoint, void* stream)
+size_t vct_sysmat_workspace(const vct_sysmat* m, int n_phantoms)
 {
-    if (!groups || n_groups < 1 || n_groups > kMaxGroups)
-        return set_error(VCT_ERR_ARG, "n_groups %d outside 1..%d", n_groups, kMaxGroups);
-    if (M < 1 || M > 8) return set_error(VCT_ERR_ARG, "M=%d outside 1..8", M);
-    if (lanes != 1 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
-        return set_error(VCT_ERR_ARG, "lanes_per_row %d not 1/8/16/32/64", lanes);
-    // lanes_per_row 1: one phantom per group, each row split over kSplit lanes
-    constexpr int kSplit = 16;
-    GatherArgs a{};
-    a.n_groups = n_groups;
-    a.rows_per_item = 256 / (lanes == 1 ? kSplit : lanes);
-    int items = 0;
-    for (int k = 0; k < n_groups; k++) {
-        const vct_batch_group& G = groups[k];
-        if (!G.sysmat || !G.d_in || !G.d_out) return set_error(VCT_ERR_ARG, "group %d has NULL pointers", k);
-        if (G.n_phantoms < 1 || G.n_phantoms > lanes)
-            return set_error(VCT_ERR_ARG, "group %d: %d phantoms with %d lanes per row", k, G.n_phantoms, lanes);
-        const vct_sysmat* m = G.sysmat;
-        if (adjoint && !m->csc) return set_error(VCT_ERR_ARG, "group %d: system matrix built without transpose", k);
-        GatherGroup& d = a.g[k];
-        d.ptr = adjoint ? m->col_ptr : m->row_ptr;
-        d.ent = adjoint ? m->csc : m->csr;
-        d.rows = adjoint ? m->NN : m->nrays;
-        d.in = G.d_in;
-        d.out = G.d_out;
-        d.B = G.n_phantoms;
-        d.scale = m->scale;
-        d.item_begin = items;
-        items += (d.rows + a.rows_per_item - 1) / a.rows_per_item;
-    }
+    if (!m || n_phantoms < 1) return 0;
+    const size_t rows1 = m->fwd.rows1 > m->adj.rows1 ? m->fwd.rows1 : m->adj.rows1;
+    return rows1 * (size_t)n_phantoms * 8 * sizeof(float);
+}
+
+// lanes_per_row 1: one phantom per group, each row split over kSplit lanes
+constexpr int kSplit = 16;
+
+static void add_group(GatherArgs& a, int& items, const uint32_t* ptr, const uint2* ent, int rows, const float* in,
+                      float* out, int B, float scale)
+{
+    GatherGroup& d = a.g[a.n_groups++];
+    d.ptr = ptr;
+    d.ent = ent;
+    d.rows = rows;
+    d.in = in;
+    d.out = out;
+    d.B = B;
+    d.scale = scale;
+    d.item_begin = items;
+    items += (rows + a.rows_per_item - 1) / a.rows_per_item;
+}
+
+static vct_status run_gather(GatherArgs& a, int items, int M, int lanes, int blocks, hipStream_t s)
+{
     a.n_items = items;
-    if (blocks <= 0) blocks = 2048;
+    if (items == 0) return VCT_OK;
+    if (blocks <= 0) blocks = 1024;  // 4 workgroups per CU of MI355X
     blocks = blocks < items ? blocks : items;
     if (blocks >= kXcds) blocks -= blocks % kXcds;
-    hipStream_t s = (hipStream_t)stream;
 #define VCT_GATHER_CASE(MM, LL) \
     if (M == MM && lanes == LL) hipLaunchKernelGGL((k_gather<MM, LL>), dim3(blocks), dim3(256), 0, s, a);
 #define VCT_GATHER_M(MM)                                                                                   \
@@ -554,6 +927,43 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
 #undef VCT_GATHER_CASE
     VCT_HIP(hipGetLastError());
     return VCT_OK;
+}
+
+// Groups with a workspace and a blocked form of the direction run in two
+// launches (stage 1 into the workspace, then stage 2); the others gather
+// directly in the first launch.
+static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int M, int lanes, int blocks,
+                                bool adjoint, void* stream)
+{
+    if (!groups || n_groups < 1 || n_groups > kMaxGroups)
+        return set_error(VCT_ERR_ARG, "n_groups %d outside 1..%d", n_groups, kMaxGroups);
+    if (M < 1 || M > 8) return set_error(VCT_ERR_ARG, "M=%d outside 1..8", M);
+    if (lanes != 1 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+        return set_error(VCT_ERR_ARG, "lanes_per_row %d not 1/8/16/32/64", lanes);
+    GatherArgs a{}, b{};
+    a.rows_per_item = b.rows_per_item = 256 / (lanes == 1 ? kSplit : lanes);
+    int items_a = 0, items_b = 0;
+    for (int k = 0; k < n_groups; k++) {
+        const vct_batch_group& G = groups[k];
+        if (!G.sysmat || !G.d_in || !G.d_out) return set_error(VCT_ERR_ARG, "group %d has NULL pointers", k);
+        if (G.n_phantoms < 1 || G.n_phantoms > lanes)
+            return set_error(VCT_ERR_ARG, "group %d: %d phantoms with %d lanes per row", k, G.n_phantoms, lanes);
+        const vct_sysmat* m = G.sysmat;
+        if (adjoint && !m->csc) return set_error(VCT_ERR_ARG, "group %d: system matrix built without transpose", k);
+        const vct_blocked& f = adjoint ? m->adj : m->fwd;
+        const int rows = adjoint ? m->NN : m->nrays;
+        if (G.d_work && f.rows1) {
+            add_group(a, items_a, f.ptr1, f.ent1, (int)f.rows1, G.d_in, G.d_work, G.n_phantoms, 1.f);
+            add_group(b, items_b, f.ptr2, f.ent2, rows, G.d_work, G.d_out, G.n_phantoms, m->scale);
+            a.chunked = 1;
+        } else {
+            add_group(a, items_a, adjoint ? m->col_ptr : m->row_ptr, adjoint ? m->csc : m->csr, rows, G.d_in, G.d_out,
+                      G.n_phantoms, m->scale);
+        }
+    }
+    hipStream_t s = (hipStream_t)stream;
+    VCT_CHECK(run_gather(a, items_a, M, lanes, blocks, s));
+    return run_gather(b, items_b, M, lanes, blocks, s);
 }
 
 vct_status vct_batch_paths(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row, int blocks,
