@@ -318,6 +318,55 @@ vct_status vct_reconstruct_image_host(vct_ctx* ctx, const vct_geometry* g, const
 vct_status vct_lengthproj_host(vct_ctx* ctx, const vct_geometry* g, const float* proj, double z_cm,
                                double couch_speed_cm, double iso_det_cm, float* npf);
 
+/* ------------------------------------------- measured (real-scanner) data */
+
+/* Shape of measured detector data, as loadData derives it
+ * (M/projectionDataTOMO.cpp:126-134, :331-333): detector rows of 609
+ * channels of which the first 27 are skipped; air-scan files of 1 + 5 blocks
+ * of 800 x 609 floats (block 0 skipped). */
+typedef struct vct_measured {
+    int32_t in_mode;    /* 0: "PR" pre-treatment, 1: "IN" in-treatment        */
+    int32_t rotations;  /* totalProjNumber                                   */
+    int32_t height;     /* views per rotation after sino_fact (projImgHeight) */
+    int32_t width;      /* channels kept (projImgWidth), width + 27 <= 609   */
+    int32_t sino_fact;  /* PR keeps rows k with k % sino_fact == 0           */
+    int32_t skip_rows;  /* IN: rows after each view that are not used (4 when
+                           the sampling rate is >= 200 Hz, else 0)          */
+} vct_measured;
+
+/* Air-scan normalisation, replacing loadProjectionData
+ * (M/projectionDataTOMO.cpp:380-497) on device data:
+ *   d_air [5][800][609] (the five averaged blocks);
+ *   d_raw PR [rotations][800][609], IN [rotations][height][609];
+ *   d_volume float and d_mask uint8 [rotations*height][width];
+ *   d_max_bits: float bits of the largest IN value (set by the call).
+ * PR: kept rows divided by the air average, packed rotation after rotation.
+ * IN: only view 0 of each rotation holds data (the reference indexes its
+ * one-row read buffer at the view's row, which reads zeros otherwise),
+ * divided by twice the air average of row 0; values below 0.1 become 0 with
+ * mask 0.  Rejects shapes for which the reference reads or writes past its
+ * buffers (width + 27 > 609; PR with ceil(800/sino_fact) > height; IN with
+ * height > 800).  Allocates nothing. */
+vct_status vct_measured_normalise(const vct_measured* m, const float* d_air, const float* d_raw, float* d_volume,
+                                  uint8_t* d_mask, uint32_t* d_max_bits, void* stream);
+/* The helical window of M/main.cpp:352-371 (view 0, 360 degrees): rows
+ * n < height/2*2 of d_volume [rotations*height][width] blended in FP64
+ * with the same row one rotation later, weight 0.5*|n - height/2|/(height/2),
+ * for the first half when that rotation exists -> d_window [height/2*2][width]
+ * (the sinogram vct_lengthproj takes). */
+vct_status vct_helical_window(const float* d_volume, int rotations, int height, int width, float* d_window,
+                              void* stream);
+/* loadProjectionData on files (the detector file and the air-scan file the
+ * txt names): host volume and mask [rotations*height][width] and the largest
+ * IN value.  Where the reference prints "Maximum projection ... > 1.0...
+ * STOP" and exits, this returns VCT_ERR_ARG; a short file is VCT_ERR_IO (the
+ * reference leaves stale buffer contents). */
+vct_status vct_load_measured_host(vct_ctx* ctx, const vct_measured* m, const char* detector_path,
+                                  const char* airscan_path, float* volume, uint8_t* mask, double* max_proj);
+/* vct_helical_window on host buffers. */
+vct_status vct_helical_window_host(vct_ctx* ctx, const float* volume, int rotations, int height, int width,
+                                   float* window);
+
 #ifdef __cplusplus
 }
 #endif

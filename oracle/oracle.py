@@ -101,6 +101,10 @@ class Oracle:
         for fn, out in (("orc_lengthproj_ref", _f32p), ("orc_lengthproj_f64", _f64p)):
             getattr(L, fn).argtypes = [G, _f32p, C.c_double, C.c_double, C.c_double, out]
             getattr(L, fn).restype = C.c_int
+        L.orc_load_measured.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f32p, _f32p, _f32p, _u8p,
+                                        C.POINTER(C.c_double)]
+        L.orc_load_measured.restype = C.c_int
+        L.orc_helical_window.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, _f32p]
 
     # -- tables
     def view_tables(self, g: Geom):
@@ -244,6 +248,24 @@ class Oracle:
         return out if rc == 0 else None
 
 
+    # -- measured data
+    def load_measured(self, in_mode: int, rot: int, H: int, W: int, nskip: int, air, raw):
+        """(volume [rot*H, W] float32, mask uint8, max) or None for a shape
+        the reference reads out of bounds.  air [6,800,609]; raw PR
+        [rot,800,609], IN [rot,H,609]."""
+        out = np.zeros((rot * H, W), np.float32)
+        mask = np.zeros((rot * H, W), np.uint8)
+        mx = C.c_double(0.0)
+        rc = self.lib.orc_load_measured(in_mode, rot, H, W, nskip, np.ascontiguousarray(air, np.float32),
+                                        np.ascontiguousarray(raw, np.float32), out, mask, C.byref(mx))
+        return (out, mask, mx.value) if rc == 0 else None
+
+    def helical_window(self, volume, rot: int, H: int, W: int):
+        win = np.zeros((H // 2 * 2, W), np.float32)
+        self.lib.orc_helical_window(np.ascontiguousarray(volume, np.float32), rot, H, W, win)
+        return win
+
+
 class RefMBFPA:
     """The reference's own MBFPA functions (oracle/_ref/libref_mbfpa.so)."""
 
@@ -262,6 +284,22 @@ class RefMBFPA:
         L.ref_lengthproj.argtypes = [C.c_int, _f32p, _f32p, C.c_double, C.c_int, C.c_int, _f64p, _f64p,
                                      C.c_double, C.c_double]
         L.ref_lengthproj.restype = C.c_int
+        L.ref_load_measured.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_char_p,
+                                        C.c_char_p, _f32p, _u8p, C.POINTER(C.c_int)]
+        L.ref_load_measured.restype = C.c_int
+
+    def load_measured(self, inpr: str, W: int, H_txt: int, rot: int, sino_fact: int, rate_orig: float,
+                      proj_file: str, air_file: str):
+        """The reference's loadProjectionData on real files: (result, volume
+        [rot*H', W], mask, H').  It writes projections.raw into the working
+        directory and stops the process when an IN value exceeds 1."""
+        H = H_txt // sino_fact
+        vol = np.zeros((rot * H, W), np.float32)
+        mask = np.zeros((rot * H, W), np.uint8)
+        rows = C.c_int(0)
+        rc = self.lib.ref_load_measured(inpr.encode(), W, H_txt, rot, sino_fact, rate_orig, proj_file.encode(),
+                                        air_file.encode(), vol, mask, C.byref(rows))
+        return rc, vol, mask, rows.value
 
     def lengthproj(self, g: Geom, proj, cosT, sinT, z_cm: float, couch_cm: float):
         """The reference's lengthproj_TOMO: npf [N,N] (zeros outside the mask)."""

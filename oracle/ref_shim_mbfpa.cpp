@@ -139,3 +139,42 @@ int ref_lengthproj(int N, float* npf, float* proj, double pix, int W, int H, dou
 }
 
 }  // extern "C"
+
+#include "projectionDataTOMO.h"
+
+extern "C" {
+
+// loadProjectionData (M/projectionDataTOMO.cpp:323-524) on a PROJECTION_DATA
+// holding the fields loadData (:68-134) would have set: the class label, the
+// sinogram width and derived height, the rotation count, sino_fact, the
+// original sampling rate and the two file names (float data).  Copies the
+// W x H' x rotations volume and mask out; returns the call's result and the
+// height after sino_fact in *rows.  The reference writes projections.raw into
+// the working directory.
+int ref_load_measured(const char* inpr, int width, int height, int rotations, int sino_fact, double rate_orig,
+                      const char* proj_file, const char* air_file, float* volume, unsigned char* mask, int* rows)
+{
+    PROJECTION_DATA* pd = newProjectionData();
+    strncpy(pd->InPrClassString, inpr, sizeof(pd->InPrClassString) - 1);
+    strncpy(pd->projImgDataType, "float", sizeof(pd->projImgDataType) - 1);
+    strncpy(pd->projImgFileName, proj_file, sizeof(pd->projImgFileName) - 1);
+    strncpy(pd->AirScanImgFileName, air_file, sizeof(pd->AirScanImgFileName) - 1);
+    pd->projImgWidth = width;
+    pd->projImgHeight = height;
+    pd->totalProjNumber = rotations;
+    pd->sino_fact = sino_fact;
+    pd->DataSamplingRateOrig = rate_orig;
+    pd->DataSamplingRate = rate_orig >= 200 ? rate_orig / 5 : rate_orig;
+    pd->StartViewAngle = 0.0;
+    pd->datasize = sizeof(float);
+    pd->datatype = UNIT_FLOAT32;
+    const int r = loadProjectionData(pd);
+    *rows = pd->projImgHeight;
+    const size_t n = (size_t)pd->projImgWidth * pd->projImgHeight * pd->usedProjNumber;
+    memcpy(volume, pd->projVolume, n * sizeof(float));
+    memcpy(mask, pd->mask_projVolume, n);
+    deleteProjectionData(pd);
+    return r;
+}
+
+}  // extern "C"

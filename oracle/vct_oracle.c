@@ -865,3 +865,90 @@ int orc_lengthproj_f64(const orc_geom* g, const float* proj, double z_cm, double
     LP_BODY(double, acc += t)
 }
 #undef LP_BODY
+
+/* ------------------------------------------------------ measured-data loader */
+
+enum { ORC_NC = 609, ORC_NS = 27, ORC_AH = 800, ORC_AN = 5 };
+
+/* M/projectionDataTOMO.cpp:380-422 (air scans), :434-496 (sinogram). */
+int orc_load_measured(int in_mode, int rot, int H, int W, int nskip, const float* air, const float* raw,
+                      float* out, uint8_t* mask, double* max_proj)
+{
+    const size_t blk = (size_t)ORC_NC * ORC_AH;
+    if (rot < 1 || H < 1 || W < 1 || nskip < 1 || W + ORC_NS > ORC_NC) return -1;
+    memset(out, 0, sizeof(float) * (size_t)rot * H * W);
+    memset(mask, 0, (size_t)rot * H * W);
+    *max_proj = 0.0;
+    if (in_mode == 0) {
+        /* PR: rows k1 with k1 % nskip == 0 are kept, packed rotation after
+         * rotation (:434-458) */
+        if ((ORC_AH + nskip - 1) / nskip > H) return -1;
+        float* avg = (float*)calloc(blk, sizeof(float));
+        for (int b = 1; b <= ORC_AN; b++)
+            for (size_t k = 0; k < blk; k++) avg[k] += air[b * blk + k] / ORC_AN;
+        size_t kkk = 0;
+        for (int t = 0; t < rot; t++)
+            for (int k1 = 0; k1 < ORC_AH; k1++)
+                for (int k2 = ORC_NS; k2 < ORC_NS + W; k2++) {
+                    const size_t kk = (size_t)k1 * ORC_NC + k2;
+                    const float d = raw[t * blk + kk] / avg[kk];
+                    if (k1 / nskip * nskip == k1) {
+                        out[kkk] = d;
+                        mask[kkk] = 1;
+                        kkk++;
+                    }
+                }
+        free(avg);
+        return 0;
+    }
+    /* IN: the air average is [H][W] (:404-417); each sinogram row is read into
+     * the first 609 floats of a zeroed 800 x 609 buffer that is then indexed
+     * at row j (:463-470), so only row 0 of each rotation holds data; values
+     * below 0.1 are zeroed and unmasked (:486-491) */
+    if (H > ORC_AH) return -1;
+    float* avg = (float*)calloc((size_t)H * W, sizeof(float));
+    for (int b = 1; b <= ORC_AN; b++) {
+        size_t k = 0;
+        for (int k1 = 0; k1 < H; k1++)
+            for (int k2 = ORC_NS; k2 < ORC_NS + W; k2++) avg[k++] += air[b * blk + (size_t)k1 * ORC_NC + k2] / ORC_AN;
+    }
+    double pmax = 0.0;
+    for (int t = 0; t < rot; t++)
+        for (int j = 0; j < H; j++)
+            for (int c = 0; c < W; c++) {
+                const int i = ORC_NS + c;
+                const float x = j == 0 ? raw[((size_t)t * H) * ORC_NC + i] : 0.0f;
+                float d = x / (avg[c] * 2);
+                if (pmax < d && i >= 0 && i < ORC_NC) pmax = d;
+                uint8_t m = 1;
+                if (d < 0.1) {
+                    d = 0.0f;
+                    m = 0;
+                }
+                const size_t o = ((size_t)t * H + j) * W + c;
+                out[o] = d;
+                mask[o] = m;
+            }
+    free(avg);
+    *max_proj = pmax;
+    return 0;
+}
+
+/* M/main.cpp:352-371 (start view 0, 360-degree window). */
+void orc_helical_window(const float* volume, int rot, int H, int W, float* window)
+{
+    const int used = 2;
+    const int end = H / used * 2, use = H / used * 2;
+    for (int j = 0; j < end; j++) {
+        const int n = j;
+        int iphase = 0;
+        if (n < use / 2) iphase = 1;
+        else if (n > use / 2) iphase = -1;
+        if (j + iphase * H < H) iphase = 0;
+        if (j + iphase * H > (rot - 1) * H) iphase = 0;
+        const double zw = 0.5 * fabs(n - (H / 2.0)) / (H / 2.0);
+        for (int c = 0; c < W; c++)
+            window[(size_t)n * W + c] =
+                (float)((1.0 - zw) * volume[(size_t)j * W + c] + zw * volume[(size_t)(j + iphase * H) * W + c]);
+    }
+}

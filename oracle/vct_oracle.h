@@ -117,6 +117,23 @@ int orc_lengthproj_ref(const orc_geom* g, const float* proj, double z_cm, double
 int orc_lengthproj_f64(const orc_geom* g, const float* proj, double z_cm, double couch_cm,
                        double iso_det, double* npf);
 
+/* Measured-data loader, M/projectionDataTOMO.cpp:323-524, on arrays holding
+ * the file contents (detector of 609 channels, the first 27 skipped; air
+ * scans of 1 + 5 blocks of 800 x 609, block 0 skipped).
+ *   air: float [6][800][609]
+ *   raw: in_mode 0 (PR): float [rot][800][609], one block per rotation;
+ *        in_mode 1 (IN): float [rot][H][609], the rows the reference uses
+ *        (without the 4 rows it skips after each row at >= 200 Hz)
+ *   out: float [rot*H][W] and mask uint8 [rot*H][W] (H = rows after
+ *        sino_fact); max_proj: the largest normalised value (IN).
+ * Returns 0, or -1 for shapes the reference reads out of bounds. */
+int orc_load_measured(int in_mode, int rot, int H, int W, int nskip, const float* air, const float* raw,
+                      float* out, uint8_t* mask, double* max_proj);
+/* Helical window, M/main.cpp:352-371: rows H/2*2 of the volume [rot*H][W]
+ * blended with the same rows one rotation later (first half only, when that
+ * rotation exists), weight 0.5*|n - H/2|/(H/2). */
+void orc_helical_window(const float* volume, int rot, int H, int W, float* window);
+
 #ifdef __cplusplus
 }
 #endif

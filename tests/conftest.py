@@ -105,3 +105,27 @@ def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# The measured-data loader cases and the seeded inputs that
+# tests/golden/make_golden.py gave the reference: (name, class, height from
+# the txt, original rate [Hz], sino_fact, rotations, width, seed).
+MEASURED_CASES = [
+    ("pr_f1", "PR", 800, 80.0, 1, 2, 500, 11),
+    ("pr_f2", "PR", 800, 80.0, 2, 2, 582, 12),
+    ("in_80", "IN", 800, 80.0, 1, 2, 500, 13),
+    ("in_300", "IN", 600, 300.0, 1, 2, 500, 14),
+    ("in_f2", "IN", 800, 80.0, 2, 3, 582, 15),
+]
+
+
+def measured_inputs(inpr, H, rate, rot, seed):
+    """(air file [6][800][609], detector file contents, the rows used)."""
+    rng = np.random.default_rng(seed)
+    air = rng.uniform(0.5, 1.5, (6, 800, 609)).astype(np.float32)
+    if inpr == "PR":
+        det = rng.uniform(0.0, 1.2, (rot, 800, 609)).astype(np.float32)
+        return air, det, det
+    step = 5 if rate >= 200 else 1
+    det = rng.uniform(0.0, 0.9, (rot * H * step, 609)).astype(np.float32)
+    return air, det, det[::step].reshape(rot, H, 609).copy()

@@ -25,6 +25,10 @@ Outputs (all data, no code from the reference):
                                 geometries, whole; full geometry, sample rows; the
                                 full-image SHA-256 go to kat_hashes.json)
     (--npf-only regenerates ref_npf.npz and its hashes alone)
+  measured_<case>_{volume,mask}_sha256 in kat_hashes.json
+                                the reference's measured-data loader on seeded
+                                synthetic detector / air-scan files
+    (--measured-only regenerates these alone)
 """
 from __future__ import annotations
 
@@ -251,9 +255,63 @@ def npf_goldens():
         json.dump(hashes, f, indent=1)
 
 
+# (name, class, height from the txt (period x rate, rate / 5 at >= 200 Hz),
+#  original rate [Hz], sino_fact, rotations, width, seed)
+MEASURED_CASES = [
+    ("pr_f1", "PR", 800, 80.0, 1, 2, 500, 11),
+    ("pr_f2", "PR", 800, 80.0, 2, 2, 582, 12),
+    ("in_80", "IN", 800, 80.0, 1, 2, 500, 13),
+    ("in_300", "IN", 600, 300.0, 1, 2, 500, 14),
+    ("in_f2", "IN", 800, 80.0, 2, 3, 582, 15),
+]
+
+
+def measured_inputs(inpr: str, H: int, rate: float, rot: int, seed: int):
+    """Seeded air-scan file [6][800][609] uniform(0.5, 1.5) and detector file:
+    PR [rot][800][609] uniform(0, 1.2); IN rot*H rows of 609 uniform(0, 0.9),
+    each followed by 4 more rows at >= 200 Hz.  Returns (air, detector file
+    contents, the rows the loader uses)."""
+    rng = np.random.default_rng(seed)
+    air = rng.uniform(0.5, 1.5, (6, 800, 609)).astype(np.float32)
+    if inpr == "PR":
+        det = rng.uniform(0.0, 1.2, (rot, 800, 609)).astype(np.float32)
+        return air, det, det
+    step = 5 if rate >= 200 else 1
+    det = rng.uniform(0.0, 0.9, (rot * H * step, 609)).astype(np.float32)
+    return air, det, det[::step].reshape(rot, H, 609).copy()
+
+
+def measured_goldens():
+    """The reference's loadProjectionData (oracle/_ref/libref_mbfpa.so) on the
+    seeded files of MEASURED_CASES: SHA-256 of the volume and mask."""
+    import tempfile
+    ref = RefMBFPA()
+    with open(os.path.join(HERE, "kat_hashes.json")) as f:
+        hashes = json.load(f)
+    cwd = os.getcwd()
+    for name, inpr, H_txt, rate, nskip, rot, W, seed in MEASURED_CASES:
+        air, det, _ = measured_inputs(inpr, H_txt // nskip, rate, rot, seed)
+        with tempfile.TemporaryDirectory() as d:
+            air.tofile(os.path.join(d, "air.raw"))
+            det.tofile(os.path.join(d, "det.raw"))
+            os.chdir(d)
+            try:
+                rc, vol, mask, rows = ref.load_measured(inpr, W, H_txt, rot, nskip, rate, "det.raw", "air.raw")
+            finally:
+                os.chdir(cwd)
+        assert rc == rot and rows == H_txt // nskip, (name, rc, rows)
+        hashes[f"measured_{name}_volume_sha256"] = hashlib.sha256(vol.tobytes()).hexdigest()
+        hashes[f"measured_{name}_mask_sha256"] = hashlib.sha256(mask.tobytes()).hexdigest()
+    with open(os.path.join(HERE, "kat_hashes.json"), "w") as f:
+        json.dump(hashes, f, indent=1)
+
+
 if __name__ == "__main__":
     if "--npf-only" in sys.argv:
         npf_goldens()
+    elif "--measured-only" in sys.argv:
+        measured_goldens()
     else:
         main()
         npf_goldens()
+        measured_goldens()

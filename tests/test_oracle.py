@@ -17,7 +17,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import rel_l2, rel_max
+from conftest import MEASURED_CASES, measured_inputs, rel_l2, rel_max
 from oracle.oracle import Geom
 
 SMALL = {"n64": (64, 0.7, 80), "n128": (128, 0.35, 160)}
@@ -281,3 +281,61 @@ def test_lengthproj_closed_form(oracle):
     # a uniform sinogram gives the summed chord lengths
     ones = oracle.lengthproj_f64(g, np.ones((V, 609), np.float32), z, couch)
     np.testing.assert_allclose(ones, chord.sum(0) * mask, rtol=1e-12, atol=0)
+
+
+# ------------------------------------------------------------ measured data
+
+@pytest.mark.parametrize("case", MEASURED_CASES, ids=[c[0] for c in MEASURED_CASES])
+def test_measured_loader_matches_reference(oracle, golden, case):
+    """Air-scan normalisation (PR and IN, sino_fact 1 and 2, 80 and 300 Hz):
+    volume and mask bit-identical to the reference's loadProjectionData
+    (SHA-256 from tests/golden/make_golden.py)."""
+    name, inpr, H_txt, rate, nskip, rot, W, seed = case
+    H = H_txt // nskip
+    air, _, raw = measured_inputs(inpr, H, rate, rot, seed)
+    vol, mask, mx = oracle.load_measured(int(inpr == "IN"), rot, H, W, nskip, air, raw)
+    h = golden.hashes()
+    assert sha(vol) == h[f"measured_{name}_volume_sha256"]
+    assert sha(mask) == h[f"measured_{name}_mask_sha256"]
+    if inpr == "IN":
+        # values below 0.1 are zeroed, so the largest value survives; only
+        # view 0 of each rotation holds data
+        assert mx == float(vol.max()) and 0.1 < mx <= 1.0
+        assert not vol.reshape(rot, H, W)[:, 1:].any() and not mask.reshape(rot, H, W)[:, 1:].any()
+
+
+def test_measured_loader_rejects_out_of_bounds_shapes(oracle):
+    air = np.ones((6, 800, 609), np.float32)
+    raw = np.ones((1, 800, 609), np.float32)
+    assert oracle.load_measured(0, 1, 800, 583, 1, air, raw) is None   # 583 + 27 > 609 channels
+    assert oracle.load_measured(0, 1, 800 // 3, 500, 3, air, raw) is None   # 267 kept rows > 266
+    assert oracle.load_measured(1, 1, 801, 500, 1, air, np.ones((1, 801, 609), np.float32)) is None
+    # air scans of 5 average to 1 + 1 + 1 + 1 + 1 = 5 exactly, so a PR value
+    # is the detector value / 5, correctly rounded, at the kept rows
+    air5 = np.full((6, 800, 609), 5.0, np.float32)
+    raw = np.random.default_rng(4).uniform(0, 2, (1, 800, 609)).astype(np.float32)
+    vol, mask, _ = oracle.load_measured(0, 1, 400, 582, 2, air5, raw)
+    np.testing.assert_array_equal(vol, raw[0, ::2, 27:609] / np.float32(5))
+    assert np.all(mask == 1)
+
+
+def test_helical_window_closed_forms(oracle):
+    """One rotation: the window is the first H/2*2 views.  Rotations that are
+    constant c_t: with two, only view 0 blends (weight 1/2, c0/2 + c1/2), all
+    other views keep c0; with three, views n < H/2 are c0 + w (c1 - c0),
+    w = 1/2 - n/H, and views n >= H/2 keep c0."""
+    H, W = 10, 7
+    rng = np.random.default_rng(3)
+    one = rng.uniform(0, 1, (H, W)).astype(np.float32)
+    np.testing.assert_array_equal(oracle.helical_window(one, 1, H, W), one)
+    c = np.array([0.25, 0.75, 0.5], np.float32)
+    for rot in (2, 3):
+        vol = np.repeat(c[:rot], H * W).reshape(rot * H, W)
+        win = oracle.helical_window(vol, rot, H, W).astype(np.float64)
+        for n in range(H):
+            w = 0.5 * abs(n - H / 2) / (H / 2)
+            if rot == 2:
+                expect = (c[0] + c[1]) / 2 if n == 0 else c[0]
+            else:
+                expect = c[0] + w * (c[1] - c[0]) if n < H // 2 else c[0]
+            np.testing.assert_allclose(win[n], expect, rtol=2 ** -23, atol=0)

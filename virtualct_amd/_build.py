@@ -25,7 +25,7 @@ OBJDIR = os.path.join(PKG, "_build")
 ARCH = os.environ.get("VCT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-LIB_SOURCES = ["vct_api.cpp", "vct_trace.hip", "vct_fbp.hip", "vct_batch.hip", "vct_npf.hip"]
+LIB_SOURCES = ["vct_api.cpp", "vct_trace.hip", "vct_fbp.hip", "vct_batch.hip", "vct_npf.hip", "vct_measured.hip"]
 DRIVERS = {"mvct_mbfpa": "mvct_mbfpa.cpp", "mvct_fbp": "mvct_fbp.cpp", "mvct_bhc": "mvct_bhc.cpp"}
 
 COMMON = [

@@ -88,6 +88,22 @@ _SIGS = {
 }
 
 
+class Measured(C.Structure):
+    """Mirror of ``vct_measured``."""
+
+    _fields_ = [("in_mode", C.c_int32), ("rotations", C.c_int32), ("height", C.c_int32), ("width", C.c_int32),
+                ("sino_fact", C.c_int32), ("skip_rows", C.c_int32)]
+
+
+_SIGS.update({
+    "vct_measured_normalise": (C.c_int, [C.POINTER(Measured), _P, _P, _P, _P, _P, _P]),
+    "vct_helical_window": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "vct_load_measured_host": (C.c_int, [_P, C.POINTER(Measured), C.c_char_p, C.c_char_p, _P, _P,
+                                         C.POINTER(C.c_double)]),
+    "vct_helical_window_host": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P]),
+})
+
+
 class BatchGroup(C.Structure):
     """Mirror of ``vct_batch_group``."""
 

@@ -11,13 +11,16 @@
 //   writes reprojection_float.raw (float32 [V][W], max-normalised with
 //          Gaussian detector noise; M/main.cpp:508-530) and npf_float.raw
 //          (M/main.cpp:407-419: zeros when the input sinogram is "*")
+// With the detector and air-scan files the txt names present, npf_float.raw
+// is computed from them as the reference does (air-scan normalisation, PR or
+// IN, M/projectionDataTOMO.cpp:323-524, which also writes projections.raw;
+// helical window, M/main.cpp:352-371; npf for slice z = reconstruction
+// start z); the reference's "Maximum projection ... STOP" ends the run with
+// status 0 as there.
 // Environment overrides (argv is unchanged): VCT_PHANTOM (density file),
 // VCT_NUM_MATERIAL (6 or 8), VCT_SEED (noise seed), VCT_DEVICE, and
-// VCT_NPF_SINOGRAM: a normalised float32 [V/2*2][W] sinogram window from
-// which npf_float.raw is computed for slice z = reconstruction start z with
-// the txt's couch speed (the reference's measured-data loader, which would
-// build that window from raw detector and air-scan files, is not part of
-// this build).
+// VCT_NPF_SINOGRAM: a normalised float32 [V/2*2][W] sinogram window used for
+// npf_float.raw in place of the measured files.
 // The iterative-reconstruction leftovers (ReconImageRaw/, M/main.cpp:558-624)
 // are not produced.  Exit status is 0 on success and on the reference's own
 // soft failures (missing info file, usage), 1 on a library error.
@@ -187,14 +190,22 @@ int main(int argc, char* argv[])
     st = vct_init(device, &ctx);
     if (st != VCT_OK) return fail("vct_init", st);
 
-    // npf side output, M/main.cpp:407-419: zeros without an input sinogram;
-    // from the normalised window rows named by VCT_NPF_SINOGRAM otherwise
+    // npf side output, M/main.cpp:352-419: the measured volume of the txt's
+    // detector and air-scan files (loadProjectionData, which loadData calls,
+    // M/projectionDataTOMO.cpp:273, :323-524), its helical window (:352-371)
+    // and the chord-weighted backprojection; zeros when either file is
+    // missing (the reference's "*"); or the window VCT_NPF_SINOGRAM names
     const size_t NN = (size_t)in.recon_size * in.recon_size;
     const char* env_npf = std::getenv("VCT_NPF_SINOGRAM");
-    std::vector<float> npf(NN, 0.f);
+    const int rows = H / used_recon_angle * 2;
+    std::vector<float> npf(NN, 0.f), win;
+    auto stop = [&](const char* what, vct_status s2) {
+        const int rc = fail(what, s2);  // before vct_terminate clears the message
+        vct_terminate(ctx);
+        return rc;
+    };
     if (env_npf) {
-        const int rows = H / used_recon_angle * 2;
-        std::vector<float> win((size_t)rows * W);
+        win.resize((size_t)rows * W);
         FILE* fp = std::fopen(env_npf, "rb");
         const size_t got = fp ? std::fread(win.data(), sizeof(float), win.size(), fp) : 0;
         if (fp) std::fclose(fp);
@@ -203,15 +214,47 @@ int main(int argc, char* argv[])
             vct_terminate(ctx);
             return 1;
         }
-        st = vct_lengthproj_host(ctx, &g, win.data(), zstart, in.couch_speed, 57.6, npf.data());
-        if (st != VCT_OK) {
-            const int rc = fail("lengthproj", st);  // before vct_terminate clears the message
-            vct_terminate(ctx);
-            return rc;
-        }
     } else if (have_sino) {
-        std::fprintf(stderr, "WARNING: the measured-data loader is not part of this build; npf_float.raw is "
-                             "written as zeros (set VCT_NPF_SINOGRAM to a normalised float32 sinogram)\n");
+        bool have_air = false;
+        if (FILE* fp = std::fopen(in.air_file.c_str(), "rb")) {
+            have_air = true;
+            std::fclose(fp);
+        } else {
+            std::fprintf(stderr, "data file: %s not found\n", in.air_file.c_str());
+        }
+        if (have_air) {
+            const bool in_mode = in.inpr != "PR";
+            vct_measured m{in_mode ? 1 : 0, total_proj, H, W, in.sino_fact > 0 ? in.sino_fact : 1,
+                           in_mode && rate_orig >= 200 ? 4 : 0};
+            std::vector<float> vol((size_t)total_proj * H * W);
+            std::vector<uint8_t> vmask(vol.size());
+            double pmax = 0.0;
+            if (in_mode) std::fprintf(stderr, "%d %d %d %d\n", total_proj, H, W, (int)(609 * sizeof(float)));
+            st = vct_load_measured_host(ctx, &m, in.proj_file.c_str(), in.air_file.c_str(), vol.data(), vmask.data(),
+                                        &pmax);
+            if (st == VCT_ERR_ARG && in_mode && pmax > 1.0) {
+                // the reference's own stop (M/projectionDataTOMO.cpp:479-483): exit status 0
+                std::fprintf(stderr, "Maximum projection: %lf > 1.0... STOP \n", pmax);
+                vct_terminate(ctx);
+                return 0;
+            }
+            if (st != VCT_OK) return stop("load_measured", st);
+            if (!in_mode)
+                for (int t = 0; t < total_proj; t++)
+                    std::fprintf(stderr, "Z scan step: %lf \n", in.couch_start + t * in.couch_speed);
+            std::fprintf(stderr, "Maximum projection: %lf \n", pmax);
+            if (FILE* fp = std::fopen("projections.raw", "wb")) {  // :513-521
+                std::fwrite(vol.data(), sizeof(float), vol.size(), fp);
+                std::fclose(fp);
+            }
+            win.resize((size_t)rows * W);
+            st = vct_helical_window_host(ctx, vol.data(), total_proj, H, W, win.data());
+            if (st != VCT_OK) return stop("helical_window", st);
+        }
+    }
+    if (!win.empty()) {
+        st = vct_lengthproj_host(ctx, &g, win.data(), zstart, in.couch_speed, 57.6, npf.data());
+        if (st != VCT_OK) return stop("lengthproj", st);
     }
     if (FILE* fp = std::fopen("npf_float.raw", "wb")) {
         std::fwrite(npf.data(), sizeof(float), NN, fp);

@@ -11,6 +11,7 @@ import ctypes as C
 import threading
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -237,6 +238,42 @@ def lengthproj(plan: Plan, proj: torch.Tensor, z_cm: float, couch_speed_cm: floa
     call("vct_lengthproj", plan.handle, _ptr(proj), float(z_cm), float(couch_speed_cm), float(iso_det_cm),
          _ptr(npf), _stream(plan.device))
     return npf
+
+
+def measured_shape(in_mode: bool, rotations: int, height: int, width: int, sino_fact: int = 1,
+                   skip_rows: int = 0) -> _lib.Measured:
+    """``vct_measured``: PR (in_mode False) or IN data of ``rotations`` x
+    ``height`` views (after sino_fact) x ``width`` channels."""
+    return _lib.Measured(int(bool(in_mode)), rotations, height, width, sino_fact, skip_rows)
+
+
+def load_measured(m: _lib.Measured, detector_path: str, airscan_path: str, device: int = 0):
+    """Air-scan normalised measured data from the detector and air-scan files
+    (the MBFPA program's loadProjectionData): (volume [rot*H, W] float32,
+    mask [rot*H, W] uint8, largest IN value) as numpy arrays."""
+    n = m.rotations * m.height * m.width
+    vol = (C.c_float * n)()
+    mask = (C.c_uint8 * n)()
+    mx = C.c_double(0.0)
+    with torch.cuda.device(device):
+        call("vct_load_measured_host", context(device), C.byref(m), detector_path.encode(), airscan_path.encode(),
+             C.cast(vol, C.c_void_p), C.cast(mask, C.c_void_p), C.byref(mx))
+    shape = (m.rotations * m.height, m.width)
+    return (np.frombuffer(vol, np.float32).reshape(shape).copy(), np.frombuffer(mask, np.uint8).reshape(shape).copy(),
+            mx.value)
+
+
+def helical_window(volume: torch.Tensor, rotations: int, height: int) -> torch.Tensor:
+    """The npf input window [height/2*2, W] blended from the measured volume
+    [rotations*height, W] (M/main.cpp:352-371)."""
+    _require_gpu(volume)
+    volume = volume.contiguous()
+    if volume.dtype != torch.float32 or volume.dim() != 2 or volume.shape[0] != rotations * height:
+        raise ValueError(f"volume must be float32 [{rotations * height}, W]")
+    W = volume.shape[1]
+    win = torch.empty((height // 2 * 2, W), dtype=torch.float32, device=volume.device)
+    call("vct_helical_window", _ptr(volume), rotations, height, W, _ptr(win), _stream(volume.device.index or 0))
+    return win
 
 
 def fbp(plan: Plan, proj: torch.Tensor, filt: int = FILTER_SHEPP_LOGAN, in_mode: bool = False,
