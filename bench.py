@@ -28,12 +28,12 @@ HBM_PEAK_GBS = 8000.0
 SEGMENTS_512 = 282_824_404  # sum of per-ray segments at 512^2, 0.1 cm, 800x609
 # HBM bytes per tracer launch on this workload, from rocprofv3 --pmc passes
 # over `bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-batched
-# --copy-probe 256`: FETCH_SIZE 27730.3 KiB x 2.00 + WRITE_SIZE 15225.0 KiB
+# --copy-probe 256`: FETCH_SIZE 27644.9 KiB x 2.00 + WRITE_SIZE 15225.0 KiB
 # x 1.00, the factors being what the same counters read for the 256 MiB
 # device copy of the same run (131088.7 KiB fetched, 262144 KiB written).
 # The densities (4 MB) stay in L2/MALL: each XCD fetches them about twice,
 # and the write is the 15.6 MB of path integrals.
-TRACE_TRAFFIC_BYTES = int((27730.3 * 2.0 + 15225.0 * 1.0) * 1024)
+TRACE_TRAFFIC_BYTES = int((27644.9 * 2.0 + 15225.0 * 1.0) * 1024)
 METRIC = "MBFPA ray·energy-bins/s, FBP voxel·views/s; batched grad steps/s vs plain"
 
 
@@ -163,7 +163,8 @@ def bench_decomposition(B=16, iters=50, plain_iters=1):
                       "for all phantoms; plain = per phantom and per spectrum autograd calls: on-the-fly trace "
                       "+ spectral sum, adjoint by the ray-driven transposed march with atomics; plain_sysmat = "
                       "the same with the adjoint gathered over the plan's cached system matrix; batched = cached "
-                      "system matrix, one gather launch per direction for all phantoms, HIP-graph replay",
+                      "blocked system matrix, per direction one two-stage gather (tile / view-block partial sums, "
+                      "then per-row sums) for all phantoms, HIP-graph replay",
     }
 
 
