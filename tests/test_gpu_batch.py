@@ -339,3 +339,31 @@ def test_blocked_solver_graph_replay(gpu, blocked_setup):
         assert torch.allclose(lg, le, rtol=1e-6, atol=0), it
         assert rel_l2(graphed.x.cpu().numpy(), eager.x.cpu().numpy()) < 1e-6, it
         assert rel_max(le.cpu().numpy(), ld.cpu().numpy()) < 1e-4, it
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
+def test_gather_every_lane_width(gpu, blocked_setup, lanes):
+    """Every lanes-per-row variant (entry broadcast by permute for 8 and 32,
+    DPP row broadcast for 16, lane read for 64) gives the same FP64 sums as
+    another lane width, bit for bit, in both directions; the blocked form
+    agrees to float rounding of its partial sums."""
+    import torch
+    ops, batch, _ = _mods()
+    plan, sm, smb = blocked_setup
+    # fewer phantoms than lanes (idle lanes still take part in the
+    # broadcasts), checked against another lane width
+    B, ref_lanes = {8: (5, 16), 16: (11, 32), 32: (19, 64), 64: (19, 32)}[lanes]
+    rays = plan.V * plan.W
+    x = batch.interleave(_phantoms(B, 128, 0.35, gpu, seed=3900))
+    w = _random_weights(rays, B, gpu, 15)
+    ref_L, ref_g = torch.empty((rays, B, 8), device=gpu), torch.empty((128 * 128, B, 8), device=gpu)
+    batch.batch_paths([(sm, x, ref_L, B)], 6, ref_lanes)
+    batch.batch_adjoint([(sm, w, ref_g, B)], 6, ref_lanes)
+    L, g = torch.empty_like(ref_L), torch.empty_like(ref_g)
+    batch.batch_paths([(sm, x, L, B)], 6, lanes)
+    batch.batch_adjoint([(sm, w, g, B)], 6, lanes)
+    assert torch.equal(L, ref_L) and torch.equal(g, ref_g)
+    batch.batch_paths([(smb, x, L, B)], 6, lanes)
+    batch.batch_adjoint([(smb, w, g, B)], 6, lanes)
+    assert rel_max(L.cpu().numpy(), ref_L.cpu().numpy()) < 1e-6
+    assert rel_l2(g.cpu().numpy(), ref_g.cpu().numpy()) < 1e-6
