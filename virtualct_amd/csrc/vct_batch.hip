@@ -126,23 +126,25 @@ __global__ __launch_bounds__(256) void k_csc_gather(const uint32_t* __restrict__
 
 // ------------------------------------------------------ blocked form build
 
-constexpr int kBlockTile = 64;  // image tile edge (pixels) of both blocked directions
+constexpr int kBlockTile = 64;  // image tile edge (pixels) of the forward blocks
+constexpr int kAdjTile = 64;    // image tile edge (pixels) of the adjoint blocks
 constexpr int kAdjViews = 50;   // views per adjoint block
 
 struct TileGrid {
-    int N, ntx, ntiles;
+    int N, T, ntx, ntiles;
     __host__ __device__ __forceinline__ uint32_t tile(uint32_t v) const
     {
         const uint32_t iy = v / (uint32_t)N, ix = v - iy * (uint32_t)N;
-        return (iy / kBlockTile) * (uint32_t)ntx + ix / kBlockTile;
+        return (iy / (uint32_t)T) * (uint32_t)ntx + ix / (uint32_t)T;
     }
 };
 
-static TileGrid tile_grid(int N)
+static TileGrid tile_grid(int N, int T)
 {
     TileGrid t;
     t.N = N;
-    t.ntx = (N + kBlockTile - 1) / kBlockTile;
+    t.T = T;
+    t.ntx = (N + T - 1) / T;
     t.ntiles = t.ntx * t.ntx;
     return t;
 }
@@ -823,7 +825,7 @@ vct_status vct_sysmat_build(vct_plan* p, int flags, vct_sysmat** out, void* stre
     if (st == VCT_OK && blocked && total > 0) {
         // keys (the per-entry voxel of the ray-major order) is free again and
         // holds the block keys; keys2 holds the voxel of each CSC entry
-        const TileGrid tg = tile_grid(g.recon_size);
+        const TileGrid tg = tile_grid(g.recon_size, kBlockTile), ta = tile_grid(g.recon_size, kAdjTile);
         const dim3 ge((unsigned)(((int64_t)total + 255) / 256));
         uint32_t* bkey = keys;
         if (!bkey) hip(hipMalloc(&bkey, sizeof(uint32_t) * (size_t)total + 8), "hipMalloc block keys");
@@ -834,10 +836,10 @@ vct_status vct_sysmat_build(vct_plan* p, int flags, vct_sysmat** out, void* stre
         if (st == VCT_OK) st = build_blocked(m->csr, eray, bkey, total, (uint32_t)tg.ntiles, nrays, m->fwd, s);
         if (st == VCT_OK && transpose) {
             const int nvb = (g.n_views + kAdjViews - 1) / kAdjViews;
-            hipLaunchKernelGGL(k_keys_adj, ge, dim3(256), 0, s, m->csc, keys2, (int64_t)total, tg, g.n_channels, bkey);
+            hipLaunchKernelGGL(k_keys_adj, ge, dim3(256), 0, s, m->csc, keys2, (int64_t)total, ta, g.n_channels, bkey);
             hip(hipGetLastError(), "k_keys_adj");
             if (st == VCT_OK)
-                st = build_blocked(m->csc, keys2, bkey, total, (uint32_t)(nvb * tg.ntiles), NN, m->adj, s);
+                st = build_blocked(m->csc, keys2, bkey, total, (uint32_t)(nvb * ta.ntiles), NN, m->adj, s);
         }
         if (!keys && bkey) (void)hipFree(bkey);
     }
