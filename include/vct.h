@@ -295,6 +295,18 @@ vct_status vct_batch_loss(vct_plan* plan, const float* d_L, int B, int M, const 
                           const float* d_pdf, int NE, int S, const float* d_target, float* d_w,
                           float* d_loss, void* stream);
 
+/* Projected Barzilai-Borwein step of the batched decomposition (no
+ * counterpart in the reference, whose iterative loop, M/IR_ImageTOMO.cpp:19-72,
+ * is disabled) on phantom-interleaved arrays [n_vox][B][8]:
+ *   s = x - x_prev, y = g - g_prev, per phantom sy = sum s.y, ss = sum s.s
+ *   (FP64, fixed order), alpha_b = clamp(sy > 0 ? ss / max(sy, 1e-30)
+ *   : 2 alpha_b, 1e-8, 1e2) (skipped when first != 0),
+ *   x_prev = x, g_prev = g, x = max(x - alpha_b g, 0).
+ * d_work: vct_batch_bb_workspace(B) bytes; B <= 64.  Allocates nothing. */
+size_t vct_batch_bb_workspace(int n_phantoms);
+vct_status vct_batch_bb_step(float* d_x, float* d_x_prev, float* d_g_prev, const float* d_g, float* d_alpha,
+                             int n_vox, int n_phantoms, int first, double* d_work, void* stream);
+
 /* ------------------------------------------------ host-buffer drop-ins */
 
 /* Replaces reprojectionOnGPU_TOMO (M/main.cpp:439-446) = the energy loop

@@ -367,3 +367,40 @@ def test_gather_every_lane_width(gpu, blocked_setup, lanes):
     batch.batch_adjoint([(smb, w, g, B)], 6, lanes)
     assert rel_max(L.cpu().numpy(), ref_L.cpu().numpy()) < 1e-6
     assert rel_l2(g.cpu().numpy(), ref_g.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("B", [1, 3, 16])
+def test_bb_step_matches_fp64_formula(gpu, B):
+    """vct_batch_bb_step: the step length from FP64 sums of s.y and s.s
+    (s = x - x_prev, y = g - g_prev, differences in float), then the
+    projected update with that step; the first step keeps alpha."""
+    import ctypes as C
+    import torch
+    from virtualct_amd import _lib
+    from virtualct_amd._lib import call
+    n = 4099
+    gen = torch.Generator(device=gpu).manual_seed(40 + B)
+    x, xp, g, gp = (torch.zeros((n, B, 8), device=gpu) for _ in range(4))
+    for t in (x, xp, g, gp):
+        t[..., :6] = torch.rand((n, B, 6), generator=gen, device=gpu) - 0.3
+    x.clamp_(min=0)
+    xp.clamp_(min=0)
+    alpha = torch.linspace(1e-3, 2e-3, B, device=gpu)
+    work = torch.empty(int(_lib.lib().vct_batch_bb_workspace(B)) // 8, dtype=torch.float64, device=gpu)
+    P = lambda t: C.c_void_p(t.data_ptr())
+    s_ = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for first in (1, 0):
+        x2, xp2, gp2, al2 = x.clone(), xp.clone(), gp.clone(), alpha.clone()
+        call("vct_batch_bb_step", P(x2), P(xp2), P(gp2), P(g), P(al2), n, B, first, P(work), s_)
+        torch.cuda.synchronize()
+        if first:
+            assert torch.equal(al2, alpha)
+        else:
+            s = (x - xp).double()
+            y = (g - gp).double()
+            sy, ss = (s * y).sum(dim=(0, 2)), (s * s).sum(dim=(0, 2))
+            bb = torch.where(sy > 0, ss / sy.clamp_min(1e-30), alpha.double() * 2)
+            a_ref = bb.clamp(1e-8, 1e2).float()
+            assert torch.allclose(al2, a_ref, rtol=2e-7, atol=0)
+        assert torch.equal(xp2, x) and torch.equal(gp2, g)
+        assert torch.equal(x2, (x - al2.view(1, -1, 1) * g).clamp_min(0.0))

@@ -117,6 +117,8 @@ _SIGS.update({
     "vct_sysmat_segments": (C.c_int64, [_P]),
     "vct_sysmat_bytes": (C.c_size_t, [_P]),
     "vct_sysmat_workspace": (C.c_size_t, [_P, C.c_int]),
+    "vct_batch_bb_workspace": (C.c_size_t, [C.c_int]),
+    "vct_batch_bb_step": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "vct_batch_paths": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "vct_batch_adjoint": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "vct_batch_spectral": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P]),

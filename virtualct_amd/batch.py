@@ -183,6 +183,10 @@ class BatchedSolver:
         self.prev_x = torch.zeros_like(self.x)
         self.prev_g = torch.zeros_like(self.x)
         self.alpha = torch.full((self.B,), alpha0, device=dev)
+        nb = int(_lib.lib().vct_batch_bb_workspace(self.B))
+        if nb == 0:
+            raise ValueError("the solver takes 1..64 phantoms")
+        self.bb_work = torch.empty(nb // 8, dtype=torch.float64, device=dev)
         self.have_prev = False
         self.sm.workspace(self.B)  # allocated here, never under graph capture
         key = ("gather", self.N, self.B, self.M, "blocked" if self.sm.blocked else "direct")
@@ -218,16 +222,12 @@ class BatchedSolver:
         batch_loss(self.plan, self.L, self.M, self.att, self.pdf, self.target, self.w, self.loss)
         batch_adjoint([(self.sm, self.w, self.g, self.B)], self.M, self.lanes, self.blocks)
 
-    def _update(self):
-        s = self.x - self.prev_x
-        y = self.g - self.prev_g
-        sy = (s * y).sum(dim=(0, 2))
-        ss = (s * s).sum(dim=(0, 2))
-        bb = torch.where(sy > 0, ss / sy.clamp_min(1e-30), self.alpha * 2)
-        self.alpha.copy_(bb.clamp(1e-8, 1e2))
-        self.prev_x.copy_(self.x)
-        self.prev_g.copy_(self.g)
-        self.x.copy_((self.x - self.alpha.view(1, -1, 1) * self.g).clamp_min(0.0))
+    def _update(self, first: bool = False):
+        """Projected Barzilai-Borwein step (``vct_batch_bb_step``): the
+        per-phantom step length from s = x - x_prev, y = g - g_prev (kept on
+        the first step), then x_prev = x, g_prev = g, x = max(x - alpha g, 0)."""
+        call("vct_batch_bb_step", _p(self.x), _p(self.prev_x), _p(self.prev_g), _p(self.g), _p(self.alpha),
+             self.x.shape[0], self.B, int(first), _p(self.bb_work), _stream(self.x.device))
 
     def step(self):
         """One iteration; returns the per-phantom loss before the update."""
@@ -235,13 +235,8 @@ class BatchedSolver:
             self.graph.replay()
             return self.loss
         self._forward_loss_grad()
-        if not self.have_prev:
-            self.prev_x.copy_(self.x)
-            self.prev_g.copy_(self.g)
-            self.x.copy_((self.x - self.alpha.view(1, -1, 1) * self.g).clamp_min(0.0))
-            self.have_prev = True
-        else:
-            self._update()
+        self._update(first=not self.have_prev)
+        self.have_prev = True
         return self.loss
 
     def capture(self):

@@ -610,6 +610,100 @@ __global__ __launch_bounds__(256) void k_batch_loss(BatchSpecArgs a, int lpr)
     for (int i = threadIdx.x; i < min(lpr, a.B); i += blockDim.x) atomicAdd(a.loss + i, lsum[i]);
 }
 
+// ------------------------------------------- Barzilai-Borwein descent step
+//
+// Arrays are phantom-interleaved [n][B][8]: slot k = (voxel, phantom k % B)
+// holds 8 floats.  The grid stride (kBBBlocksPerPhantom * B blocks of 256
+// threads) is a multiple of B, so every thread stays on one phantom; sums are
+// FP64 in a fixed order (per thread, then per block over the threads of that
+// phantom, then over the blocks), so the step is deterministic.
+constexpr int kBBBlocksPerPhantom = 64;
+
+struct BBArgs {
+    float* x;          // [n][B][8] densities (updated)
+    float* x_prev;     // [n][B][8] previous densities (overwritten with x)
+    float* g_prev;     // [n][B][8] previous gradient (overwritten with g)
+    const float* g;    // [n][B][8] gradient
+    float* alpha;      // [B] step lengths
+    double* part;      // [blocks][B][2] partial sums (s.y, s.s)
+    int64_t slots;     // n * B
+    int B;
+    int first;         // 1: no previous iterate (plain projected step)
+};
+
+__global__ __launch_bounds__(256) void k_bb_partial(BBArgs a)
+{
+    __shared__ double red[2][256];
+    double sy = 0.0, ss = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.slots; k += stride) {
+        const float4* x4 = reinterpret_cast<const float4*>(a.x) + k * 2;
+        const float4* xp4 = reinterpret_cast<const float4*>(a.x_prev) + k * 2;
+        const float4* g4 = reinterpret_cast<const float4*>(a.g) + k * 2;
+        const float4* gp4 = reinterpret_cast<const float4*>(a.g_prev) + k * 2;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const float4 x = x4[h], xp = xp4[h], g = g4[h], gp = gp4[h];
+            const float s[4] = {x.x - xp.x, x.y - xp.y, x.z - xp.z, x.w - xp.w};
+            const float y[4] = {g.x - gp.x, g.y - gp.y, g.z - gp.z, g.w - gp.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                sy += (double)s[i] * (double)y[i];
+                ss += (double)s[i] * (double)s[i];
+            }
+        }
+    }
+    red[0][threadIdx.x] = sy;
+    red[1][threadIdx.x] = ss;
+    __syncthreads();
+    // thread j sums the threads j, j+B, ... of the block: one phantom each
+    if ((int)threadIdx.x < a.B && (int)threadIdx.x < (int)blockDim.x) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int t = threadIdx.x; t < (int)blockDim.x; t += a.B) {
+            t0 += red[0][t];
+            t1 += red[1][t];
+        }
+        const int b = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % a.B);
+        a.part[((size_t)blockIdx.x * a.B + b) * 2] = t0;
+        a.part[((size_t)blockIdx.x * a.B + b) * 2 + 1] = t1;
+    }
+}
+
+// alpha_b = clamp(sy > 0 ? ss / max(sy, 1e-30) : 2 alpha_b, 1e-8, 1e2)
+__global__ void k_bb_alpha(BBArgs a, int blocks)
+{
+    const int b = threadIdx.x;
+    if (b >= a.B || a.first) return;
+    double sy = 0.0, ss = 0.0;
+    for (int k = 0; k < blocks; k++) {
+        sy += a.part[((size_t)k * a.B + b) * 2];
+        ss += a.part[((size_t)k * a.B + b) * 2 + 1];
+    }
+    const double bb = sy > 0.0 ? ss / fmax(sy, 1e-30) : 2.0 * (double)a.alpha[b];
+    a.alpha[b] = (float)fmin(fmax(bb, 1e-8), 1e2);
+}
+
+// x_prev = x, g_prev = g, x = max(x - alpha_b g, 0)
+__global__ __launch_bounds__(256) void k_bb_apply(BBArgs a)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.slots; k += stride) {
+        const float al = a.alpha[k % a.B];
+        float4* x4 = reinterpret_cast<float4*>(a.x) + k * 2;
+        float4* xp4 = reinterpret_cast<float4*>(a.x_prev) + k * 2;
+        float4* gp4 = reinterpret_cast<float4*>(a.g_prev) + k * 2;
+        const float4* g4 = reinterpret_cast<const float4*>(a.g) + k * 2;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const float4 x = x4[h], g = g4[h];
+            xp4[h] = x;
+            gp4[h] = g;
+            x4[h] = make_float4(fmaxf(x.x - al * g.x, 0.f), fmaxf(x.y - al * g.y, 0.f), fmaxf(x.z - al * g.z, 0.f),
+                                fmaxf(x.w - al * g.w, 0.f));
+        }
+    }
+}
+
 __global__ void k_batch_stats_init(uint32_t* stats, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -980,6 +1074,31 @@ vct_status vct_batch_adjoint(const vct_batch_group* groups, int n_groups, int M,
 {
     clear_error();
     return launch_gather(groups, n_groups, M, lanes_per_row, blocks, true, stream);
+}
+
+size_t vct_batch_bb_workspace(int n_phantoms)
+{
+    if (n_phantoms < 1 || n_phantoms > 64) return 0;
+    return (size_t)kBBBlocksPerPhantom * n_phantoms * n_phantoms * 2 * sizeof(double);
+}
+
+vct_status vct_batch_bb_step(float* d_x, float* d_x_prev, float* d_g_prev, const float* d_g, float* d_alpha,
+                             int n_vox, int n_phantoms, int first, double* d_work, void* stream)
+{
+    clear_error();
+    if (!d_x || !d_x_prev || !d_g_prev || !d_g || !d_alpha || !d_work) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (n_vox < 1 || n_phantoms < 1 || n_phantoms > 64)
+        return set_error(VCT_ERR_ARG, "n_vox %d / n_phantoms %d invalid (1..64 phantoms)", n_vox, n_phantoms);
+    BBArgs a{d_x, d_x_prev, d_g_prev, d_g, d_alpha, d_work, (int64_t)n_vox * n_phantoms, n_phantoms, first ? 1 : 0};
+    const int blocks = kBBBlocksPerPhantom * n_phantoms;
+    hipStream_t s = (hipStream_t)stream;
+    if (!a.first) {
+        hipLaunchKernelGGL(k_bb_partial, dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_bb_alpha, dim3(1), dim3(64), 0, s, a, blocks);
+    }
+    hipLaunchKernelGGL(k_bb_apply, dim3(blocks), dim3(256), 0, s, a);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
 }
 
 static vct_status check_spec(vct_plan* p, const float* L, int B, int M, const float* att, const float* pdf, int NE,
