@@ -583,11 +583,15 @@ __global__ __launch_bounds__(256) void k_batch_loss(BatchSpecArgs a, int lpr)
             const float* mus = mu2 + (size_t)s * NE * 8;
             const float* ps = pdf + (size_t)s * NE;
             for (int e = 0; e < NE; e++) {
-                const float* mu = mus + e * 8;
+                // the 8 table entries of bin e in two 16-byte LDS reads
+                const float4 m0 = reinterpret_cast<const float4*>(mus)[2 * e];
+                const float4 m1 = reinterpret_cast<const float4*>(mus)[2 * e + 1];
+                const float mu[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
                 float x = 0.f;
 #pragma unroll
                 for (int m = 0; m < M; m++) x = fmaf(mu[m], Lm[m], x);
-                const float t = ps[e] * exp2f(x);
+                // hardware exp2: only terms below 2^-126 (flushed) differ from exp2f
+                const float t = ps[e] * __builtin_amdgcn_exp2f(x);
                 I += t;
 #pragma unroll
                 for (int m = 0; m < M; m++) d[m] = fmaf(t, mu[m], d[m]);
