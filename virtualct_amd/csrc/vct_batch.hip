@@ -514,11 +514,13 @@ __global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
         const float4 l0 = Lv[0], l1 = Lv[1];
         const float Lm[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
         for (int e = 0; e < NE; e++) {
-            const float* mu = mu2 + e * 8;
+            const float4 m0 = reinterpret_cast<const float4*>(mu2)[2 * e];
+            const float4 m1 = reinterpret_cast<const float4*>(mu2)[2 * e + 1];
+            const float mu[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
             float x = 0.f;
 #pragma unroll
             for (int m = 0; m < M; m++) x = fmaf(mu[m], Lm[m], x);
-            I += pdf[e] * exp2f(x);
+            I += pdf[e] * __builtin_amdgcn_exp2f(x);  // as k_spectral
         }
     }
     const size_t bs = (size_t)b * a.S + s;
@@ -563,7 +565,7 @@ __global__ __launch_bounds__(256) void k_batch_loss(BatchSpecArgs a, int lpr)
     __syncthreads();
     if (threadIdx.x < S) {
         float t = 0.f;
-        for (int e = 0; e < NE; e++) t += pdf[threadIdx.x * NE + e] * exp2f(0.f);
+        for (int e = 0; e < NE; e++) t += pdf[threadIdx.x * NE + e] * __builtin_amdgcn_exp2f(0.f);
         imax[threadIdx.x] = t;
     }
     __syncthreads();

@@ -286,11 +286,15 @@ __global__ __launch_bounds__(256) void k_spectral(SpectralArgs a)
         const float4 l0 = Lv[0], l1 = Lv[1];
         const float Lm[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
         for (int e = 0; e < NE; e++) {
-            const float* mu = mu2 + e * 8;
+            // the 8 table entries of bin e in two 16-byte LDS reads
+            const float4 m0 = reinterpret_cast<const float4*>(mu2)[2 * e];
+            const float4 m1 = reinterpret_cast<const float4*>(mu2)[2 * e + 1];
+            const float mu[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
             float x = 0.f;
 #pragma unroll
             for (int m = 0; m < M; m++) x = fmaf(mu[m], Lm[m], x);
-            const float t = pdf[e] * exp2f(x);
+            // hardware exp2: only terms below 2^-126 (flushed) differ from exp2f
+            const float t = pdf[e] * __builtin_amdgcn_exp2f(x);
             I += t;
             if (DI) {
 #pragma unroll
