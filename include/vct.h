@@ -254,8 +254,11 @@ size_t vct_sysmat_bytes(const vct_sysmat* m);
 size_t vct_sysmat_workspace(const vct_sysmat* m, int n_phantoms);
 
 /* One group of phantoms sharing a system matrix.  Batched arrays are
- * phantom-interleaved: densities [N*N][B][8], path integrals and adjoint
- * weights [rays][B][8], gradients [N*N][B][8] (float32). */
+ * phantom-interleaved rows of 8*B floats: densities and gradients one row per
+ * voxel (N*N rows), path integrals and adjoint weights one row per ray.  A
+ * row holds three phantom-major planes: materials 0-3 of phantom b at 4b,
+ * materials 4-5 at 4B + 2b, materials 6-7 at 6B + 2b; materials >= M are 0.
+ * Below, x[row][b][m] names that element.  For B = 1 a row is the plain [8]. */
 typedef struct vct_batch_group {
     const vct_sysmat* sysmat;
     const float* d_in;   /* paths: densities; adjoint: weights            */
@@ -297,7 +300,7 @@ vct_status vct_batch_loss(vct_plan* plan, const float* d_L, int B, int M, const 
 
 /* Projected Barzilai-Borwein step of the batched decomposition (no
  * counterpart in the reference, whose iterative loop, M/IR_ImageTOMO.cpp:19-72,
- * is disabled) on phantom-interleaved arrays [n_vox][B][8]:
+ * is disabled) on phantom-interleaved arrays of n_vox rows:
  *   s = x - x_prev, y = g - g_prev, per phantom sy = sum s.y, ss = sum s.s
  *   (FP64, fixed order), alpha_b = clamp(sy > 0 ? ss / max(sy, 1e-30)
  *   : 2 alpha_b, 1e-8, 1e2) (skipped when first != 0),

@@ -58,10 +58,11 @@ def test_csr_replay_equals_tracer(gpu, small_setup):
     x = _phantoms(3, 128, 0.35, gpu)
     L = torch.empty((plan.V * plan.W, 3, 8), device=gpu)
     batch.batch_paths([(sm, batch.interleave(x), L, 3)], 6)
+    Ln = batch.from_planes(L)
     for b in range(3):
         pk, dt = ops.pack_densities(plan, x[b].contiguous())
         ref = ops.trace_paths(plan, pk, dt, 6).reshape(-1, 8)
-        assert rel_max(L[:, b].cpu().numpy(), ref.cpu().numpy()) < 2e-6
+        assert rel_max(Ln[:, b].cpu().numpy(), ref.cpu().numpy()) < 2e-6
 
 
 def test_csc_adjoint_equals_atomic_adjoint(gpu, small_setup):
@@ -69,13 +70,15 @@ def test_csc_adjoint_equals_atomic_adjoint(gpu, small_setup):
     ops, batch, _ = _mods()
     plan, sm = small_setup
     g = torch.Generator(device=gpu).manual_seed(5)
-    w = torch.zeros((plan.V * plan.W, 2, 8), device=gpu)
-    w[..., :6] = torch.randn((plan.V * plan.W, 2, 6), generator=g, device=gpu)
+    wn = torch.zeros((plan.V * plan.W, 2, 8), device=gpu)
+    wn[..., :6] = torch.randn((plan.V * plan.W, 2, 6), generator=g, device=gpu)
+    w = batch.to_planes(wn)
     out = torch.empty((128 * 128, 2, 8), device=gpu)
     batch.batch_adjoint([(sm, w, out, 2)], 6)
+    outn = batch.from_planes(out)
     for b in range(2):
-        ref = ops.trace_adjoint(plan, w[:, b].contiguous().view(plan.V, plan.W, 8), 6).reshape(-1, 8)
-        assert rel_l2(out[:, b].cpu().numpy(), ref.cpu().numpy()) < 1e-5
+        ref = ops.trace_adjoint(plan, wn[:, b].contiguous().view(plan.V, plan.W, 8), 6).reshape(-1, 8)
+        assert rel_l2(outn[:, b].cpu().numpy(), ref.cpu().numpy()) < 1e-5
     # deterministic: a second launch gives identical bits
     out2 = torch.empty_like(out)
     batch.batch_adjoint([(sm, w, out2, 2)], 6)
@@ -92,14 +95,15 @@ def test_batch_loss_matches_plain_ops(gpu, small_setup):
     L = torch.empty((plan.V * plan.W, 2, 8), device=gpu)
     batch.batch_paths([(sm, batch.interleave(x), L, 2)], 6)
     w, loss = batch.batch_loss(plan, L, 6, att, pdf, tgt)
+    Ln, wn = batch.from_planes(L), batch.from_planes(w)
     for b in range(2):
-        Lb = L[:, b].contiguous().view(plan.V, plan.W, 8)
+        Lb = Ln[:, b].contiguous().view(plan.V, plan.W, 8)
         raw, stats, dIdL = ops.spectral_sum(plan, Lb, 6, att, pdf, want_dIdL=True)
         imax = pdf.sum(1).view(-1, 1, 1)
         r, dI = decompose.log_residual(raw, imax, tgt[b])
         assert loss[b].item() == pytest.approx(r.square().sum().item(), rel=1e-4)
         wref = (dIdL * dI.unsqueeze(-1)).sum(0).reshape(-1, 8)
-        assert rel_l2(w[:, b].cpu().numpy(), wref.cpu().numpy()) < 1e-4
+        assert rel_l2(wn[:, b].cpu().numpy(), wref.cpu().numpy()) < 1e-4
 
 
 def test_batched_solver_matches_plain_loop_and_graph(gpu, small_setup):
@@ -228,10 +232,11 @@ def blocked_setup(gpu, small_setup):
 
 def _random_weights(rays, B, dev, seed):
     import torch
+    from virtualct_amd import batch
     g = torch.Generator(device=dev).manual_seed(seed)
     w = torch.zeros((rays, B, 8), device=dev)
     w[..., :6] = torch.randn((rays, B, 6), generator=g, device=dev)
-    return w
+    return batch.to_planes(w)
 
 
 @pytest.mark.parametrize("B", [1, 3, 16])
@@ -258,7 +263,7 @@ def test_blocked_gathers_equal_direct(gpu, blocked_setup, B):
     batch.batch_adjoint([(smb, w, ob2, B)], 6)
     assert rel_l2(ob.cpu().numpy(), od.cpu().numpy()) < 1e-6
     assert torch.equal(ob, ob2)
-    assert torch.all(ob[..., 6:] == 0)
+    assert torch.all(batch.from_planes(ob)[..., 6:] == 0)
 
 
 def test_blocked_gathers_mixed_groups(gpu, blocked_setup):
@@ -378,13 +383,15 @@ def test_bb_step_matches_fp64_formula(gpu, B):
     import torch
     from virtualct_amd import _lib
     from virtualct_amd._lib import call
+    from virtualct_amd import batch
     n = 4099
     gen = torch.Generator(device=gpu).manual_seed(40 + B)
-    x, xp, g, gp = (torch.zeros((n, B, 8), device=gpu) for _ in range(4))
-    for t in (x, xp, g, gp):
+    nat = [torch.zeros((n, B, 8), device=gpu) for _ in range(4)]
+    for t in nat:
         t[..., :6] = torch.rand((n, B, 6), generator=gen, device=gpu) - 0.3
-    x.clamp_(min=0)
-    xp.clamp_(min=0)
+    nat[0].clamp_(min=0)
+    nat[1].clamp_(min=0)
+    x, xp, g, gp = (batch.to_planes(t) for t in nat)
     alpha = torch.linspace(1e-3, 2e-3, B, device=gpu)
     work = torch.empty(int(_lib.lib().vct_batch_bb_workspace(B)) // 8, dtype=torch.float64, device=gpu)
     P = lambda t: C.c_void_p(t.data_ptr())
@@ -396,11 +403,12 @@ def test_bb_step_matches_fp64_formula(gpu, B):
         if first:
             assert torch.equal(al2, alpha)
         else:
-            s = (x - xp).double()
-            y = (g - gp).double()
+            s = (nat[0] - nat[1]).double()
+            y = (nat[2] - nat[3]).double()
             sy, ss = (s * y).sum(dim=(0, 2)), (s * s).sum(dim=(0, 2))
             bb = torch.where(sy > 0, ss / sy.clamp_min(1e-30), alpha.double() * 2)
             a_ref = bb.clamp(1e-8, 1e2).float()
             assert torch.allclose(al2, a_ref, rtol=2e-7, atol=0)
         assert torch.equal(xp2, x) and torch.equal(gp2, g)
-        assert torch.equal(x2, (x - al2.view(1, -1, 1) * g).clamp_min(0.0))
+        ref = (nat[0] - al2.view(1, -1, 1) * nat[2]).clamp_min(0.0)
+        assert torch.equal(batch.from_planes(x2), ref)

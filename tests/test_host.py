@@ -137,3 +137,26 @@ def test_spectrum_writer_roundtrip(tmp_path, oracle):
         np.testing.assert_array_equal(att, a2)
     cf = spectra.closed_form_attenuation(np.array([0.06, 6.0]), 8)
     assert cf.shape == (8, 2) and np.all(cf > 0)
+
+
+def test_batched_row_layout():
+    """The batched kernels' rows (8*B floats): materials 0-3 of phantom b at
+    4b, 4-5 at 4B+2b, 6-7 at 6B+2b; B = 1 keeps the plain [8]; interleave /
+    deinterleave and to_planes / from_planes invert each other."""
+    import torch
+    from virtualct_amd import batch
+    R, B = 3, 5
+    nat = torch.arange(R * B * 8, dtype=torch.float32).view(R, B, 8)
+    flat = batch.to_planes(nat).view(R, 8 * B)
+    for r in range(R):
+        for b in range(B):
+            for m in range(8):
+                pos = 4 * b + m if m < 4 else (4 * B + 2 * b + m - 4 if m < 6 else 6 * B + 2 * b + m - 6)
+                assert flat[r, pos] == nat[r, b, m]
+    assert torch.equal(batch.from_planes(batch.to_planes(nat)), nat)
+    one = torch.randn(7, 1, 8)
+    assert torch.equal(batch.to_planes(one), one)
+    dens = torch.rand(B, 6, 4, 4)
+    rows = batch.interleave(dens)
+    assert torch.all(batch.from_planes(rows)[..., 6:] == 0)
+    assert torch.equal(batch.deinterleave(rows, 6, 4), dens)

@@ -4,8 +4,9 @@
   (``vct_sysmat_build``), ray-major and optionally voxel-major and blocked
   (regrouped by image tile / view block for L2-resident two-stage gathers,
   with a partial-sum workspace per phantom count).
-* ``interleave`` / ``deinterleave`` -- [B,M,N,N] <-> [N*N,B,8] (the
-  phantom-interleaved layout of the batched kernels).
+* ``interleave`` / ``deinterleave`` -- [B,M,N,N] <-> the phantom-interleaved
+  rows of the batched kernels; ``to_planes`` / ``from_planes`` convert
+  between those rows and plain [rows,B,8] (materials last).
 * ``BatchedSolver`` -- projected-gradient dual-energy decomposition of B
   phantoms: one gather launch for the forward path integrals of all
   phantoms, one fused loss/weights launch for all spectra, one atomic-free
@@ -77,18 +78,40 @@ class SystemMatrix:
             pass
 
 
-def interleave(dens: torch.Tensor) -> torch.Tensor:
-    """[B,M,N,N] float -> [N*N,B,8] (slots >= M zero)."""
-    B, M, N, _ = dens.shape
-    out = torch.zeros((N * N, B, 8), dtype=torch.float32, device=dens.device)
-    out[:, :, :M] = dens.permute(2, 3, 0, 1).reshape(N * N, B, M)
+def to_planes(nat: torch.Tensor) -> torch.Tensor:
+    """[rows,B,8] (materials last) -> the batched kernels' rows, returned in a
+    tensor of the same shape: each row of 8*B floats holds materials 0-3 of
+    phantom b at 4b, materials 4-5 at 4B+2b and 6-7 at 6B+2b (for B = 1 the
+    row is unchanged)."""
+    R, B, _ = nat.shape
+    out = torch.empty_like(nat)
+    flat = out.view(R, 8 * B)
+    flat[:, : 4 * B] = nat[:, :, 0:4].reshape(R, 4 * B)
+    flat[:, 4 * B: 6 * B] = nat[:, :, 4:6].reshape(R, 2 * B)
+    flat[:, 6 * B:] = nat[:, :, 6:8].reshape(R, 2 * B)
     return out
 
 
+def from_planes(rows: torch.Tensor) -> torch.Tensor:
+    """Inverse of ``to_planes``: the batched kernels' rows -> [rows,B,8]."""
+    R, B, _ = rows.shape
+    flat = rows.reshape(R, 8 * B)
+    return torch.cat([flat[:, : 4 * B].reshape(R, B, 4), flat[:, 4 * B: 6 * B].reshape(R, B, 2),
+                      flat[:, 6 * B:].reshape(R, B, 2)], dim=2)
+
+
+def interleave(dens: torch.Tensor) -> torch.Tensor:
+    """[B,M,N,N] float -> N*N rows of the batched layout (materials >= M zero)."""
+    B, M, N, _ = dens.shape
+    nat = torch.zeros((N * N, B, 8), dtype=torch.float32, device=dens.device)
+    nat[:, :, :M] = dens.permute(2, 3, 0, 1).reshape(N * N, B, M)
+    return to_planes(nat)
+
+
 def deinterleave(x: torch.Tensor, M: int, N: int) -> torch.Tensor:
-    """[N*N,B,8] -> [B,M,N,N]."""
+    """N*N rows of the batched layout -> [B,M,N,N]."""
     B = x.shape[1]
-    return x[:, :, :M].reshape(N, N, B, M).permute(2, 3, 0, 1).contiguous()
+    return from_planes(x)[:, :, :M].reshape(N, N, B, M).permute(2, 3, 0, 1).contiguous()
 
 
 def lanes_for(B: int) -> int:
@@ -122,21 +145,23 @@ def _groups(items):
 
 
 def batch_paths(items, M: int, lanes: int | None = None, blocks: int = 0):
-    """items: [(SystemMatrix, dens [NN,B,8], out L [rays,B,8], B)] -> one launch."""
+    """items: [(SystemMatrix, densities (NN rows), out L (rays rows), B)] -> one
+    launch; rows in the batched layout (``to_planes``)."""
     lanes = lanes or lanes_for(max(it[3] for it in items))
     dev = items[0][1].device
     call("vct_batch_paths", _groups(items), len(items), M, lanes, blocks, _stream(dev))
 
 
 def batch_adjoint(items, M: int, lanes: int | None = None, blocks: int = 0):
-    """items: [(SystemMatrix, w [rays,B,8], out grad [NN,B,8], B)] -> one launch."""
+    """items: [(SystemMatrix, w (rays rows), out grad (NN rows), B)] -> one
+    launch; rows in the batched layout (``to_planes``)."""
     lanes = lanes or lanes_for(max(it[3] for it in items))
     dev = items[0][1].device
     call("vct_batch_adjoint", _groups(items), len(items), M, lanes, blocks, _stream(dev))
 
 
 def batch_spectral(plan: ops.Plan, L: torch.Tensor, M: int, att: torch.Tensor, pdf: torch.Tensor):
-    """L [rays,B,8] -> raw [B,S,V,W], stats [B,S,2] (float bits in int32)."""
+    """L (rays rows, batched layout) -> raw [B,S,V,W], stats [B,S,2] (float bits in int32)."""
     B = L.shape[1]
     S, _, NE = att.shape
     raw = torch.empty((B, S, plan.V, plan.W), dtype=torch.float32, device=L.device)
@@ -147,7 +172,7 @@ def batch_spectral(plan: ops.Plan, L: torch.Tensor, M: int, att: torch.Tensor, p
 
 
 def batch_loss(plan: ops.Plan, L, M: int, att, pdf, target, w=None, loss=None):
-    """-> w [rays,B,8] (d loss / d L), loss [B]."""
+    """-> w (rays rows, batched layout: d loss / d L), loss [B]."""
     B = L.shape[1]
     S, _, NE = att.shape
     w = torch.empty_like(L) if w is None else w
@@ -272,8 +297,8 @@ class BatchedSolver:
 class Group:
     plan: ops.Plan
     sysmat: SystemMatrix
-    dens: torch.Tensor    # [NN,B,8]
-    L: torch.Tensor       # [rays,B,8]
+    dens: torch.Tensor    # NN rows, batched layout
+    L: torch.Tensor       # rays rows, batched layout
     B: int
 
 

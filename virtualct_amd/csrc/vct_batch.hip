@@ -8,9 +8,10 @@
 // entries by voxel, so each column lists its rays in increasing order and
 // the gather adjoint is deterministic and atomic-free.
 //
-// Batched data is phantom-interleaved: densities [N*N][B][8] and path
-// integrals / adjoint weights [rays][B][8], so the B lanes that share a row
-// read one contiguous B*32-byte block per entry.  One persistent kernel
+// Batched data is phantom-interleaved in rows of 8*B floats -- densities one
+// row per voxel, path integrals and adjoint weights one row per ray -- each
+// row in three phantom-major planes (see load_slot), so the B lanes that
+// share a row read each plane of an entry's row as one dense run.  One persistent kernel
 // (k_gather) serves both directions over up to kMaxGroups groups of
 // phantoms (one group per geometry), pulling work items round-robin.
 //
@@ -220,8 +221,8 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 struct GatherGroup {
     const uint32_t* ptr;  // [rows+1]
     const uint2* ent;     // entries (index, float bits of value)
-    const float* in;      // [index][B][8]
-    float* out;           // [row][B][8]
+    const float* in;      // [index] rows of 8*B floats
+    float* out;           // [row] rows of 8*B floats
     int B;
     int rows;
     int item_begin;       // first work item of this group
@@ -324,20 +325,40 @@ __device__ __forceinline__ uint2 entry_bcast(uint2 mine)
     return make_uint2(row_bcast<LPR, K>(mine.x), row_bcast<LPR, K>(mine.y));
 }
 
-// x[0..M) = in[idx][b][0..M): one 16-byte load, plus 8 or 16 bytes for M > 4
+// Row layout of the batched arrays.  Row r (a voxel, a ray or a stage-1 run)
+// holds 8*B floats in three planes, each phantom-major:
+//   [0, 4B)    materials 0-3 of phantom b at 4b,
+//   [4B, 6B)   materials 4-5 at 4B + 2b,
+//   [6B, 8B)   materials 6-7 at 6B + 2b,
+// with the materials >= M zero.  The lanes of a row (one per phantom) read
+// an entry's row as one 16-byte load per lane for M <= 4, plus one 8-byte
+// load for M <= 6 and another for M <= 8, each plane a dense run of lines
+// (B = 16: 256 + 128 + 128 bytes).  For B = 1 a row is the plain [8] of the
+// single-phantom kernels.
 template <int M>
-__device__ __forceinline__ void load_slot(const float* __restrict__ in, size_t slot, float (&x)[8])
+__device__ __forceinline__ void load_slot(const float* __restrict__ in, size_t row, int B, int b, float (&x)[8])
 {
-    const float4 lo = *reinterpret_cast<const float4*>(in + slot * 8);
+    const float* r = in + row * (size_t)(8 * B);
+    const float4 lo = *reinterpret_cast<const float4*>(r + 4 * b);
     x[0] = lo.x; x[1] = lo.y; x[2] = lo.z; x[3] = lo.w;
     x[4] = x[5] = x[6] = x[7] = 0.f;
-    if constexpr (M > 6) {
-        const float4 hi = *reinterpret_cast<const float4*>(in + slot * 8 + 4);
-        x[4] = hi.x; x[5] = hi.y; x[6] = hi.z; x[7] = hi.w;
-    } else if constexpr (M > 4) {
-        const float2 hi = *reinterpret_cast<const float2*>(in + slot * 8 + 4);
-        x[4] = hi.x; x[5] = hi.y;
+    if constexpr (M > 4) {
+        const float2 p = *reinterpret_cast<const float2*>(r + 4 * B + 2 * b);
+        x[4] = p.x; x[5] = p.y;
     }
+    if constexpr (M > 6) {
+        const float2 p = *reinterpret_cast<const float2*>(r + 6 * B + 2 * b);
+        x[6] = p.x; x[7] = p.y;
+    }
+}
+
+// All 8 values of (row, b), padding included.
+__device__ __forceinline__ void store_slot(float* __restrict__ out, size_t row, int B, int b, const float (&x)[8])
+{
+    float* r = out + row * (size_t)(8 * B);
+    *reinterpret_cast<float4*>(r + 4 * b) = make_float4(x[0], x[1], x[2], x[3]);
+    *reinterpret_cast<float2*>(r + 4 * B + 2 * b) = make_float2(x[4], x[5]);
+    *reinterpret_cast<float2*>(r + 6 * B + 2 * b) = make_float2(x[6], x[7]);
 }
 
 template <int M>
@@ -355,7 +376,7 @@ __device__ __forceinline__ void gather_one(uint2 mine, int n, const float* __res
     const uint2 e = entry_bcast<LPR, K>(mine);  // every lane of the row takes part
     if (act && K < n) {
         float x[8];
-        load_slot<M>(in, (size_t)e.x * B + b, x);
+        load_slot<M>(in, e.x, B, b, x);
         fma_slot<M>(x, e.y, acc);
     }
 }
@@ -374,10 +395,10 @@ __device__ __forceinline__ void gather_group(uint2 mine, int n, const float* __r
         const uint2 e2 = entry_bcast<LPR, U0 + 2>(mine), e3 = entry_bcast<LPR, U0 + 3>(mine);
         if (!act) return;
         float x0[8], x1[8], x2[8], x3[8];
-        load_slot<M>(in, (size_t)e0.x * B + b, x0);
-        load_slot<M>(in, (size_t)e1.x * B + b, x1);
-        load_slot<M>(in, (size_t)e2.x * B + b, x2);
-        load_slot<M>(in, (size_t)e3.x * B + b, x3);
+        load_slot<M>(in, e0.x, B, b, x0);
+        load_slot<M>(in, e1.x, B, b, x1);
+        load_slot<M>(in, e2.x, B, b, x2);
+        load_slot<M>(in, e3.x, B, b, x3);
         fma_slot<M>(x0, e0.y, acc);
         fma_slot<M>(x1, e1.y, acc);
         fma_slot<M>(x2, e2.y, acc);
@@ -429,13 +450,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a)
         float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int m = 0; m < M; m++) o[m] = (float)(acc[m] * G.scale);
-        float4* dst = reinterpret_cast<float4*>(G.out) + ((size_t)row * G.B + b) * 2;
-        dst[0] = make_float4(o[0], o[1], o[2], o[3]);
-        dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+        store_slot(G.out, row, G.B, b, o);
     }
 }
 
-// One phantom per group (B = 1): the SPL lanes of a row split its entries
+// One phantom per group (B = 1, where a row is the plain [8]): the SPL lanes of a row split its entries
 // (lane k takes entries j0+k, j0+k+SPL, ...: the row is read in contiguous
 // SPL*8-byte runs) and combine their FP64 partial sums by a butterfly over
 // the SPL lanes; 256/SPL rows per item.
@@ -479,13 +498,13 @@ __global__ __launch_bounds__(256) void k_gather_split(GatherArgs a)
 // ---------------------------------------------------- batched spectral sum
 
 struct BatchSpecArgs {
-    const float* L;       // [rays][B][8]
+    const float* L;       // [rays] rows of 8*B floats
     const float* att;     // [S][M][NE]
     const float* pdf;     // [S][NE]
     const float* target;  // [B][S][rays] (loss mode)
     float* raw;           // [B][S][rays] (forward mode)
     uint32_t* stats;      // [B][S][2]
-    float* w;             // [rays][B][8] (loss mode)
+    float* w;             // [rays] rows of 8*B floats (loss mode)
     float* loss;          // [B] (loss mode)
     int rays, W, VT, B, S, M, NE;
 };
@@ -510,9 +529,8 @@ __global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
     const bool valid = ray < a.rays;
     float I = 0.f;
     if (valid && ray / a.W < a.VT) {
-        const float4* Lv = reinterpret_cast<const float4*>(a.L) + ((size_t)ray * a.B + b) * 2;
-        const float4 l0 = Lv[0], l1 = Lv[1];
-        const float Lm[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+        float Lm[8];
+        load_slot<M>(a.L, ray, a.B, b, Lm);
         for (int e = 0; e < NE; e++) {
             const float4 m0 = reinterpret_cast<const float4*>(mu2)[2 * e];
             const float4 m1 = reinterpret_cast<const float4*>(mu2)[2 * e + 1];
@@ -575,9 +593,8 @@ __global__ __launch_bounds__(256) void k_batch_loss(BatchSpecArgs a, int lpr)
     float l = 0.f;
     const bool valid = ray < a.rays && b < a.B;
     if (valid && ray / a.W < a.VT) {
-        const float4* Lv = reinterpret_cast<const float4*>(a.L) + ((size_t)ray * a.B + b) * 2;
-        const float4 l0 = Lv[0], l1 = Lv[1];
-        const float Lm[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+        float Lm[8];
+        load_slot<M>(a.L, ray, a.B, b, Lm);
         for (int s = 0; s < S; s++) {
             float I = 0.f, d[M];
 #pragma unroll
@@ -607,9 +624,7 @@ __global__ __launch_bounds__(256) void k_batch_loss(BatchSpecArgs a, int lpr)
         }
     }
     if (valid) {
-        float4* dst = reinterpret_cast<float4*>(a.w) + ((size_t)ray * a.B + b) * 2;
-        dst[0] = make_float4(wout[0], wout[1], wout[2], wout[3]);
-        dst[1] = make_float4(wout[4], wout[5], wout[6], wout[7]);
+        store_slot(a.w, ray, a.B, b, wout);
         atomicAdd(lsum + b, l);
     }
     __syncthreads();
@@ -618,18 +633,19 @@ __global__ __launch_bounds__(256) void k_batch_loss(BatchSpecArgs a, int lpr)
 
 // ------------------------------------------- Barzilai-Borwein descent step
 //
-// Arrays are phantom-interleaved [n][B][8]: slot k = (voxel, phantom k % B)
-// holds 8 floats.  The grid stride (kBBBlocksPerPhantom * B blocks of 256
-// threads) is a multiple of B, so every thread stays on one phantom; sums are
-// FP64 in a fixed order (per thread, then per block over the threads of that
-// phantom, then over the blocks), so the step is deterministic.
+// The arrays hold n rows of 8*B floats (the layout of load_slot); slot
+// k = (row k / B, phantom k % B) holds the 8 values of one voxel of one
+// phantom.  The grid stride (kBBBlocksPerPhantom * B blocks of 256 threads)
+// is a multiple of B, so every thread stays on one phantom; sums are FP64 in
+// a fixed order (per thread, then per block over the threads of that phantom,
+// then over the blocks), so the step is deterministic.
 constexpr int kBBBlocksPerPhantom = 64;
 
 struct BBArgs {
-    float* x;          // [n][B][8] densities (updated)
-    float* x_prev;     // [n][B][8] previous densities (overwritten with x)
-    float* g_prev;     // [n][B][8] previous gradient (overwritten with g)
-    const float* g;    // [n][B][8] gradient
+    float* x;          // densities (updated)
+    float* x_prev;     // previous densities (overwritten with x)
+    float* g_prev;     // previous gradient (overwritten with g)
+    const float* g;    // gradient
     float* alpha;      // [B] step lengths
     double* part;      // [blocks][B][2] partial sums (s.y, s.s)
     int64_t slots;     // n * B
@@ -637,26 +653,36 @@ struct BBArgs {
     int first;         // 1: no previous iterate (plain projected step)
 };
 
+// This thread's phantom, first row and row stride (the grid stride is a
+// multiple of B).
+struct BBLane {
+    int b;
+    int64_t row, step, rows;
+};
+
+__device__ __forceinline__ BBLane bb_lane(const BBArgs& a)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    return {(int)(k % a.B), k / a.B, stride / a.B, a.slots / a.B};
+}
+
 __global__ __launch_bounds__(256) void k_bb_partial(BBArgs a)
 {
     __shared__ double red[2][256];
     double sy = 0.0, ss = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.slots; k += stride) {
-        const float4* x4 = reinterpret_cast<const float4*>(a.x) + k * 2;
-        const float4* xp4 = reinterpret_cast<const float4*>(a.x_prev) + k * 2;
-        const float4* g4 = reinterpret_cast<const float4*>(a.g) + k * 2;
-        const float4* gp4 = reinterpret_cast<const float4*>(a.g_prev) + k * 2;
+    const BBLane l = bb_lane(a);
+    for (int64_t r = l.row; r < l.rows; r += l.step) {
+        float x[8], xp[8], g[8], gp[8];
+        load_slot<8>(a.x, r, a.B, l.b, x);
+        load_slot<8>(a.x_prev, r, a.B, l.b, xp);
+        load_slot<8>(a.g, r, a.B, l.b, g);
+        load_slot<8>(a.g_prev, r, a.B, l.b, gp);
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const float4 x = x4[h], xp = xp4[h], g = g4[h], gp = gp4[h];
-            const float s[4] = {x.x - xp.x, x.y - xp.y, x.z - xp.z, x.w - xp.w};
-            const float y[4] = {g.x - gp.x, g.y - gp.y, g.z - gp.z, g.w - gp.w};
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                sy += (double)s[i] * (double)y[i];
-                ss += (double)s[i] * (double)s[i];
-            }
+        for (int i = 0; i < 8; i++) {
+            const float s = x[i] - xp[i], y = g[i] - gp[i];
+            sy += (double)s * (double)y;
+            ss += (double)s * (double)s;
         }
     }
     red[0][threadIdx.x] = sy;
@@ -692,21 +718,17 @@ __global__ void k_bb_alpha(BBArgs a, int blocks)
 // x_prev = x, g_prev = g, x = max(x - alpha_b g, 0)
 __global__ __launch_bounds__(256) void k_bb_apply(BBArgs a)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.slots; k += stride) {
-        const float al = a.alpha[k % a.B];
-        float4* x4 = reinterpret_cast<float4*>(a.x) + k * 2;
-        float4* xp4 = reinterpret_cast<float4*>(a.x_prev) + k * 2;
-        float4* gp4 = reinterpret_cast<float4*>(a.g_prev) + k * 2;
-        const float4* g4 = reinterpret_cast<const float4*>(a.g) + k * 2;
+    const BBLane l = bb_lane(a);
+    const float al = a.alpha[l.b];
+    for (int64_t r = l.row; r < l.rows; r += l.step) {
+        float x[8], g[8], xn[8];
+        load_slot<8>(a.x, r, a.B, l.b, x);
+        load_slot<8>(a.g, r, a.B, l.b, g);
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const float4 x = x4[h], g = g4[h];
-            xp4[h] = x;
-            gp4[h] = g;
-            x4[h] = make_float4(fmaxf(x.x - al * g.x, 0.f), fmaxf(x.y - al * g.y, 0.f), fmaxf(x.z - al * g.z, 0.f),
-                                fmaxf(x.w - al * g.w, 0.f));
-        }
+        for (int i = 0; i < 8; i++) xn[i] = fmaxf(x[i] - al * g[i], 0.f);
+        store_slot(a.x_prev, r, a.B, l.b, x);
+        store_slot(a.g_prev, r, a.B, l.b, g);
+        store_slot(a.x, r, a.B, l.b, xn);
     }
 }
 
