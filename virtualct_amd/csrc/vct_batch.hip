@@ -21,7 +21,7 @@
 // adjoint start at different times and so read the rays of every view.
 // The blocked form regroups the entries of each direction by a block key,
 //   forward: the kBlockTile x kBlockTile image tile of the voxel,
-//   adjoint: (view / kAdjViews, tile of the voxel),
+//   adjoint: (view / kAdjViews, tile of the voxel, ranked along the rays),
 // keeping row order inside a block (stable radix sort).  Stage 1 gathers
 // one row per (block, row) run into a partial-sum workspace, block after
 // block, so what a block reads (one tile of densities, or the rays of
@@ -164,12 +164,26 @@ __global__ __launch_bounds__(256) void k_keys_fwd(const uint2* __restrict__ ent,
     if (i < n) key[i] = tg.tile(ent[i].x);
 }
 
-// adjoint block key: (view block of the entry's ray, tile of the row's voxel)
+// adjoint block key: (view block of the entry's ray, rank of the row's tile).
+// Within a view block the tiles are ranked along the block's rays: down the
+// tile columns when its middle view's central ray (source (sid sin t,
+// sid cos t) through the origin) runs closer to the y axis, along the tile
+// rows otherwise, so consecutive blocks share most of their rays and read
+// them from L2 (28 -> 18 GB of HBM reads per adjoint launch at 512^2,
+// 800 views, 16 phantoms).
 __global__ __launch_bounds__(256) void k_keys_adj(const uint2* __restrict__ ent, const uint32_t* __restrict__ vox,
-                                                  int64_t n, TileGrid tg, int W, uint32_t* __restrict__ key)
+                                                  int64_t n, TileGrid tg, int W, int V, const double* __restrict__ cosT,
+                                                  const double* __restrict__ sinT, uint32_t* __restrict__ key)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) key[i] = (ent[i].x / (uint32_t)W / kAdjViews) * (uint32_t)tg.ntiles + tg.tile(vox[i]);
+    if (i >= n) return;
+    const uint32_t vb = ent[i].x / (uint32_t)W / kAdjViews;
+    const int mid = min((int)vb * kAdjViews + kAdjViews / 2, V - 1);
+    const uint32_t v = vox[i];
+    const uint32_t iy = v / (uint32_t)tg.N, ix = v - iy * (uint32_t)tg.N;
+    const uint32_t ty = iy / (uint32_t)tg.T, tx = ix / (uint32_t)tg.T;
+    const bool along_y = fabs(cosT[mid]) > fabs(sinT[mid]);
+    key[i] = vb * (uint32_t)tg.ntiles + (along_y ? tx * (uint32_t)tg.ntx + ty : ty * (uint32_t)tg.ntx + tx);
 }
 
 // head[i] = 1 where a new (block, row) run starts in sorted order; head[n] = 0
@@ -958,7 +972,8 @@ vct_status vct_sysmat_build(vct_plan* p, int flags, vct_sysmat** out, void* stre
         if (st == VCT_OK) st = build_blocked(m->csr, eray, bkey, total, (uint32_t)tg.ntiles, nrays, m->fwd, s);
         if (st == VCT_OK && transpose) {
             const int nvb = (g.n_views + kAdjViews - 1) / kAdjViews;
-            hipLaunchKernelGGL(k_keys_adj, ge, dim3(256), 0, s, m->csc, keys2, (int64_t)total, ta, g.n_channels, bkey);
+            hipLaunchKernelGGL(k_keys_adj, ge, dim3(256), 0, s, m->csc, keys2, (int64_t)total, ta, g.n_channels,
+                               g.n_views, a.cosT, a.sinT, bkey);
             hip(hipGetLastError(), "k_keys_adj");
             if (st == VCT_OK)
                 st = build_blocked(m->csc, keys2, bkey, total, (uint32_t)(nvb * ta.ntiles), NN, m->adj, s);
