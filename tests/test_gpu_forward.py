@@ -252,3 +252,22 @@ def test_fast_march_equals_exact_march(gpu, golden):
         assert np.array_equal(c_fast.cpu().numpy(), c_exact.cpu().numpy())
         assert rel_max(L_fast.cpu().numpy(), L_exact.cpu().numpy()) < 1e-6
         plan.close()
+
+
+@pytest.mark.parametrize("M", [3, 5])
+def test_odd_material_count_u16(gpu, golden, M):
+    """Odd M: integer densities give the float densities' path integrals, and
+    the unused high half of the last packed word is never read."""
+    import torch
+    ops = _ops()
+    plan = ops.plan_for(_geom(128, 0.35, 160))
+    ph = golden.npz("ref_small.npz")["n128_phantom"][:M]
+    pk, dt = ops.pack_densities(plan, _t(ph, gpu))
+    L = ops.trace_paths(plan, pk, dt, M)
+    pkf, dtf = ops.pack_densities(plan, _t(ph.astype(np.float32) / np.float32(1000), gpu))
+    Lf = ops.trace_paths(plan, pkf, dtf, M)
+    assert rel_max(L[..., :M].cpu().numpy(), Lf[..., :M].cpu().numpy()) < 1e-6
+    dirty = pk.clone()
+    dirty[..., M] = 12345  # the high half of word M // 2
+    L2 = ops.trace_paths(plan, dirty, dt, M)
+    assert torch.equal(L2, L)

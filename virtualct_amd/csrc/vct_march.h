@@ -8,9 +8,10 @@
 //
 // march_ray<true> produces the same segments (same voxels, same count, same
 // order) with far fewer FP64 divisions.  The per-step divisions by the ray's
-// direction components are replaced by products with per-ray reciprocals,
-// which moves a computed coordinate by at most ~1.2e-13 cm (|quotient| <= 1.2,
-// |multiplier| <= 143 cm, plus two roundings of values below 2^7).  Every
+// direction components are replaced by one fused multiply-add with a per-ray
+// slope (reciprocal times the other component), which moves a computed
+// coordinate by at most ~1.2e-13 cm (slope within 2 ulp, |offset| <= 143 cm,
+// plus the rounding of a value below 2^7).  Every
 // discrete decision taken from such a coordinate -- the loop bound +-L, the
 // minor-axis cell index, the sign split at 0, the crossing test chy != 0 and
 // the midpoint voxel index -- is first compared against a margin of 1e-9 (cm,
@@ -129,8 +130,9 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
     // plane: exact (:121, :129, :137) and fast forms
     auto b_exact = [&](double a_) { return (a_ - sa) / dA * dB + sb; };
     auto a_exact = [&](double b_) { return (b_ - sb) / dB * dA + sa; };
-    auto b_calc = [&](double a_) { return FAST ? (a_ - sa) * rA * dB + sb : b_exact(a_); };
-    auto a_calc = [&](double b_) { return FAST ? (b_ - sb) * rB * dA + sa : a_exact(b_); };
+    const double kAB = rA * dB, kBA = rB * dA;
+    auto b_calc = [&](double a_) { return FAST ? __builtin_fma(a_ - sa, kAB, sb) : b_exact(a_); };
+    auto a_calc = [&](double b_) { return FAST ? __builtin_fma(b_ - sb, kBA, sa) : a_exact(b_); };
     auto make_exact = [&](MarchPoint& p) {
         if (p.exact) return;
         if (p.kind == 1) p.b = b_exact(p.a);
@@ -157,12 +159,15 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
     // and voxel = ia * stA + ib * stB; no per-segment selects between x and y.
     const double sA = xmajor ? 1.0 : -1.0;
     const int stA = xmajor ? 1 : a.N, stB = xmajor ? a.N : 1;
+    // fast midpoint index: fma(p0 + p1, +-0.5/D, nxm) is the fma of the
+    // reference's +-(p0 + p1) * 0.5 with 1/D (power-of-two scalings are exact)
+    const double hA = 0.5 * sA * a.invD, hB = -0.5 * sA * a.invD;
     auto emit = [&](MarchPoint& p0, MarchPoint& p1) {
         int ia, ib;
         bool decided = false;
         if (FAST) {
-            decided = index_checked(sA * ((p0.a + p1.a) * 0.5), a.invD, (double)a.nxm, ia) &&
-                      index_checked(-sA * ((p0.b + p1.b) * 0.5), a.invD, (double)a.nxm, ib);
+            decided = index_checked(p0.a + p1.a, hA, (double)a.nxm, ia) &&
+                      index_checked(p0.b + p1.b, hB, (double)a.nxm, ib);
             if (!decided) {
                 make_exact(p0);
                 make_exact(p1);
@@ -192,10 +197,12 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
     int iks = 0;
 
     const double nxm = (double)a.nxm;
-    auto in_image = [&](int ia, int ib) { return ia >= 0 && ib >= 0 && ia <= a.N - 1 && ib <= a.N - 1; };
+    // 0 <= ia, ib < N in one unsigned comparison (negatives wrap above N)
+    auto in_image = [&](int ia, int ib) { return max((uint32_t)ia, (uint32_t)ib) < (uint32_t)a.N; };
 
     bool alive = alive_in;
-    for (int ik = 0;; ik++) {
+    double pik = 0.0;  // pha * ik, exact
+    for (int ik = 0;; ik++, pik += pha) {
         if constexpr (SYNC) {
             // wave-uniform loop: finished lanes idle until every lane is done
             if (alive && !(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) alive = false;
@@ -220,7 +227,7 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
                 // below would make no point exact and take exactly these
                 // decisions with exactly these values, so the step is committed
                 // here without its branches; otherwise the general step runs.
-                const double na = ain + pha * ik * D;
+                const double na = ain + pik * D;
                 const double nb = b_calc(na);
                 const double nab = fabs(nb);
                 int nbma;
@@ -232,13 +239,13 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
                 const double xa = a_calc(xb);
                 const double dxa = na - xa, dxb = nb - xb;
                 sure &= !(cand & (fabs(dxa) < kMarchMargin) & (fabs(dxb) < kMarchMargin));
-                const bool cross = cand & ((dxa * dxa + dxb * dxb) != 0);
+                // chy != 0 of :141: when the step is committed, cand implies
+                // |dxa| or |dxb| >= margin, so chy > 0
+                const bool cross = cand;
                 int ia1, ib1, ia2, ib2;
-                const bool ok1 = index_checked(sA * ((p0.a + xa) * 0.5), a.invD, nxm, ia1) &
-                                 index_checked(-sA * ((p0.b + xb) * 0.5), a.invD, nxm, ib1);
+                const bool ok1 = index_checked(p0.a + xa, hA, nxm, ia1) & index_checked(p0.b + xb, hB, nxm, ib1);
                 const double s_a = cross ? xa : p0.a, s_b = cross ? xb : p0.b;
-                const bool ok2 = index_checked(sA * ((s_a + na) * 0.5), a.invD, nxm, ia2) &
-                                 index_checked(-sA * ((s_b + nb) * 0.5), a.invD, nxm, ib2);
+                const bool ok2 = index_checked(s_a + na, hA, nxm, ia2) & index_checked(s_b + nb, hB, nxm, ib2);
                 sure &= (ok1 | !cross) & ok2;
                 if (sure) {
                     const bool v1 = cross && in_image(ia1, ib1);
@@ -264,7 +271,7 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
                 }
             }
             if (!committed) {
-                pc.a = ain + pha * ik * D;  // :126
+                pc.a = ain + pik * D;  // :126
                 pc.b = b_calc(pc.a);        // :129
                 pc.exact = !FAST;
                 if (FAST) {

@@ -29,36 +29,58 @@ namespace vct {
 
 // ------------------------------------------------------------------ sinks
 
-// Forward sinks accumulate L_m += rho_m(voxel) * length in FP64.  add2
+// Forward sinks accumulate L_m += rho_m(voxel) * length in FP64 (the u16
+// sink per 32-bit word of two materials: the word as one integer and its high
+// half, which resolves the low half to ~1e-11 relative).  add2
 // takes the two candidate segments of one march step and issues both gathers
 // before using either; a segment of length 0 at voxel 0 adds exactly nothing
 // (fma(rho, +0, acc) == acc), so the march calls it without branching.
 template <int M>
 struct SinkU16 {
     static constexpr bool kAdd2 = true;
-    const uint4* __restrict__ vox;
+    const char* __restrict__ vox;
+    // word k of a voxel holds materials 2k (low half) and 2k+1 (high half);
+    // accw[k] sums the whole word as one integer lo + 65536 hi and acch[k]
+    // the high half, so material 2k is accw[k] - 65536 acch[k] at the end
+    double accw[(M + 1) / 2], acch[M / 2];
     double acc[M];
     __device__ __forceinline__ void init(const void* p)
     {
-        vox = reinterpret_cast<const uint4*>(p);
+        vox = reinterpret_cast<const char*>(p);
 #pragma unroll
-        for (int m = 0; m < M; m++) acc[m] = 0.0;
+        for (int k = 0; k < (M + 1) / 2; k++) accw[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < M / 2; k++) acch[k] = 0.0;
+    }
+    __device__ __forceinline__ uint4 load(int v) const
+    {
+        // 32-bit byte offset (N*N*16 bytes < 4 GB)
+        return *reinterpret_cast<const uint4*>(vox + (uint32_t)v * 16u);
     }
     __device__ __forceinline__ void accumulate(const uint4& q, double rr)
     {
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            const uint32_t u = (m & 1) ? (w[m >> 1] >> 16) : (w[m >> 1] & 0xffffu);
-            acc[m] = __builtin_fma((double)u, rr, acc[m]);
+        for (int k = 0; k < (M + 1) / 2; k++) {
+            // odd M: the last word's high half is not a material of this call
+            const uint32_t wk = (2 * k + 1 < M) ? w[k] : (w[k] & 0xffffu);
+            accw[k] = __builtin_fma((double)wk, rr, accw[k]);
         }
+#pragma unroll
+        for (int k = 0; k < M / 2; k++) acch[k] = __builtin_fma((double)(w[k] >> 16), rr, acch[k]);
     }
-    __device__ __forceinline__ void add(int v, double rr) { accumulate(vox[v], rr); }
+    __device__ __forceinline__ void add(int v, double rr) { accumulate(load(v), rr); }
     __device__ __forceinline__ void add2(int v1, double r1, int v2, double r2)
     {
-        const uint4 q1 = vox[v1], q2 = vox[v2];
+        const uint4 q1 = load(v1), q2 = load(v2);
         accumulate(q1, r1);
         accumulate(q2, r2);
+    }
+    __device__ __forceinline__ void finish()
+    {
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            acc[m] = (m & 1) ? acch[m >> 1] : accw[m >> 1] - (m / 2 < M / 2 ? 65536.0 * acch[m >> 1] : 0.0);
     }
 };
 
@@ -198,6 +220,7 @@ __global__ __launch_bounds__(256) void k_trace_fwd(TraceArgs a)
             SinkU16<M> sink;
             sink.init(a.dens);
             iks = march_ray<FAST>(a, view, ch, sink, ovf);
+            sink.finish();
 #pragma unroll
             for (int m = 0; m < M; m++) out[m] = (float)(sink.acc[m] * a.out_scale);
         } else {
