@@ -241,8 +241,13 @@ vct_status vct_trace_adjoint(vct_plan* plan, const void* d_packed, int dens_type
  *                         (forward) and by 50-view block x tile (adjoint),
  *                         so that each block's reads stay in one L2; used by
  *                         the batched gathers of groups that pass a
- *                         workspace.  No counterpart in the reference. */
-enum { VCT_SYSMAT_TRANSPOSE = 1, VCT_SYSMAT_BLOCKED = 2 };
+ *                         workspace.  With it, the forward stage 1 of a
+ *                         group reads the 64x64 tile of its phantom from LDS
+ *                         (runs sorted by length in interleaved 64-run
+ *                         slices), unless
+ *   VCT_SYSMAT_NO_TILE    is also given (the same sums, read through L2).
+ * No counterpart in the reference. */
+enum { VCT_SYSMAT_TRANSPOSE = 1, VCT_SYSMAT_BLOCKED = 2, VCT_SYSMAT_NO_TILE = 4 };
 typedef struct vct_sysmat vct_sysmat;
 vct_status vct_sysmat_build(vct_plan* plan, int flags, vct_sysmat** out, void* stream);
 vct_status vct_sysmat_destroy(vct_sysmat* m);

@@ -412,3 +412,31 @@ def test_bb_step_matches_fp64_formula(gpu, B):
         assert torch.equal(xp2, x) and torch.equal(gp2, g)
         ref = (nat[0] - al2.view(1, -1, 1) * nat[2]).clamp_min(0.0)
         assert torch.equal(batch.from_planes(x2), ref)
+
+
+@pytest.mark.parametrize("N,pix,B,M", [(128, 0.35, 3, 6), (96, 0.45, 16, 5), (160, 0.28, 2, 8), (128, 0.35, 1, 3)])
+def test_tile_stage1_equals_blocked_stage1(gpu, N, pix, B, M):
+    """The forward stage 1 over the tile-resident form (the tile of each
+    phantom in LDS, runs sorted by length in 64-run slices; partial edge
+    tiles at 96^2 and 160^2) gives the bits of the blocked stage 1 read
+    through L2, for 1..16 phantoms and 3..8 materials."""
+    import torch
+    ops, batch, _ = _mods()
+    plan = ops.plan_for(_geom(N, pix, 160))
+    smt = batch.SystemMatrix(plan, transpose=True, blocked=True)
+    smn = batch.SystemMatrix(plan, transpose=True, blocked=True, tiles=False)
+    assert smt.bytes > smn.bytes
+    rays = plan.V * plan.W
+    dens = _phantoms(B, N, pix, gpu, seed=4000 + N)
+    if M > 6:
+        dens = torch.cat([dens, dens[:, : M - 6] * 0.5], dim=1)
+    x = batch.interleave(dens[:, :M].contiguous())
+    Lt, Ln = torch.empty((rays, B, 8), device=gpu), torch.empty((rays, B, 8), device=gpu)
+    smt.workspace(B, 0).zero_()
+    smn.workspace(B, 0).zero_()
+    batch.batch_paths([(smt, x, Lt, B)], M)
+    batch.batch_paths([(smn, x, Ln, B)], M)
+    assert torch.equal(Lt, Ln)
+    assert torch.equal(smt.workspace(B, 0), smn.workspace(B, 0))
+    smt.close()
+    smn.close()

@@ -21,6 +21,7 @@ FILTER_RAMLAK, FILTER_SHEPP_LOGAN = 1, 2
 OPT_EXACT_TRACE = 1
 SYSMAT_TRANSPOSE = 1
 SYSMAT_BLOCKED = 2
+SYSMAT_NO_TILE = 4
 
 
 class VctError(RuntimeError):

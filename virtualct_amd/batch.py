@@ -38,11 +38,13 @@ def _p(t: torch.Tensor | None) -> C.c_void_p:
 
 
 class SystemMatrix:
-    def __init__(self, plan: ops.Plan, transpose: bool = True, blocked: bool = False):
+    def __init__(self, plan: ops.Plan, transpose: bool = True, blocked: bool = False, tiles: bool = True):
         self.plan = plan
         self.handle = C.c_void_p()
         dev = torch.device("cuda", plan.device)
         flags = (_lib.SYSMAT_TRANSPOSE if transpose else 0) | (_lib.SYSMAT_BLOCKED if blocked else 0)
+        if blocked and not tiles:
+            flags |= _lib.SYSMAT_NO_TILE
         with torch.cuda.device(plan.device):
             call("vct_sysmat_build", plan.handle, flags, C.byref(self.handle), _stream(dev))
         L = _lib.lib()

@@ -28,7 +28,9 @@
 // kAdjViews views that cross one tile) stays in the L2 of the XCD that owns
 // it; stage 2 sums each row's partials in block order.  Both stages are
 // ordinary k_gather launches and the result is deterministic.
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -45,6 +47,23 @@ struct vct_blocked {
     uint2* ent2 = nullptr;        // [rows1] (stage-1 row, 1.0f), block order per row
 };
 
+// Tile-resident form of the forward stage 1 (built with the blocked form
+// unless VCT_SYSMAT_NO_TILE): the stage-1 runs of each forward tile sorted by
+// length (longest first) in slices of 64, whose entries are interleaved so
+// that lane l of a wave reads entry k of run l at slice_ptr + 64 k + l.  An
+// entry holds its voxel's index inside the tile (ly * 64 + lx) and the
+// length; padding entries are (0, 0.0f).
+struct vct_tiled {
+    int ntiles = 0, nslices = 0, kmax = 0, ntx = 0;
+    uint2* ell = nullptr;            // [sum slice_len * 64]
+    uint32_t* slice_ptr = nullptr;   // [nslices] first entry of the slice
+    uint32_t* slice_len = nullptr;   // [nslices] entries per lane
+    int32_t* slice_runs = nullptr;   // [nslices * 64] stage-1 row of each lane, -1: none
+    int32_t* tile_slices = nullptr;  // [ntiles + 1]
+    int32_t* xcd_tiles = nullptr;    // [8 * kmax] tiles of each XCD, -1: none
+    size_t n_ell = 0;
+};
+
 struct vct_sysmat {
     vct_plan* plan = nullptr;
     int64_t nseg = 0;
@@ -56,6 +75,7 @@ struct vct_sysmat {
     uint32_t* col_ptr = nullptr;  // [NN+1]
     uint2* csc = nullptr;         // [nseg] (ray, float bits of length)
     vct_blocked fwd, adj;         // blocked forms (rows1 == 0: not built)
+    vct_tiled tile;               // tile-resident forward stage 1 (ell == nullptr: not built)
 };
 
 namespace vct {
@@ -761,6 +781,155 @@ __global__ void k_zero_f32(float* p, int n)
     if (i < n) p[i] = 0.f;
 }
 
+
+// ------------------------------------------- tile-resident forward stage 1
+
+constexpr int kTileThreads = 1024;  // one workgroup per CU (LDS)
+constexpr int kTileUnroll = 8;      // entries in flight per lane
+
+// Build: per stage-1 run, sort key tile << 16 | (0xffff - length) (longest
+// first within a tile) and the tile's run count.
+__global__ __launch_bounds__(256) void k_tile_run_keys(const uint32_t* __restrict__ ptr1, const uint2* __restrict__ ent1,
+                                                       uint32_t rows1, TileGrid tg, uint32_t* __restrict__ key,
+                                                       uint32_t* __restrict__ val, uint32_t* __restrict__ tile_cnt,
+                                                       uint32_t* __restrict__ err)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows1) return;
+    const uint32_t len = ptr1[r + 1] - ptr1[r];
+    const uint32_t t = tg.tile(ent1[ptr1[r]].x);
+    if (len == 0u || len >= 0xffffu) atomicOr(err, 1u);
+    key[r] = (t << 16) | (0xffffu - min(len, 0xfffeu));
+    val[r] = r;
+    atomicAdd(tile_cnt + t, 1u);
+}
+
+// Build: entries per lane of each slice = the length of its first (longest) run.
+__global__ __launch_bounds__(256) void k_tile_slice_len(const uint32_t* __restrict__ ptr1, const uint32_t* __restrict__ order,
+                                                        const int32_t* __restrict__ slice_tile,
+                                                        const uint32_t* __restrict__ tile_begin,
+                                                        const int32_t* __restrict__ tile_slices, int nslices,
+                                                        uint32_t* __restrict__ slice_len)
+{
+    const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= nslices) return;
+    const int t = slice_tile[sl];
+    const uint32_t r = order[tile_begin[t] + (uint32_t)(sl - tile_slices[t]) * 64u];
+    slice_len[sl] = ptr1[r + 1] - ptr1[r];
+}
+
+// Build: the interleaved entries with tile-local voxel indices, one thread
+// per run (in sorted order).
+__global__ __launch_bounds__(256) void k_tile_fill(const uint32_t* __restrict__ ptr1, const uint2* __restrict__ ent1,
+                                                   const uint32_t* __restrict__ key_s, const uint32_t* __restrict__ order,
+                                                   uint32_t rows1, const uint32_t* __restrict__ tile_begin,
+                                                   const int32_t* __restrict__ tile_slices,
+                                                   const uint32_t* __restrict__ slice_ptr, TileGrid tg,
+                                                   uint2* __restrict__ ell, int32_t* __restrict__ slice_runs,
+                                                   uint32_t* __restrict__ err)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows1) return;
+    const uint32_t r = order[i], t = key_s[i] >> 16;
+    const uint32_t p = i - tile_begin[t];
+    const uint32_t sl = (uint32_t)tile_slices[t] + p / 64u, lane = p % 64u;
+    slice_runs[sl * 64u + lane] = (int32_t)r;
+    const uint32_t tx0 = (t % (uint32_t)tg.ntx) * 64u, ty0 = (t / (uint32_t)tg.ntx) * 64u;
+    uint2* dst = ell + slice_ptr[sl] + lane;
+    for (uint32_t j = ptr1[r], k = 0; j < ptr1[r + 1]; j++, k++) {
+        const uint2 e = ent1[j];
+        const uint32_t iy = e.x / (uint32_t)tg.N, ix = e.x - iy * (uint32_t)tg.N;
+        const uint32_t ly = iy - ty0, lx = ix - tx0;  // wraps above 63 if outside the tile
+        if (ly > 63u || lx > 63u) atomicOr(err, 2u);
+        dst[(size_t)k * 64u] = make_uint2((ly & 63u) * 64u + (lx & 63u), e.y);
+    }
+}
+
+struct TileArgs {
+    const float* in;             // densities: rows of 8*B floats
+    float* out;                  // stage-1 partial sums: rows of 8*B floats
+    const uint2* ell;
+    const uint32_t* slice_ptr;
+    const uint32_t* slice_len;
+    const int32_t* slice_runs;
+    const int32_t* tile_slices;
+    const int32_t* xcd_tiles;
+    int kmax, ntx, N, B;
+};
+
+// Forward stage 1 with the tile in LDS.  Workgroup g runs on XCD g % 8
+// (round-robin dispatch) and takes phantom b of tile xcd_tiles[xcd][k], so
+// the B workgroups of a tile share one L2 for its entries.  The tile's M
+// materials of phantom b go to LDS (64 rows of 64 voxels x MP floats, rows
+// padded by 2 floats so that voxels one row apart fall 2 banks apart); each
+// lane then sums one run of a slice in FP64, 8 entries in flight, and writes
+// its partial sums as stage-1 row run, phantom b.  The sums are those of
+// k_gather over the same runs, term for term in the same order.
+template <int M>
+__global__ __launch_bounds__(kTileThreads) void k_gather_tile(TileArgs a)
+{
+    constexpr int MP = (M + 1) & ~1;  // floats per voxel in LDS
+    constexpr int ROW = 64 * MP + 2;  // floats per tile row
+    constexpr int U = kTileUnroll;
+    __shared__ float lds[64 * ROW];
+    const int g = blockIdx.x, xcd = g % kXcds, slot = g / kXcds;
+    const int k = slot / a.B, b = slot % a.B;
+    if (k >= a.kmax) return;  // the whole workgroup
+    const int t = a.xcd_tiles[xcd * a.kmax + k];
+    if (t < 0) return;
+    const int tx0 = (t % a.ntx) * 64, ty0 = (t / a.ntx) * 64;
+    for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
+        const int ly = i >> 6, lx = i & 63, gy = ty0 + ly, gx = tx0 + lx;
+        float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (gy < a.N && gx < a.N) load_slot<M>(a.in, (size_t)gy * a.N + gx, a.B, b, v);
+        float2* d = reinterpret_cast<float2*>(lds + ly * ROW + lx * MP);
+#pragma unroll
+        for (int q = 0; q < MP / 2; q++) d[q] = make_float2(v[2 * q], v[2 * q + 1]);
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int sl = a.tile_slices[t] + w; sl < a.tile_slices[t + 1]; sl += nw) {
+        const int len = (int)a.slice_len[sl];
+        const uint2* e = a.ell + a.slice_ptr[sl] + lane;
+        double acc[MP];
+#pragma unroll
+        for (int m = 0; m < MP; m++) acc[m] = 0.0;
+        uint2 q[U], nq[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) q[u] = u < len ? e[u * 64] : make_uint2(0u, 0u);
+        for (int j = 0; j < len; j += U) {
+            // padding entries past len are (0, 0.0f): they add exact zeros
+#pragma unroll
+            for (int u = 0; u < U; u++) nq[u] = j + U + u < len ? e[(j + U + u) * 64] : make_uint2(0u, 0u);
+            float2 f[U][MP / 2];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const float2* p = reinterpret_cast<const float2*>(lds + (q[u].x >> 6) * ROW + (q[u].x & 63) * MP);
+#pragma unroll
+                for (int h = 0; h < MP / 2; h++) f[u][h] = p[h];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const double v = __uint_as_float(q[u].y);
+#pragma unroll
+                for (int h = 0; h < MP / 2; h++) {
+                    acc[2 * h] = __builtin_fma((double)f[u][h].x, v, acc[2 * h]);
+                    acc[2 * h + 1] = __builtin_fma((double)f[u][h].y, v, acc[2 * h + 1]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) q[u] = nq[u];
+        }
+        const int run = a.slice_runs[sl * 64 + lane];
+        if (run >= 0) {
+            float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int m = 0; m < M; m++) o[m] = (float)acc[m];
+            store_slot(a.out, (size_t)run, a.B, b, o);
+        }
+    }
+}
+
 }  // namespace vct
 
 using namespace vct;
@@ -875,15 +1044,158 @@ static vct_status build_blocked(const uint2* ent, const uint32_t* row_of, uint32
     return st;
 }
 
+
+static void free_tiled(vct_tiled& t)
+{
+    void* bufs[] = {t.ell, t.slice_ptr, t.slice_len, t.slice_runs, t.tile_slices, t.xcd_tiles};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    t = vct_tiled{};
+}
+
+// Tile-resident form of the forward stage 1 from its blocked form f (block =
+// forward tile of tg).  Leaves t empty, and returns VCT_OK, when a run does
+// not fit the form (a run longer than 0xfffe entries, more than 65535 tiles
+// or 2^32 interleaved entries); HIP failures are errors.
+static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tiled& t, hipStream_t s)
+{
+    vct_status st = VCT_OK;
+    auto hip = [&](hipError_t e, const char* what) {
+        if (st == VCT_OK && e != hipSuccess) st = set_error(VCT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    };
+    t = vct_tiled{};
+    const uint32_t R = f.rows1;
+    if (R == 0 || tg.ntiles > 65535 || tg.T != 64) return VCT_OK;
+    uint32_t *key = nullptr, *key_s = nullptr, *val = nullptr, *order = nullptr, *cnt = nullptr, *err = nullptr;
+    uint32_t* tile_begin = nullptr;
+    int32_t* slice_tile = nullptr;
+    void* tmp = nullptr;
+    size_t tb = 0;
+    const dim3 gr((R + 255) / 256), blk(256);
+    hip(hipMalloc(&key, 4 * (size_t)R), "hipMalloc tile keys");
+    hip(hipMalloc(&key_s, 4 * (size_t)R), "hipMalloc tile keys");
+    hip(hipMalloc(&val, 4 * (size_t)R), "hipMalloc tile order");
+    hip(hipMalloc(&order, 4 * (size_t)R), "hipMalloc tile order");
+    hip(hipMalloc(&cnt, 4 * (size_t)tg.ntiles + 16), "hipMalloc tile counts");
+    hip(hipMalloc(&err, 16), "hipMalloc tile err");
+    if (st == VCT_OK) {
+        hipLaunchKernelGGL(k_zero_f32, dim3((tg.ntiles + 256) / 256), blk, 0, s, (float*)cnt, tg.ntiles + 1);
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), blk, 0, s, (float*)err, 1);
+        hipLaunchKernelGGL(k_tile_run_keys, gr, blk, 0, s, f.ptr1, f.ent1, R, tg, key, val, cnt, err);
+        hip(hipGetLastError(), "k_tile_run_keys");
+    }
+    hip(rocprim::radix_sort_pairs(nullptr, tb, key, key_s, val, order, (size_t)R, 0u, 32u, s), "sort size");
+    hip(hipMalloc(&tmp, tb + 16), "hipMalloc sort");
+    hip(rocprim::radix_sort_pairs(tmp, tb, key, key_s, val, order, (size_t)R, 0u, 32u, s), "sort");
+    // host: tile begins (in sorted order), slices per tile, slice -> tile
+    std::vector<uint32_t> n_t(tg.ntiles);
+    uint32_t herr = 0;
+    if (st == VCT_OK) hip(hipMemcpyAsync(n_t.data(), cnt, 4 * (size_t)tg.ntiles, hipMemcpyDeviceToHost, s), "copy");
+    if (st == VCT_OK) hip(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s), "copy");
+    if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
+    std::vector<uint32_t> begin(tg.ntiles);
+    std::vector<int32_t> tslices(tg.ntiles + 1, 0), stile;
+    uint32_t acc = 0;
+    for (int k = 0; k < tg.ntiles; k++) {
+        begin[k] = acc;
+        acc += n_t[k];
+        tslices[k + 1] = tslices[k] + (int32_t)((n_t[k] + 63) / 64);
+        for (uint32_t q = 0; q < (n_t[k] + 63) / 64; q++) stile.push_back(k);
+    }
+    const int nslices = tslices[tg.ntiles];
+    bool ok = st == VCT_OK && herr == 0 && acc == R && nslices > 0;
+    if (ok) {
+        hip(hipMalloc(&tile_begin, 4 * (size_t)tg.ntiles), "hipMalloc tile begin");
+        hip(hipMalloc(&t.tile_slices, 4 * ((size_t)tg.ntiles + 1)), "hipMalloc tile slices");
+        hip(hipMalloc(&slice_tile, 4 * (size_t)nslices), "hipMalloc slice tile");
+        hip(hipMalloc(&t.slice_len, 4 * (size_t)nslices), "hipMalloc slice len");
+        hip(hipMalloc(&t.slice_ptr, 4 * (size_t)nslices), "hipMalloc slice ptr");
+        hip(hipMalloc(&t.slice_runs, 4 * 64 * (size_t)nslices), "hipMalloc slice runs");
+        if (st == VCT_OK) {
+            hip(hipMemcpyAsync(tile_begin, begin.data(), 4 * (size_t)tg.ntiles, hipMemcpyHostToDevice, s), "copy");
+            hip(hipMemcpyAsync(t.tile_slices, tslices.data(), 4 * ((size_t)tg.ntiles + 1), hipMemcpyHostToDevice, s),
+                "copy");
+            hip(hipMemcpyAsync(slice_tile, stile.data(), 4 * (size_t)nslices, hipMemcpyHostToDevice, s), "copy");
+            hipLaunchKernelGGL(k_tile_slice_len, dim3((nslices + 255) / 256), blk, 0, s, f.ptr1, order, slice_tile,
+                               tile_begin, t.tile_slices, nslices, t.slice_len);
+            hip(hipGetLastError(), "k_tile_slice_len");
+        }
+    }
+    // host: slice offsets, per-tile work and the XCD tile lists (largest first
+    // onto the least loaded XCD)
+    std::vector<uint32_t> slen(ok ? nslices : 0), sptr(ok ? nslices : 0);
+    if (ok && st == VCT_OK) {
+        hip(hipMemcpyAsync(slen.data(), t.slice_len, 4 * (size_t)nslices, hipMemcpyDeviceToHost, s), "copy");
+        hip(hipStreamSynchronize(s), "sync");
+    }
+    uint64_t total = 0;
+    std::vector<uint64_t> work(tg.ntiles, 0);
+    if (ok && st == VCT_OK) {
+        for (int sl = 0; sl < nslices; sl++) {
+            sptr[sl] = (uint32_t)total;
+            total += 64ull * slen[sl];
+            work[stile[sl]] += 64ull * slen[sl];
+        }
+        ok = total < (1ull << 32);
+    }
+    if (ok && st == VCT_OK) {
+        std::vector<int> idx(tg.ntiles);
+        for (int k = 0; k < tg.ntiles; k++) idx[k] = k;
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return work[x] > work[y]; });
+        std::vector<std::vector<int32_t>> lists(kXcds);
+        std::vector<uint64_t> load(kXcds, 0);
+        for (int k : idx) {
+            if (n_t[k] == 0) continue;
+            int best = 0;
+            for (int x = 1; x < kXcds; x++)
+                if (load[x] < load[best]) best = x;
+            lists[best].push_back(k);
+            load[best] += work[k];
+        }
+        int kmax = 0;
+        for (auto& l : lists) kmax = std::max(kmax, (int)l.size());
+        std::vector<int32_t> xt((size_t)kXcds * kmax, -1);
+        for (int x = 0; x < kXcds; x++)
+            for (size_t q = 0; q < lists[x].size(); q++) xt[(size_t)x * kmax + q] = lists[x][q];
+        t.ntiles = tg.ntiles;
+        t.ntx = tg.ntx;
+        t.nslices = nslices;
+        t.kmax = kmax;
+        t.n_ell = (size_t)total;
+        hip(hipMalloc(&t.xcd_tiles, 4 * xt.size()), "hipMalloc xcd tiles");
+        hip(hipMalloc(&t.ell, 8 * (size_t)total + 16), "hipMalloc tile entries");
+        if (st == VCT_OK) {
+            hip(hipMemcpyAsync(t.xcd_tiles, xt.data(), 4 * xt.size(), hipMemcpyHostToDevice, s), "copy");
+            hip(hipMemcpyAsync(t.slice_ptr, sptr.data(), 4 * (size_t)nslices, hipMemcpyHostToDevice, s), "copy");
+            hip(hipMemsetAsync(t.ell, 0, 8 * (size_t)total + 16, s), "memset");
+            hip(hipMemsetAsync(t.slice_runs, 0xff, 4 * 64 * (size_t)nslices, s), "memset");
+            hipLaunchKernelGGL(k_zero_f32, dim3(1), blk, 0, s, (float*)err, 1);
+            hipLaunchKernelGGL(k_tile_fill, gr, blk, 0, s, f.ptr1, f.ent1, key_s, order, R, tile_begin, t.tile_slices,
+                               t.slice_ptr, tg, t.ell, t.slice_runs, err);
+            hip(hipGetLastError(), "k_tile_fill");
+            hip(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s), "copy");
+            hip(hipStreamSynchronize(s), "sync");
+            ok = herr == 0;
+        }
+    }
+    void* bufs[] = {key, key_s, val, order, cnt, err, tile_begin, slice_tile, tmp};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (st != VCT_OK || !ok || !t.ell) free_tiled(t);
+    return st;
+}
+
 extern "C" {
 
 vct_status vct_sysmat_build(vct_plan* p, int flags, vct_sysmat** out, void* stream)
 {
     clear_error();
     if (!p || !out) return set_error(VCT_ERR_ARG, "NULL argument");
-    if (flags & ~(VCT_SYSMAT_TRANSPOSE | VCT_SYSMAT_BLOCKED)) return set_error(VCT_ERR_ARG, "unknown flags %#x", flags);
+    if (flags & ~(VCT_SYSMAT_TRANSPOSE | VCT_SYSMAT_BLOCKED | VCT_SYSMAT_NO_TILE))
+        return set_error(VCT_ERR_ARG, "unknown flags %#x", flags);
     const bool transpose = (flags & VCT_SYSMAT_TRANSPOSE) != 0;
     const bool blocked = (flags & VCT_SYSMAT_BLOCKED) != 0;
+    const bool tiled = blocked && !(flags & VCT_SYSMAT_NO_TILE);
     *out = nullptr;
     hipStream_t s = (hipStream_t)stream;
     const vct_geometry& g = p->g;
@@ -970,6 +1282,7 @@ vct_status vct_sysmat_build(vct_plan* p, int flags, vct_sysmat** out, void* stre
             hip(hipGetLastError(), "k_keys_fwd");
         }
         if (st == VCT_OK) st = build_blocked(m->csr, eray, bkey, total, (uint32_t)tg.ntiles, nrays, m->fwd, s);
+        if (st == VCT_OK && tiled) st = build_tiled(m->fwd, tg, m->tile, s);
         if (st == VCT_OK && transpose) {
             const int nvb = (g.n_views + kAdjViews - 1) / kAdjViews;
             hipLaunchKernelGGL(k_keys_adj, ge, dim3(256), 0, s, m->csc, keys2, (int64_t)total, ta, g.n_channels,
@@ -1003,6 +1316,7 @@ vct_status vct_sysmat_destroy(vct_sysmat* m)
         if (b) (void)hipFree(b);
     free_blocked(m->fwd);
     free_blocked(m->adj);
+    free_tiled(m->tile);
     delete m;
     return VCT_OK;
 }
@@ -1020,6 +1334,9 @@ size_t vct_sysmat_bytes(const vct_sysmat* m)
         if (fs[k]->rows1)
             b += sizeof(uint2) * (size_t)m->nseg + sizeof(uint32_t) * ((size_t)fs[k]->rows1 + 1) +
                  sizeof(uint32_t) * ((size_t)rows[k] + 1) + sizeof(uint2) * ((size_t)fs[k]->rows1 + 1);
+    if (m->tile.ell)
+        b += sizeof(uint2) * m->tile.n_ell + sizeof(uint32_t) * 66 * (size_t)m->tile.nslices +
+             sizeof(int32_t) * ((size_t)m->tile.ntiles + 1 + (size_t)kXcds * m->tile.kmax);
     return b;
 }
 
@@ -1068,6 +1385,23 @@ static vct_status run_gather(GatherArgs& a, int items, int M, int lanes, int blo
     return VCT_OK;
 }
 
+// Forward stage 1 of one group over the tile-resident form: B workgroups per
+// tile, kmax tiles per XCD.
+static vct_status run_gather_tile(const vct_batch_group& G, int M, hipStream_t s)
+{
+    const vct_tiled& t = G.sysmat->tile;
+    TileArgs ta{G.d_in, G.d_work, t.ell, t.slice_ptr, t.slice_len, t.slice_runs, t.tile_slices, t.xcd_tiles,
+                t.kmax, t.ntx, G.sysmat->plan->g.recon_size, G.n_phantoms};
+    const unsigned grid = (unsigned)(kXcds * t.kmax * G.n_phantoms);
+#define VCT_TILE_M(MM) \
+    if (M == MM) hipLaunchKernelGGL(k_gather_tile<MM>, dim3(grid), dim3(kTileThreads), 0, s, ta);
+    VCT_TILE_M(1) VCT_TILE_M(2) VCT_TILE_M(3) VCT_TILE_M(4)
+    VCT_TILE_M(5) VCT_TILE_M(6) VCT_TILE_M(7) VCT_TILE_M(8)
+#undef VCT_TILE_M
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
 // Groups with a workspace and a blocked form of the direction run in two
 // launches (stage 1 into the workspace, then stage 2); the others gather
 // directly in the first launch.
@@ -1082,6 +1416,7 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
     GatherArgs a{}, b{};
     a.rows_per_item = b.rows_per_item = 256 / (lanes == 1 ? kSplit : lanes);
     int items_a = 0, items_b = 0;
+    int tiles[kMaxGroups], n_tiles = 0;
     for (int k = 0; k < n_groups; k++) {
         const vct_batch_group& G = groups[k];
         if (!G.sysmat || !G.d_in || !G.d_out) return set_error(VCT_ERR_ARG, "group %d has NULL pointers", k);
@@ -1092,15 +1427,20 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
         const vct_blocked& f = adjoint ? m->adj : m->fwd;
         const int rows = adjoint ? m->NN : m->nrays;
         if (G.d_work && f.rows1) {
-            add_group(a, items_a, f.ptr1, f.ent1, (int)f.rows1, G.d_in, G.d_work, G.n_phantoms, 1.f);
+            if (!adjoint && m->tile.ell) {
+                tiles[n_tiles++] = k;  // stage 1 by k_gather_tile
+            } else {
+                add_group(a, items_a, f.ptr1, f.ent1, (int)f.rows1, G.d_in, G.d_work, G.n_phantoms, 1.f);
+                a.chunked = 1;
+            }
             add_group(b, items_b, f.ptr2, f.ent2, rows, G.d_work, G.d_out, G.n_phantoms, m->scale);
-            a.chunked = 1;
         } else {
             add_group(a, items_a, adjoint ? m->col_ptr : m->row_ptr, adjoint ? m->csc : m->csr, rows, G.d_in, G.d_out,
                       G.n_phantoms, m->scale);
         }
     }
     hipStream_t s = (hipStream_t)stream;
+    for (int q = 0; q < n_tiles; q++) VCT_CHECK(run_gather_tile(groups[tiles[q]], M, s));
     VCT_CHECK(run_gather(a, items_a, M, lanes, blocks, s));
     return run_gather(b, items_b, M, lanes, blocks, s);
 }
