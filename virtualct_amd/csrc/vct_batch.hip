@@ -250,6 +250,12 @@ __global__ __launch_bounds__(256) void k_stage2_fill(const uint32_t* __restrict_
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
+__global__ __launch_bounds__(256) void k_set_u32_n(uint32_t* p, int n, uint32_t v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ------------------------------------------------------------ gather SpMM
 
 struct GatherGroup {
@@ -788,20 +794,34 @@ constexpr int kTileThreads = 1024;  // one workgroup per CU (LDS)
 constexpr int kTileUnroll = 8;      // entries in flight per lane
 
 // Build: per stage-1 run, sort key tile << 16 | (0xffff - length) (longest
-// first within a tile) and the tile's run count.
+// first within a tile), and the first run of each tile (the runs are in tile
+// order, so each tile has one first run; no counter is contended).
 __global__ __launch_bounds__(256) void k_tile_run_keys(const uint32_t* __restrict__ ptr1, const uint2* __restrict__ ent1,
                                                        uint32_t rows1, TileGrid tg, uint32_t* __restrict__ key,
-                                                       uint32_t* __restrict__ val, uint32_t* __restrict__ tile_cnt,
+                                                       uint32_t* __restrict__ val, uint32_t* __restrict__ tile_first,
                                                        uint32_t* __restrict__ err)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows1) return;
     const uint32_t len = ptr1[r + 1] - ptr1[r];
     const uint32_t t = tg.tile(ent1[ptr1[r]].x);
-    if (len == 0u || len >= 0xffffu) atomicOr(err, 1u);
+    const uint32_t tp = r ? tg.tile(ent1[ptr1[r - 1]].x) : 0xffffffffu;
+    if (len == 0u || len >= 0xffffu || (r && t < tp)) atomicOr(err, 1u);
     key[r] = (t << 16) | (0xffffu - min(len, 0xfffeu));
     val[r] = r;
-    atomicAdd(tile_cnt + t, 1u);
+    if (t != tp) tile_first[t] = r;
+}
+
+// Build: run counts from first runs (0xffffffff: a tile without runs), in
+// place; one thread.
+__global__ void k_first_to_count(uint32_t* __restrict__ c, int ntiles, uint32_t rows1)
+{
+    uint32_t next = rows1;
+    for (int t = ntiles - 1; t >= 0; t--) {
+        const uint32_t f = c[t];
+        c[t] = f == 0xffffffffu ? 0u : next - f;
+        if (f != 0xffffffffu) next = f;
+    }
 }
 
 // Build: entries per lane of each slice = the length of its first (longest) run.
@@ -1079,9 +1099,10 @@ static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tile
     hip(hipMalloc(&cnt, 4 * (size_t)tg.ntiles + 16), "hipMalloc tile counts");
     hip(hipMalloc(&err, 16), "hipMalloc tile err");
     if (st == VCT_OK) {
-        hipLaunchKernelGGL(k_zero_f32, dim3((tg.ntiles + 256) / 256), blk, 0, s, (float*)cnt, tg.ntiles + 1);
+        hipLaunchKernelGGL(k_set_u32_n, dim3((tg.ntiles + 255) / 256), blk, 0, s, cnt, tg.ntiles, 0xffffffffu);
         hipLaunchKernelGGL(k_zero_f32, dim3(1), blk, 0, s, (float*)err, 1);
         hipLaunchKernelGGL(k_tile_run_keys, gr, blk, 0, s, f.ptr1, f.ent1, R, tg, key, val, cnt, err);
+        hipLaunchKernelGGL(k_first_to_count, dim3(1), dim3(1), 0, s, cnt, tg.ntiles, R);
         hip(hipGetLastError(), "k_tile_run_keys");
     }
     hip(rocprim::radix_sort_pairs(nullptr, tb, key, key_s, val, order, (size_t)R, 0u, 32u, s), "sort size");
