@@ -28,6 +28,14 @@
 // kAdjViews views that cross one tile) stays in the L2 of the XCD that owns
 // it; stage 2 sums each row's partials in block order.  Both stages are
 // ordinary k_gather launches and the result is deterministic.
+//
+// Tile-resident forward stage 1 (built with the blocked form unless
+// VCT_SYSMAT_NO_TILE).  The runs of each forward tile, sorted by length into
+// slices of 64 with interleaved entries, are summed by k_gather_tile, one
+// workgroup per (tile, phantom) with the tile of that phantom in LDS: the
+// vector memory path then carries only the 8-byte entries, read once from
+// HBM by the B workgroups of a tile on one XCD.  Same runs, same order of
+// terms: the partial sums are those of the k_gather stage 1, bit for bit.
 #include <algorithm>
 #include <cstring>
 #include <vector>
