@@ -414,12 +414,13 @@ def test_bb_step_matches_fp64_formula(gpu, B):
         assert torch.equal(batch.from_planes(x2), ref)
 
 
-@pytest.mark.parametrize("N,pix,B,M", [(128, 0.35, 3, 6), (96, 0.45, 16, 5), (160, 0.28, 2, 8), (128, 0.35, 1, 3)])
+@pytest.mark.parametrize("N,pix,B,M", [(128, 0.35, 3, 6), (96, 0.45, 16, 5), (160, 0.28, 2, 8), (128, 0.35, 1, 3),
+                                     (64, 0.7, 64, 2), (65, 0.69, 5, 1)])
 def test_tile_stage1_equals_blocked_stage1(gpu, N, pix, B, M):
     """The forward stage 1 over the tile-resident form (the tile of each
-    phantom in LDS, runs sorted by length in 64-run slices; partial edge
-    tiles at 96^2 and 160^2) gives the bits of the blocked stage 1 read
-    through L2, for 1..16 phantoms and 3..8 materials."""
+    phantom in LDS, runs sorted by length in 64-run slices; one tile at 64^2,
+    partial edge tiles at 65^2, 96^2 and 160^2) gives the bits of the blocked
+    stage 1 read through L2, for 1..64 phantoms and 1..8 materials."""
     import torch
     ops, batch, _ = _mods()
     plan = ops.plan_for(_geom(N, pix, 160))
