@@ -225,8 +225,8 @@ def main():
                     help="run one device-to-device copy of MB MiB first (calibrates PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (rays and views would shard "
+              "across GPUs, one process each); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
