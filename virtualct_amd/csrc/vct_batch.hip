@@ -1003,9 +1003,9 @@ static vct_status build_blocked(const uint2* ent, const uint32_t* row_of, uint32
     hip(hipMalloc(&order, 4 * nz + 16), "hipMalloc order");
     if (st == VCT_OK) hipLaunchKernelGGL(k_iota, g1, blk, 0, s, vals, n);
     const unsigned kb = key_bits(n_keys);
-    hip(rocprim::radix_sort_pairs(nullptr, tb, key, key_s, vals, order, nz, 0u, kb, s), "sort size");
+    if (st == VCT_OK) hip(rocprim::radix_sort_pairs(nullptr, tb, key, key_s, vals, order, nz, 0u, kb, s), "sort size");
     hip(hipMalloc(&tmp, tb + 16), "hipMalloc sort");
-    hip(rocprim::radix_sort_pairs(tmp, tb, key, key_s, vals, order, nz, 0u, kb, s), "sort");
+    if (st == VCT_OK) hip(rocprim::radix_sort_pairs(tmp, tb, key, key_s, vals, order, nz, 0u, kb, s), "sort");
     if (tmp) { (void)hipStreamSynchronize(s); (void)hipFree(tmp); tmp = nullptr; }
     if (vals) { (void)hipFree(vals); vals = nullptr; }
     // 2. run heads and their exclusive scan = stage-1 row of each run
@@ -1013,9 +1013,9 @@ static vct_status build_blocked(const uint2* ent, const uint32_t* row_of, uint32
     hip(hipMalloc(&run, 4 * nz + 16), "hipMalloc run");
     if (st == VCT_OK) hipLaunchKernelGGL(k_run_heads, g1p, blk, 0, s, key_s, order, row_of, n, head);
     tb = 0;
-    hip(rocprim::exclusive_scan(nullptr, tb, head, run, 0u, nz + 1, rocprim::plus<uint32_t>(), s), "scan size");
+    if (st == VCT_OK) hip(rocprim::exclusive_scan(nullptr, tb, head, run, 0u, nz + 1, rocprim::plus<uint32_t>(), s), "scan size");
     hip(hipMalloc(&tmp, tb + 16), "hipMalloc scan");
-    hip(rocprim::exclusive_scan(tmp, tb, head, run, 0u, nz + 1, rocprim::plus<uint32_t>(), s), "scan");
+    if (st == VCT_OK) hip(rocprim::exclusive_scan(tmp, tb, head, run, 0u, nz + 1, rocprim::plus<uint32_t>(), s), "scan");
     uint32_t rows1 = 0;
     if (st == VCT_OK) hip(hipMemcpyAsync(&rows1, run + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "copy");
     if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
@@ -1050,15 +1050,15 @@ static vct_status build_blocked(const uint2* ent, const uint32_t* row_of, uint32
     }
     tb = 0;
     const unsigned rb = key_bits((uint64_t)rows);
-    hip(rocprim::radix_sort_pairs(nullptr, tb, row1, row1_s, iota2, order2, (size_t)rows1, 0u, rb, s), "sort size");
+    if (st == VCT_OK) hip(rocprim::radix_sort_pairs(nullptr, tb, row1, row1_s, iota2, order2, (size_t)rows1, 0u, rb, s), "sort size");
     hip(hipMalloc(&tmp, tb + 16), "hipMalloc sort");
-    hip(rocprim::radix_sort_pairs(tmp, tb, row1, row1_s, iota2, order2, (size_t)rows1, 0u, rb, s), "sort");
+    if (st == VCT_OK) hip(rocprim::radix_sort_pairs(tmp, tb, row1, row1_s, iota2, order2, (size_t)rows1, 0u, rb, s), "sort");
     if (tmp) { (void)hipStreamSynchronize(s); (void)hipFree(tmp); tmp = nullptr; }
     tb = 0;
-    hip(rocprim::exclusive_scan(nullptr, tb, cnt, f.ptr2, 0u, (size_t)rows + 1, rocprim::plus<uint32_t>(), s),
+    if (st == VCT_OK) hip(rocprim::exclusive_scan(nullptr, tb, cnt, f.ptr2, 0u, (size_t)rows + 1, rocprim::plus<uint32_t>(), s),
         "scan size");
     hip(hipMalloc(&tmp, tb + 16), "hipMalloc scan");
-    hip(rocprim::exclusive_scan(tmp, tb, cnt, f.ptr2, 0u, (size_t)rows + 1, rocprim::plus<uint32_t>(), s), "scan");
+    if (st == VCT_OK) hip(rocprim::exclusive_scan(tmp, tb, cnt, f.ptr2, 0u, (size_t)rows + 1, rocprim::plus<uint32_t>(), s), "scan");
     if (st == VCT_OK && rows1) {
         hipLaunchKernelGGL(k_stage2_fill, g2, blk, 0, s, order2, (int64_t)rows1, f.ent2);
         hip(hipGetLastError(), "k_stage2_fill");
@@ -1113,9 +1113,9 @@ static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tile
         hipLaunchKernelGGL(k_first_to_count, dim3(1), dim3(1), 0, s, cnt, tg.ntiles, R);
         hip(hipGetLastError(), "k_tile_run_keys");
     }
-    hip(rocprim::radix_sort_pairs(nullptr, tb, key, key_s, val, order, (size_t)R, 0u, 32u, s), "sort size");
+    if (st == VCT_OK) hip(rocprim::radix_sort_pairs(nullptr, tb, key, key_s, val, order, (size_t)R, 0u, 32u, s), "sort size");
     hip(hipMalloc(&tmp, tb + 16), "hipMalloc sort");
-    hip(rocprim::radix_sort_pairs(tmp, tb, key, key_s, val, order, (size_t)R, 0u, 32u, s), "sort");
+    if (st == VCT_OK) hip(rocprim::radix_sort_pairs(tmp, tb, key, key_s, val, order, (size_t)R, 0u, 32u, s), "sort");
     // host: tile begins (in sorted order), slices per tile, slice -> tile
     std::vector<uint32_t> n_t(tg.ntiles);
     uint32_t herr = 0;
@@ -1250,10 +1250,10 @@ vct_status vct_sysmat_build(vct_plan* p, int flags, vct_sysmat** out, void* stre
         hip(hipGetLastError(), "k_sm_count");
         hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(1), 0, s, (float*)(counts + nrays), 1);
         size_t tb = 0;
-        hip(rocprim::exclusive_scan(nullptr, tb, counts, m->row_ptr, 0u, (size_t)nrays + 1, rocprim::plus<uint32_t>(), s),
+        if (st == VCT_OK) hip(rocprim::exclusive_scan(nullptr, tb, counts, m->row_ptr, 0u, (size_t)nrays + 1, rocprim::plus<uint32_t>(), s),
             "scan size");
         hip(hipMalloc(&tmp, tb + 16), "hipMalloc scan");
-        hip(rocprim::exclusive_scan(tmp, tb, counts, m->row_ptr, 0u, (size_t)nrays + 1, rocprim::plus<uint32_t>(), s),
+        if (st == VCT_OK) hip(rocprim::exclusive_scan(tmp, tb, counts, m->row_ptr, 0u, (size_t)nrays + 1, rocprim::plus<uint32_t>(), s),
             "scan");
     }
     uint32_t total = 0;
@@ -1282,16 +1282,16 @@ vct_status vct_sysmat_build(vct_plan* p, int flags, vct_sysmat** out, void* stre
         unsigned end_bit = 1;
         while ((1u << end_bit) < (unsigned)NN) end_bit++;
         size_t tb = 0;
-        hip(rocprim::radix_sort_pairs(nullptr, tb, keys, keys2, vals, vals2, (size_t)total, 0u, end_bit, s), "sort size");
+        if (st == VCT_OK) hip(rocprim::radix_sort_pairs(nullptr, tb, keys, keys2, vals, vals2, (size_t)total, 0u, end_bit, s), "sort size");
         hip(hipMalloc(&tmp, tb + 16), "hipMalloc sort");
-        hip(rocprim::radix_sort_pairs(tmp, tb, keys, keys2, vals, vals2, (size_t)total, 0u, end_bit, s), "sort");
+        if (st == VCT_OK) hip(rocprim::radix_sort_pairs(tmp, tb, keys, keys2, vals, vals2, (size_t)total, 0u, end_bit, s), "sort");
         if (tmp) { (void)hipStreamSynchronize(s); (void)hipFree(tmp); tmp = nullptr; }
         hip(hipMalloc(&m->col_ptr, sizeof(uint32_t) * (NN + 1)), "hipMalloc col_ptr");
         tb = 0;
-        hip(rocprim::exclusive_scan(nullptr, tb, colcnt, m->col_ptr, 0u, (size_t)NN + 1, rocprim::plus<uint32_t>(), s),
+        if (st == VCT_OK) hip(rocprim::exclusive_scan(nullptr, tb, colcnt, m->col_ptr, 0u, (size_t)NN + 1, rocprim::plus<uint32_t>(), s),
             "scan size");
         hip(hipMalloc(&tmp, tb + 16), "hipMalloc scan");
-        hip(rocprim::exclusive_scan(tmp, tb, colcnt, m->col_ptr, 0u, (size_t)NN + 1, rocprim::plus<uint32_t>(), s), "scan");
+        if (st == VCT_OK) hip(rocprim::exclusive_scan(tmp, tb, colcnt, m->col_ptr, 0u, (size_t)NN + 1, rocprim::plus<uint32_t>(), s), "scan");
         hip(hipMalloc(&m->csc, sizeof(uint2) * (size_t)total + 8), "hipMalloc csc");
         if (st == VCT_OK) {
             hipLaunchKernelGGL(k_csc_gather, dim3((unsigned)(((int64_t)total + 255) / 256)), dim3(256), 0, s, vals2, eray,
