@@ -8,6 +8,7 @@ noise (sigma 5.37e-5, Philox), the MFMA Shepp-Logan filter and the
 backprojector.  value = rays x energy bins per second over the whole step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py --gathers-only     (the batched gather workload alone, for profilers)
 
 Prints one JSON line.  Per-stage times come from HIP events recorded on the
 launch stream inside the timed region.
@@ -168,6 +169,35 @@ def bench_decomposition(B=16, iters=50, plain_iters=1):
     }
 
 
+def run_gathers(pairs=3, B=16):
+    """The batched gather workload alone (for profilers): 16 body phantoms at
+    512^2 (seeds 5000-5015), the blocked system matrix of the 800 x 609
+    geometry, then `pairs` forward + adjoint gathers with 16 lanes per row on
+    1024 persistent workgroups.  Prints one line per finished pair."""
+    import torch
+
+    from virtualct_amd import batch, default_geometry, ops, phantoms
+
+    dev = torch.device("cuda", 0)
+    plan = ops.plan_for(default_geometry())
+    x = batch.interleave(torch.stack([torch.from_numpy(phantoms.densities_f32(phantoms.body_phantom(5000 + b)))
+                                      for b in range(B)]).to(dev))
+    rays = plan.V * plan.W
+    L = torch.empty((rays, B, 8), device=dev)
+    g = torch.Generator(device=dev).manual_seed(5)
+    wn = torch.zeros((rays, B, 8), device=dev)
+    wn[..., :6] = torch.randn((rays, B, 6), generator=g, device=dev)
+    w = batch.to_planes(wn)
+    grad = torch.empty_like(x)
+    sm = batch.SystemMatrix(plan, transpose=True, blocked=True)
+    for k in range(pairs):
+        batch.batch_paths([(sm, x, L, B)], 6, 16, 1024)
+        batch.batch_adjoint([(sm, w, grad, B)], 6, 16, 1024)
+        torch.cuda.synchronize()
+        print(f"gathers: pair {k + 1} of {pairs} done", flush=True)
+    sm.close()
+
+
 def bench_batch_fbp(per_size=16):
     """64 phantoms of uneven size x 3 spectra: one graph replay vs a loop of
     plain per-phantom GPU calls (forward with 3 spectra + FBP)."""
@@ -221,6 +251,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-batched", action="store_true", help="skip the decomposition steps/s measurement")
     ap.add_argument("--batch-fbp", action="store_true", help="also time 64 uneven phantoms x 3 spectra")
+    ap.add_argument("--gathers-only", action="store_true",
+                    help="run only the batched gather workload (16 phantoms, blocked matrix, 3 forward + adjoint "
+                         "pairs) for a profiler; prints no JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="run one device-to-device copy of MB MiB first (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -228,6 +261,9 @@ def main():
         print("bench.py measures one GPU only: there is no multi-GPU path (rays and views would shard "
               "across GPUs, one process each); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.gathers_only:
+        run_gathers()
+        return
 
     import numpy as np
     import torch
