@@ -14,8 +14,9 @@
 //                  load per four MFMAs; the two halves meet in LDS.
 //   k_backproject  pixel-driven fan backprojection
 //                  (F/reconstructImageTOMO.cpp:46-100): a 16x16 pixel tile per
-//                  workgroup, two lanes per pixel, the view cos/sin in LDS, FP64
-//                  geometry and the reference's float accumulation order.
+//                  workgroup, one lane per pixel, the view loop uniform over
+//                  the wave, FP64 geometry and the reference's float
+//                  accumulation order.
 #include "vct_internal.h"
 
 #include <climits>
@@ -26,7 +27,6 @@ namespace vct {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBpTile = 16;   // pixel tile edge
-constexpr int kBpViews = 16;  // views staged in LDS per chunk
 
 __device__ __forceinline__ int trunc_i32_bp(double v)
 {
@@ -145,30 +145,21 @@ __device__ __forceinline__ double rcp_f64(double x)
     return __builtin_fma(r, e, r);
 }
 
-// Two lanes per pixel (the even and the odd lane of a pair), each forming
-// half of each group of 16 views' terms: 512 threads for a 16x16 tile, so a
-// 512x512 image gives 8 waves per SIMD to hide the FP64 and memory latencies.
-// The float accumulation keeps the reference's view order exactly: the even
-// lane adds views 0..7 of the group to the running sum, hands it to the odd
-// lane, which adds views 8..15, and the result returns to both lanes.  The
-// filtered sinogram (1.95 MB at 800 x 609) is read through L2 without
-// staging; the view cos/sin go to LDS once per block of kBpTrig views.
+// One lane per pixel, a 16x16 pixel tile per workgroup.  The view loop is the
+// same on every lane of a wave, so each view's cos/sin and the base of its
+// filtered-sinogram row are scalar loads, and four views are unrolled so that
+// their geometry and gathers overlap.  The float accumulation adds the views
+// in the reference's order (F/reconstructImageTOMO.cpp:57-95); the filtered
+// sinogram (1.95 MB at 800 x 609) is read through L2 without staging.
 //
 // WIDE: some masked pixel may see |tan(gamma)| > 0.5, beyond the polynomial;
 // those take the library atan.  The host picks the variant from the bound
 // |tan(gamma)| <= r / sqrt(sid^2 - r^2) over the mask radius r; without the
 // fallback the view loop has no branch.
-constexpr int kBpHalf = kBpViews / 2;
-constexpr int kBpTrig = 2048;  // views whose cos/sin sit in LDS at a time (32 KB)
-
 template <bool WIDE>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 8))) void k_backproject(BackprojArgs a)
+__global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject(BackprojArgs a)
 {
-    __shared__ double trig[2 * kBpTrig];  // [view][cos, sin]
-    const int half = threadIdx.x & 1;
-    const int pix = threadIdx.x >> 1;
-    const int lane = threadIdx.x & 63;
-    const int tx = pix & (kBpTile - 1), ty = pix / kBpTile;
+    const int tx = threadIdx.x & (kBpTile - 1), ty = threadIdx.x / kBpTile;
     const int x = blockIdx.x * kBpTile + tx;
     const int y = blockIdx.y * kBpTile + ty;
     const int s = blockIdx.z;
@@ -180,58 +171,29 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : 
     const double xx = -(x - a.center) * a.pix;  // :52-53
     const double yy = -(y - a.center) * a.pix;
     float acc = 0.f;
-    for (int g0 = 0; g0 < a.use_views; g0 += kBpViews) {
-        const int gt = g0 % kBpTrig;  // groups never straddle a cos/sin block
-        if (gt == 0) {
-            __syncthreads();
-            const int nb = min(kBpTrig, a.use_views - g0);
-            for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-                trig[2 * i] = a.cosT[g0 + i];
-                trig[2 * i + 1] = a.sinT[g0 + i];
-            }
-            __syncthreads();
+    if (masked) {
+#pragma unroll 4
+        for (int v = 0; v < a.use_views; v++) {
+            const double c = a.cosT[v], sn = a.sinT[v];
+            const double X = xx * c + yy * sn;
+            const double Y = -xx * sn + yy * c;
+            const double den = a.sid - Y;
+            const double L2 = den * den + X * X;
+            const double tg = X * rcp_f64(den);
+            double gam = atan_small(tg);
+            if (WIDE && fabs(tg) > 0.5) gam = atan(X / den);
+            const double fGAM = gam * a.inv_dbeta_ch + a.cent;
+            const int GAM = trunc_i32_bp(fGAM);
+            const double w = fGAM - GAM;
+            const bool ok = GAM >= 0 && GAM < a.W - 1;
+            const float* row = fl + (size_t)v * a.W + (ok ? GAM : 0);
+            const double sum1 = (1 - w) * row[0] + w * row[1];
+            const double sum = sum1 + 0.0;
+            const float term = (float)(rcp_f64(L2) * sum * a.dbeta_view * a.xfactor);
+            acc = ok ? acc + term : acc;
         }
-        const int nv = min(kBpViews, a.use_views - g0);
-        float term[kBpHalf];
-        bool ok[kBpHalf];
-#pragma unroll
-        for (int v = 0; v < kBpHalf; v++) {
-            term[v] = 0.f;
-            ok[v] = false;
-        }
-        if (masked) {
-#pragma unroll
-            for (int v = 0; v < kBpHalf; v++) {
-                const int u = half * kBpHalf + v;
-                const int tu = u < nv ? u : nv - 1;  // a clamped view is computed, not added
-                const double c = trig[2 * (gt + tu)], sn = trig[2 * (gt + tu) + 1];
-                const double X = xx * c + yy * sn;
-                const double Y = -xx * sn + yy * c;
-                const double den = a.sid - Y;
-                const double L2 = den * den + X * X;
-                const double tg = X * rcp_f64(den);
-                double gam = atan_small(tg);
-                if (WIDE && fabs(tg) > 0.5) gam = atan(X / den);
-                const double fGAM = gam * a.inv_dbeta_ch + a.cent;
-                const int GAM = trunc_i32_bp(fGAM);
-                const double w = fGAM - GAM;
-                ok[v] = u < nv && GAM >= 0 && GAM < a.W - 1;
-                const float* row = fl + (size_t)(g0 + tu) * a.W + (ok[v] ? GAM : 0);
-                const double sum1 = (1 - w) * row[0] + w * row[1];
-                const double sum = sum1 + 0.0;
-                term[v] = (float)(rcp_f64(L2) * sum * a.dbeta_view * a.xfactor);
-            }
-        }
-        // ordered merge, all lanes converged: the even lane's views, then the odd lane's
-        float se = acc;
-#pragma unroll
-        for (int v = 0; v < kBpHalf; v++) se = ok[v] ? se + term[v] : se;
-        float so = __shfl(se, lane & ~1);
-#pragma unroll
-        for (int v = 0; v < kBpHalf; v++) so = ok[v] ? so + term[v] : so;
-        acc = __shfl(so, lane | 1);
     }
-    if (in_img && half == 0) a.img[((size_t)s * a.N + y) * a.N + x] = acc;
+    if (in_img) a.img[((size_t)s * a.N + y) * a.N + x] = acc;
 }
 
 // The beam-hardening tool M/beam_hardening_correction.cpp:47-60 elementwise:
@@ -275,9 +237,9 @@ vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s)
     const double r = a.mask_r * a.pix;
     const bool wide = !(r < a.sid) || r / std::sqrt(a.sid * a.sid - r * r) > 0.49;
     if (wide)
-        hipLaunchKernelGGL(k_backproject<true>, grid, dim3(2 * kBpTile * kBpTile), 0, s, a);
+        hipLaunchKernelGGL(k_backproject<true>, grid, dim3(kBpTile * kBpTile), 0, s, a);
     else
-        hipLaunchKernelGGL(k_backproject<false>, grid, dim3(2 * kBpTile * kBpTile), 0, s, a);
+        hipLaunchKernelGGL(k_backproject<false>, grid, dim3(kBpTile * kBpTile), 0, s, a);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
