@@ -216,7 +216,7 @@ class BatchedSolver:
         self.bb_work = torch.empty(nb // 8, dtype=torch.float64, device=dev)
         self.have_prev = False
         self.sm.workspace(self.B)  # allocated here, never under graph capture
-        key = ("gather", self.N, self.B, self.M, "blocked" if self.sm.blocked else "direct")
+        key = ("gather fwd+adj", self.N, self.B, self.M, "blocked" if self.sm.blocked else "direct")
         cfg = tuner().best(key, lambda: self._candidates(), self._time_config) if lanes is None else \
             {"lanes": lanes, "blocks": blocks}
         self.lanes, self.blocks = cfg["lanes"], cfg["blocks"]
@@ -233,12 +233,19 @@ class BatchedSolver:
         return out
 
     def _time_config(self, cfg):
-        items = [(self.sm, self.x, self.L, self.B)]
-        batch_paths(items, self.M, cfg["lanes"], cfg["blocks"])
+        # a step runs both gathers with this shape (the tile-resident forward
+        # stage 1 ignores it), so both are timed
+        fwd = [(self.sm, self.x, self.L, self.B)]
+        adj = [(self.sm, self.w, self.g, self.B)]
+        self.w.zero_()
+        def pair():
+            batch_paths(fwd, self.M, cfg["lanes"], cfg["blocks"])
+            batch_adjoint(adj, self.M, cfg["lanes"], cfg["blocks"])
+        pair()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(3):
-            batch_paths(items, self.M, cfg["lanes"], cfg["blocks"])
+            pair()
         e1.record()
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 3
