@@ -29,12 +29,12 @@ HBM_PEAK_GBS = 8000.0
 SEGMENTS_512 = 282_824_404  # sum of per-ray segments at 512^2, 0.1 cm, 800x609
 # HBM bytes per tracer launch on this workload, from rocprofv3 --pmc passes
 # over `bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-batched
-# --copy-probe 256`: FETCH_SIZE 27447.0 KiB x 2.00 + WRITE_SIZE 15225.0 KiB
+# --copy-probe 256`: FETCH_SIZE 27535.3 KiB x 2.00 + WRITE_SIZE 15225.0 KiB
 # x 1.00, the factors being what the same counters read for the 256 MiB
 # device copy of the same run (131087.8 KiB fetched, 262144 KiB written).
 # The densities (4 MB) stay in L2/MALL: each XCD fetches them about twice,
 # and the write is the 15.6 MB of path integrals.
-TRACE_TRAFFIC_BYTES = int((27447.0 * 2.0 + 15225.0 * 1.0) * 1024)
+TRACE_TRAFFIC_BYTES = int((27535.3 * 2.0 + 15225.0 * 1.0) * 1024)
 METRIC = "MBFPA ray·energy-bins/s, FBP voxel·views/s; batched grad steps/s vs plain"
 
 
