@@ -275,6 +275,7 @@ struct GatherGroup {
     int rows;
     int item_begin;       // first work item of this group
     float scale;          // applied to every output of the group (1/xfactor)
+    int partial;          // 1: out is a stage-1 workspace, read back for M materials only
 };
 
 struct GatherArgs {
@@ -400,6 +401,17 @@ __device__ __forceinline__ void load_slot(const float* __restrict__ in, size_t r
     }
 }
 
+// The planes of (row, b) that load_slot<M> reads (stage-1 partial sums: the
+// zero planes of materials >= M are never read back, so not written).
+template <int M>
+__device__ __forceinline__ void store_slot_m(float* __restrict__ out, size_t row, int B, int b, const float (&x)[8])
+{
+    float* r = out + row * (size_t)(8 * B);
+    *reinterpret_cast<float4*>(r + 4 * b) = make_float4(x[0], x[1], x[2], x[3]);
+    if constexpr (M > 4) *reinterpret_cast<float2*>(r + 4 * B + 2 * b) = make_float2(x[4], x[5]);
+    if constexpr (M > 6) *reinterpret_cast<float2*>(r + 6 * B + 2 * b) = make_float2(x[6], x[7]);
+}
+
 // All 8 values of (row, b), padding included.
 __device__ __forceinline__ void store_slot(float* __restrict__ out, size_t row, int B, int b, const float (&x)[8])
 {
@@ -498,7 +510,10 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a)
         float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int m = 0; m < M; m++) o[m] = (float)(acc[m] * G.scale);
-        store_slot(G.out, row, G.B, b, o);
+        if (G.partial)
+            store_slot_m<M>(G.out, row, G.B, b, o);
+        else
+            store_slot(G.out, row, G.B, b, o);
     }
 }
 
@@ -953,7 +968,7 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_tile(TileArgs a)
             float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int m = 0; m < M; m++) o[m] = (float)acc[m];
-            store_slot(a.out, (size_t)run, a.B, b, o);
+            store_slot_m<M>(a.out, (size_t)run, a.B, b, o);
         }
     }
 }
@@ -1380,9 +1395,10 @@ size_t vct_sysmat_workspace(const vct_sysmat* m, int n_phantoms)
 constexpr int kSplit = 16;
 
 static void add_group(GatherArgs& a, int& items, const uint32_t* ptr, const uint2* ent, int rows, const float* in,
-                      float* out, int B, float scale)
+                      float* out, int B, float scale, int partial = 0)
 {
     GatherGroup& d = a.g[a.n_groups++];
+    d.partial = partial;
     d.ptr = ptr;
     d.ent = ent;
     d.rows = rows;
@@ -1459,7 +1475,7 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
             if (!adjoint && m->tile.ell) {
                 tiles[n_tiles++] = k;  // stage 1 by k_gather_tile
             } else {
-                add_group(a, items_a, f.ptr1, f.ent1, (int)f.rows1, G.d_in, G.d_work, G.n_phantoms, 1.f);
+                add_group(a, items_a, f.ptr1, f.ent1, (int)f.rows1, G.d_in, G.d_work, G.n_phantoms, 1.f, 1);
                 a.chunked = 1;
             }
             add_group(b, items_b, f.ptr2, f.ent2, rows, G.d_work, G.d_out, G.n_phantoms, m->scale);
