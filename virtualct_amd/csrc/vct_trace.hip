@@ -110,8 +110,9 @@ struct SinkF32 {
         const float r[8] = {q.lo.x, q.lo.y, q.lo.z, q.lo.w, q.hi.x, q.hi.y, q.hi.z, q.hi.w};
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            // a negative density contributes nothing (reference ph = 0, :155)
-            const double rho = r[m] > 0.f ? (double)r[m] : 0.0;
+            // a negative density contributes nothing (reference ph = 0, :155):
+            // fmaxf gives 0 for negatives and NaN, and -0 adds the same zero
+            const double rho = (double)fmaxf(r[m], 0.f);
             acc[m] = __builtin_fma(rho, rr, acc[m]);
         }
     }
