@@ -105,6 +105,27 @@ def test_batched_fbp_equals_single(gpu, golden):
         assert torch.equal(many[i], one)
 
 
+@pytest.mark.parametrize("N,pix,V,S", [(512, 0.1, 800, 5), (64, 0.7, 90, 2), (64, 0.7, 90, 3),
+                                        (96, 0.5, 120, 10), (64, 0.7, 90, 17), (128, 0.6, 160, 6)])
+def test_multi_sinogram_backprojection_equals_single(gpu, oracle, N, pix, V, S):
+    """S sinograms of one geometry share each lane's pixel-view geometry (up to
+    8 per lane, in even slices: 10 -> 5 + 5, 17 -> 6 + 6 + 6 with the last
+    slice one short); every image equals the single-sinogram backprojection
+    bit for bit.  128 x 0.6 cm is the wide field (library atan fallback)."""
+    import torch
+    ops = _ops()
+    rng = np.random.default_rng(40 + S)
+    fl = rng.normal(size=(S, V, 609)).astype(np.float32) * 1e3
+    plan = ops.plan_for(_geom(N, pix, V))
+    many = ops.backproject(plan, _t(fl, gpu))
+    for i in range(S):
+        one = ops.backproject(plan, _t(fl[i], gpu))[0]
+        assert torch.equal(many[i], one), i
+    if N <= 128:
+        ref = oracle.backproject_ref(Geom(N=N, pix=pix, V=V), fl[S - 1])
+        assert rel_max(many[S - 1].cpu().numpy(), ref) < 1e-5
+
+
 def test_mbfpa_then_fbp_recovers_mu_map(gpu, golden):
     """The pipeline MBFPA -> FBP gives a mu map in 1/cm in the same
     orientation as the phantom (correlation with sum rho_m mu_m(60 keV))."""
@@ -174,7 +195,7 @@ def test_wide_field_backprojection_takes_library_atan(gpu, oracle):
 
 
 def test_backprojection_more_views_than_one_trig_block(gpu, oracle):
-    """2100 views: the view cos/sin are staged in blocks of 2048 views."""
+    """2100 views: a view loop longer than 2048 views."""
     ops = _ops()
     N, pix, V = 64, 0.7, 2100
     g = Geom(N=N, pix=pix, V=V)

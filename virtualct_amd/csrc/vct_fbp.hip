@@ -27,6 +27,7 @@ namespace vct {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBpTile = 16;   // pixel tile edge
+constexpr int kBpSinos = 8;  // most sinograms per lane in k_backproject_multi
 
 __device__ __forceinline__ int trunc_i32_bp(double v)
 {
@@ -196,6 +197,68 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject(BackprojArgs 
     if (in_img) a.img[((size_t)s * a.N + y) * a.N + x] = acc;
 }
 
+// k_backproject for K >= 2 sinograms of the same geometry per lane.  The pixel-view geometry (X,
+// Y, L^2, gamma, the detector bin and its weights) does not depend on the
+// sinogram, so a lane computes it once and applies it to K sinograms, each
+// with the single-sinogram expression; the images are bit-identical for
+// every K.  Workgroup z covers sinograms K*z ... K*z + K - 1; past S the
+// lane re-reads sinogram S - 1 and stores nothing.  The view loop is not
+// unrolled: the K gathers of one view already overlap, and unrolling 2 or 4
+// views measured 8-17% slower.
+template <bool WIDE, int K>
+__global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_multi(BackprojArgs a, int S)
+{
+    const int tx = threadIdx.x & (kBpTile - 1), ty = threadIdx.x / kBpTile;
+    const int x = blockIdx.x * kBpTile + tx;
+    const int y = blockIdx.y * kBpTile + ty;
+    const int s0 = blockIdx.z * K;
+    const float* fl[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) fl[k] = a.fltd + (size_t)min(s0 + k, S - 1) * a.V * a.W;
+    const bool in_img = x < a.N && y < a.N;
+    // prepareMaskData, F/misc.cpp:20-36
+    const double dyc = (double)y - a.mask_r, dxc = (double)x - a.mask_r;
+    const bool masked = in_img && (dyc * dyc + dxc * dxc <= a.mask_r2);
+    const double xx = -(x - a.center) * a.pix;  // :52-53
+    const double yy = -(y - a.center) * a.pix;
+    float acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0.f;
+    if (masked) {
+#pragma unroll 1
+        for (int v = 0; v < a.use_views; v++) {
+            const double c = a.cosT[v], sn = a.sinT[v];
+            const double X = xx * c + yy * sn;
+            const double Y = -xx * sn + yy * c;
+            const double den = a.sid - Y;
+            const double L2 = den * den + X * X;
+            const double tg = X * rcp_f64(den);
+            double gam = atan_small(tg);
+            if (WIDE && fabs(tg) > 0.5) gam = atan(X / den);
+            const double fGAM = gam * a.inv_dbeta_ch + a.cent;
+            const int GAM = trunc_i32_bp(fGAM);
+            const double w = fGAM - GAM;
+            const double w0 = 1 - w;
+            const double rL2 = rcp_f64(L2);
+            const bool ok = GAM >= 0 && GAM < a.W - 1;
+            const size_t off = (size_t)v * a.W + (ok ? GAM : 0);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const float* row = fl[k] + off;
+                const double sum1 = w0 * row[0] + w * row[1];
+                const double sum = sum1 + 0.0;
+                const float term = (float)(rL2 * sum * a.dbeta_view * a.xfactor);
+                acc[k] = ok ? acc[k] + term : acc[k];
+            }
+        }
+    }
+    if (in_img) {
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (s0 + k < S) a.img[((size_t)(s0 + k) * a.N + y) * a.N + x] = acc[k];
+    }
+}
+
 // The beam-hardening tool M/beam_hardening_correction.cpp:47-60 elementwise:
 // y = -ln p (single-precision log, as the tool's C++ overload resolution
 // gives), y' = |y| + alpha |y|^beta, out = exp(-y') in FP64, stored as float.
@@ -229,17 +292,45 @@ vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipSt
     return VCT_OK;
 }
 
+// k_backproject_multi with K sinograms per lane, for K = K0 ... kBpSinos.
+template <bool WIDE, int K0>
+static void launch_bp_multi(const BackprojArgs& a, int S, int K, hipStream_t s)
+{
+    if constexpr (K0 < kBpSinos) {
+        if (K > K0) {
+            launch_bp_multi<WIDE, K0 + 1>(a, S, K, s);
+            return;
+        }
+    }
+    const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, (S + K0 - 1) / K0);
+    hipLaunchKernelGGL((k_backproject_multi<WIDE, K0>), grid, dim3(kBpTile * kBpTile), 0, s, a, S);
+}
+
+template <bool WIDE>
+static void launch_bp_w(const BackprojArgs& a, int S, hipStream_t s)
+{
+    if (S == 1) {
+        const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, 1);
+        hipLaunchKernelGGL(k_backproject<WIDE>, grid, dim3(kBpTile * kBpTile), 0, s, a);
+        return;
+    }
+    // the fewest workgroup slices of at most kBpSinos sinograms, split evenly
+    // (S = 10: two slices of 5, not 8 + 2)
+    const int slices = (S + kBpSinos - 1) / kBpSinos;
+    launch_bp_multi<WIDE, 2>(a, S, (S + slices - 1) / slices, s);
+}
+
 vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s)
 {
-    const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, S);
+    if (S <= 0) return set_error(VCT_ERR_ARG, "no sinograms to backproject");
     // largest |tan(gamma)| of a masked pixel: the tangent from the source to
     // the mask circle of radius r (pixel centres lie within mask_r pitches)
     const double r = a.mask_r * a.pix;
     const bool wide = !(r < a.sid) || r / std::sqrt(a.sid * a.sid - r * r) > 0.49;
     if (wide)
-        hipLaunchKernelGGL(k_backproject<true>, grid, dim3(kBpTile * kBpTile), 0, s, a);
+        launch_bp_w<true>(a, S, s);
     else
-        hipLaunchKernelGGL(k_backproject<false>, grid, dim3(kBpTile * kBpTile), 0, s, a);
+        launch_bp_w<false>(a, S, s);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
