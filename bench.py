@@ -8,7 +8,12 @@ noise (sigma 5.37e-5, Philox), the MFMA Shepp-Logan filter and the
 backprojector.  value = rays x energy bins per second over the whole step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--no-batched] [--no-batch-fbp]
     python bench.py --gathers-only     (the batched gather workload alone, for profilers)
+
+Beside the plain step the line carries the batched decomposition (16
+phantoms x 2 spectra, graph replay vs the plain per-phantom loop) and the
+batched forward + FBP of 64 phantoms of four sizes x 3 spectra.
 
 Prints one JSON line.  Per-stage times come from HIP events recorded on the
 launch stream inside the timed region.
@@ -29,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 SEGMENTS_512 = 282_824_404  # sum of per-ray segments at 512^2, 0.1 cm, 800x609
 # HBM bytes per tracer launch on this workload, from rocprofv3 --pmc passes
 # over `bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-batched
-# --copy-probe 256`: FETCH_SIZE 27535.3 KiB x 2.00 + WRITE_SIZE 15225.0 KiB
+# --no-batch-fbp --copy-probe 256`: FETCH_SIZE 27535.3 KiB x 2.00 + WRITE_SIZE 15225.0 KiB
 # x 1.00, the factors being what the same counters read for the 256 MiB
 # device copy of the same run (131087.8 KiB fetched, 262144 KiB written).
 # The densities (4 MB) stay in L2/MALL: each XCD fetches them about twice,
@@ -250,7 +255,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-batched", action="store_true", help="skip the decomposition steps/s measurement")
-    ap.add_argument("--batch-fbp", action="store_true", help="also time 64 uneven phantoms x 3 spectra")
+    ap.add_argument("--no-batch-fbp", action="store_true",
+                    help="skip the forward + FBP of 64 uneven phantoms x 3 spectra")
     ap.add_argument("--gathers-only", action="store_true",
                     help="run only the batched gather workload (16 phantoms, blocked matrix, 3 forward + adjoint "
                          "pairs) for a profiler; prints no JSON line")
@@ -381,7 +387,7 @@ def main():
             out["batched_decomposition"] = bench_decomposition()
         except Exception as e:
             log("batched decomposition skipped:", repr(e))
-    if args.batch_fbp:
+    if not args.no_batch_fbp:
         try:
             out["batched_forward_fbp"] = bench_batch_fbp()
         except Exception as e:

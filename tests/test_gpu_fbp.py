@@ -105,13 +105,16 @@ def test_batched_fbp_equals_single(gpu, golden):
         assert torch.equal(many[i], one)
 
 
-@pytest.mark.parametrize("N,pix,V,S", [(512, 0.1, 800, 5), (64, 0.7, 90, 2), (64, 0.7, 90, 3),
-                                        (96, 0.5, 120, 10), (64, 0.7, 90, 17), (128, 0.6, 160, 6)])
+@pytest.mark.parametrize("N,pix,V,S", [(512, 0.1, 800, 5), (512, 0.1, 200, 2), (512, 0.1, 200, 3),
+                                        (512, 0.1, 200, 10), (512, 0.1, 200, 17), (512, 0.15, 200, 6),
+                                        (128, 0.6, 160, 6), (64, 0.7, 90, 3)])
 def test_multi_sinogram_backprojection_equals_single(gpu, oracle, N, pix, V, S):
     """S sinograms of one geometry share each lane's pixel-view geometry (up to
     8 per lane, in even slices: 10 -> 5 + 5, 17 -> 6 + 6 + 6 with the last
     slice one short); every image equals the single-sinogram backprojection
-    bit for bit.  128 x 0.6 cm is the wide field (library atan fallback)."""
+    bit for bit.  512 x 0.15 cm and 128 x 0.6 cm are wide fields (library atan
+    fallback); at 128^2 and 64^2 the launch keeps 2^18 lanes by running one
+    sinogram per lane."""
     import torch
     ops = _ops()
     rng = np.random.default_rng(40 + S)

@@ -19,6 +19,7 @@
 //                  accumulation order.
 #include "vct_internal.h"
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 
@@ -28,6 +29,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBpTile = 16;   // pixel tile edge
 constexpr int kBpSinos = 8;  // most sinograms per lane in k_backproject_multi
+constexpr long long kBpMinLanes = 1 << 18;  // lanes a backprojection launch keeps (4 waves per SIMD)
 
 __device__ __forceinline__ int trunc_i32_bp(double v)
 {
@@ -309,15 +311,21 @@ static void launch_bp_multi(const BackprojArgs& a, int S, int K, hipStream_t s)
 template <bool WIDE>
 static void launch_bp_w(const BackprojArgs& a, int S, hipStream_t s)
 {
-    if (S == 1) {
-        const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, 1);
+    // the fewest workgroup slices of at most kBpSinos sinograms, split evenly
+    // (S = 10: two slices of 5, not 8 + 2), but enough slices that the launch
+    // has kBpMinLanes lanes: 256^2 pixels x 3 sinograms in one slice would
+    // leave one wave per SIMD, and 3 slices of one sinogram run 2-3x faster
+    const long long px = (long long)a.N * a.N;
+    const long long floor_slices = (kBpMinLanes + px - 1) / px;
+    int slices = (S + kBpSinos - 1) / kBpSinos;
+    if (slices < floor_slices) slices = (int)std::min<long long>(S, floor_slices);
+    const int K = (S + slices - 1) / slices;
+    if (K == 1) {
+        const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, S);
         hipLaunchKernelGGL(k_backproject<WIDE>, grid, dim3(kBpTile * kBpTile), 0, s, a);
         return;
     }
-    // the fewest workgroup slices of at most kBpSinos sinograms, split evenly
-    // (S = 10: two slices of 5, not 8 + 2)
-    const int slices = (S + kBpSinos - 1) / kBpSinos;
-    launch_bp_multi<WIDE, 2>(a, S, (S + slices - 1) / slices, s);
+    launch_bp_multi<WIDE, 2>(a, S, K, s);
 }
 
 vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s)
