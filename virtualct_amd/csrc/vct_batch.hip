@@ -572,7 +572,18 @@ struct BatchSpecArgs {
     int rays, W, VT, B, S, M, NE;
 };
 
+// One past the highest bin with a nonzero probability among the bins this
+// wave's lanes hold (bin e on this lane, consecutive bins on consecutive
+// lanes), or 0; called by every active lane, it needs no atomics.
+__device__ __forceinline__ int last_nonzero_bin(int e, float p)
+{
+    const unsigned long long m = __ballot(p != 0.f);
+    const int base = e - (int)(threadIdx.x & 63);  // the bin of lane 0
+    return m ? base + 64 - __clzll((long long)m) : 0;
+}
+
 // Forward mode: raw and per-(b,s) max/min, one (b,s) per blockIdx.y/z.
+// Launched with 256 threads (four waves).
 template <int M>
 __global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
 {
@@ -580,21 +591,31 @@ __global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
     float* mu2 = lds;
     float* pdf = lds + a.NE * 8;
     __shared__ uint32_t red[2][4];
+    __shared__ int ne_w[4];
     const int s = blockIdx.y, b = blockIdx.z, NE = a.NE;
     const float kLog2e = 1.4426950408889634f;
     for (int i = threadIdx.x; i < NE * 8; i += blockDim.x) {
         const int e = i >> 3, m = i & 7;
         mu2[i] = (m < M) ? -a.att[((size_t)s * M + m) * NE + e] * kLog2e : 0.f;
     }
-    for (int e = threadIdx.x; e < NE; e += blockDim.x) pdf[e] = a.pdf[(size_t)s * NE + e];
+    // bins past the spectrum's last nonzero probability (the zero padding of
+    // spectra of unequal length) add exactly +0 and are skipped
+    int last = 0;
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+        const float p = a.pdf[(size_t)s * NE + e];
+        pdf[e] = p;
+        last = max(last, last_nonzero_bin(e, p));
+    }
+    if ((threadIdx.x & 63) == 0) ne_w[threadIdx.x >> 6] = last;
     __syncthreads();
+    const int ne = max(max(ne_w[0], ne_w[1]), max(ne_w[2], ne_w[3]));
     const int ray = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = ray < a.rays;
     float I = 0.f;
     if (valid && ray / a.W < a.VT) {
         float Lm[8];
         load_slot<M>(a.L, ray, a.B, b, Lm);
-        for (int e = 0; e < NE; e++) {
+        for (int e = 0; e < ne; e++) {
             const float4 m0 = reinterpret_cast<const float4*>(mu2)[2 * e];
             const float4 m1 = reinterpret_cast<const float4*>(mu2)[2 * e + 1];
             const float mu[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
@@ -636,13 +657,24 @@ __global__ __launch_bounds__(256) void k_batch_loss(BatchSpecArgs a, int lpr)
     float* pdf = lds + (size_t)S * NE * 8;  // [S][NE]
     float* imax = pdf + (size_t)S * NE;     // [S]
     float* lsum = imax + S;                 // [lpr]
+    int* ne_w = reinterpret_cast<int*>(lsum + lpr);  // [S][4]
     const float kLog2e = 1.4426950408889634f, kLn2 = 0.69314718055994531f;
     for (int i = threadIdx.x; i < S * NE * 8; i += blockDim.x) {
         const int s = i / (NE * 8), r = i - s * NE * 8, e = r >> 3, m = r & 7;
         mu2[i] = (m < M) ? -a.att[((size_t)s * M + m) * NE + e] * kLog2e : 0.f;
     }
-    for (int i = threadIdx.x; i < S * NE; i += blockDim.x) pdf[i] = a.pdf[i];
     for (int i = threadIdx.x; i < lpr; i += blockDim.x) lsum[i] = 0.f;
+    // bins past a spectrum's last nonzero probability add exactly +0 to I and
+    // to every d[m], and are skipped
+    for (int s = 0; s < S; s++) {
+        int last = 0;
+        for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+            const float p = a.pdf[(size_t)s * NE + e];
+            pdf[s * NE + e] = p;
+            last = max(last, last_nonzero_bin(e, p));
+        }
+        if ((threadIdx.x & 63) == 0) ne_w[4 * s + (threadIdx.x >> 6)] = last;
+    }
     __syncthreads();
     if (threadIdx.x < S) {
         float t = 0.f;
@@ -664,7 +696,8 @@ __global__ __launch_bounds__(256) void k_batch_loss(BatchSpecArgs a, int lpr)
             for (int m = 0; m < M; m++) d[m] = 0.f;
             const float* mus = mu2 + (size_t)s * NE * 8;
             const float* ps = pdf + (size_t)s * NE;
-            for (int e = 0; e < NE; e++) {
+            const int ne = max(max(ne_w[4 * s], ne_w[4 * s + 1]), max(ne_w[4 * s + 2], ne_w[4 * s + 3]));
+            for (int e = 0; e < ne; e++) {
                 // the 8 table entries of bin e in two 16-byte LDS reads
                 const float4 m0 = reinterpret_cast<const float4*>(mus)[2 * e];
                 const float4 m1 = reinterpret_cast<const float4*>(mus)[2 * e + 1];
@@ -1574,7 +1607,7 @@ vct_status vct_batch_loss(vct_plan* p, const float* d_L, int B, int M, const flo
     int lpr = 1;
     while (lpr < B) lpr <<= 1;
     if (lpr > 64) return set_error(VCT_ERR_UNSUPPORTED, "more than 64 phantoms in one loss launch");
-    const size_t lds = sizeof(float) * ((size_t)S * NE * 9 + S + lpr);
+    const size_t lds = sizeof(float) * ((size_t)S * NE * 9 + S + lpr) + sizeof(int) * 4 * S;
     if (lds > 160 * 1024) return set_error(VCT_ERR_UNSUPPORTED, "spectrum tables too large for LDS");
     BatchSpecArgs a{};
     a.L = d_L; a.att = d_att; a.pdf = d_pdf; a.target = d_target; a.w = d_w; a.loss = d_loss;
