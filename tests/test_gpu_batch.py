@@ -138,6 +138,40 @@ def test_batched_solver_matches_plain_loop_and_graph(gpu, small_setup):
     assert graphed.loss.sum().item() < lp.sum().item() * 5
 
 
+def test_batch_spectral_zero_tail_bins(gpu, small_setup):
+    """Spectra of unequal length share one zero-padded table; the kernels stop
+    at each spectrum's last nonzero probability.  The padded 3-spectrum sum
+    equals each spectrum's own unpadded sum bit for bit (an interior zero bin
+    and a zeroed tail included), and the loss and its weights do not change
+    when the table is padded further."""
+    import torch
+    ops, batch, _ = _mods()
+    plan, sm = small_setup
+    att, pdf = _spectra(gpu, ("80kV", "120kV", "6MV"))
+    pdf[2, 7] = 0.0          # interior zero bin: summed, contributes +0
+    pdf[2, -3:] = 0.0        # tail zeros with nonzero attenuation
+    x = _phantoms(3, 128, 0.35, gpu)
+    L = torch.empty((plan.V * plan.W, 3, 8), device=gpu)
+    batch.batch_paths([(sm, batch.interleave(x), L, 3)], 6)
+    raw, stats = batch.batch_spectral(plan, L, 6, att, pdf)
+    for s in range(3):
+        ne = int(torch.nonzero(pdf[s]).max().item()) + 1
+        a1 = att[s:s + 1, :, :ne].contiguous()
+        p1 = pdf[s:s + 1, :ne].contiguous()
+        r1, s1 = batch.batch_spectral(plan, L, 6, a1, p1)
+        assert torch.equal(raw[:, s], r1[:, 0]), s
+        assert torch.equal(stats[:, s], s1[:, 0]), s
+    tgt = (raw * 0.97 + 1e-3).contiguous()
+    w0, l0 = batch.batch_loss(plan, L, 6, att, pdf, tgt)
+    NE = att.shape[2]
+    att2 = torch.zeros((3, 6, NE + 37), device=gpu)
+    pdf2 = torch.zeros((3, NE + 37), device=gpu)
+    att2[:, :, :NE], pdf2[:, :NE] = att, pdf
+    w1, l1 = batch.batch_loss(plan, L, 6, att2, pdf2, tgt)
+    assert torch.equal(w0, w1)
+    assert torch.allclose(l0, l1, rtol=1e-6)  # float atomics: order may differ
+
+
 def test_batch_forward_fbp_uneven_sizes(gpu):
     import torch
     ops, batch, _ = _mods()
