@@ -190,8 +190,10 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject(BackprojArgs 
             const double w = fGAM - GAM;
             const bool ok = GAM >= 0 && GAM < a.W - 1;
             const float* row = fl + (size_t)v * a.W + (ok ? GAM : 0);
-            const double sum1 = (1 - w) * row[0] + w * row[1];
-            const double sum = sum1 + 0.0;
+            // the reference's sum is 1.0 * sum1 + 0.0 * 0 (:92), which only turns
+            // -0 into +0; acc is never -0 (it starts at +0, and x + (-x) is +0),
+            // and acc + (-0) = acc + (+0) = acc, so the add is left out
+            const double sum = (1 - w) * row[0] + w * row[1];
             const float term = (float)(rcp_f64(L2) * sum * a.dbeta_view * a.xfactor);
             acc = ok ? acc + term : acc;
         }
@@ -247,8 +249,7 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_multi(Backpro
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const float* row = fl[k] + off;
-                const double sum1 = w0 * row[0] + w * row[1];
-                const double sum = sum1 + 0.0;
+                const double sum = w0 * row[0] + w * row[1];  // + 0.0 left out as above
                 const float term = (float)(rL2 * sum * a.dbeta_view * a.xfactor);
                 acc[k] = ok ? acc[k] + term : acc[k];
             }
