@@ -644,6 +644,75 @@ __global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
     }
 }
 
+// Forward mode with every spectrum of phantom blockIdx.z in one block: the
+// tables and the ray's path integrals are loaded once for all S spectra.
+// Each (phantom, spectrum, ray) is summed exactly as in k_batch_spectral.
+template <int M>
+__global__ __launch_bounds__(256) void k_batch_spectral_all(BatchSpecArgs a)
+{
+    extern __shared__ float lds[];
+    const int S = a.S, NE = a.NE, b = blockIdx.z;
+    float* mu2 = lds;                                          // [S][NE][8]
+    float* pdf = lds + (size_t)S * NE * 8;                     // [S][NE]
+    int* ne_w = reinterpret_cast<int*>(pdf + (size_t)S * NE);  // [S][4]
+    __shared__ uint32_t red[2][4];
+    const float kLog2e = 1.4426950408889634f;
+    for (int i = threadIdx.x; i < S * NE * 8; i += blockDim.x) {
+        const int s = i / (NE * 8), r = i - s * NE * 8, e = r >> 3, m = r & 7;
+        mu2[i] = (m < M) ? -a.att[((size_t)s * M + m) * NE + e] * kLog2e : 0.f;
+    }
+    for (int s = 0; s < S; s++) {
+        int last = 0;
+        for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+            const float p = a.pdf[(size_t)s * NE + e];
+            pdf[s * NE + e] = p;
+            last = max(last, last_nonzero_bin(e, p));
+        }
+        if ((threadIdx.x & 63) == 0) ne_w[4 * s + (threadIdx.x >> 6)] = last;
+    }
+    __syncthreads();
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = ray < a.rays;
+    const bool hit = valid && ray / a.W < a.VT;
+    float Lm[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (hit) load_slot<M>(a.L, ray, a.B, b, Lm);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int s = 0; s < S; s++) {
+        const int ne = max(max(ne_w[4 * s], ne_w[4 * s + 1]), max(ne_w[4 * s + 2], ne_w[4 * s + 3]));
+        const float* mus = mu2 + (size_t)s * NE * 8;
+        const float* ps = pdf + (size_t)s * NE;
+        float I = 0.f;
+        if (hit) {
+            for (int e = 0; e < ne; e++) {
+                const float4 m0 = reinterpret_cast<const float4*>(mus)[2 * e];
+                const float4 m1 = reinterpret_cast<const float4*>(mus)[2 * e + 1];
+                const float mu[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+                float x = 0.f;
+#pragma unroll
+                for (int m = 0; m < M; m++) x = fmaf(mu[m], Lm[m], x);
+                I += ps[e] * __builtin_amdgcn_exp2f(x);  // as k_spectral
+            }
+        }
+        const size_t bs = (size_t)b * S + s;
+        if (valid) a.raw[bs * a.rays + ray] = I;
+        uint32_t mx = valid ? __float_as_uint(I) : 0u;
+        uint32_t mn = valid ? __float_as_uint(I) : 0x7f800000u;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+            mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, 64));
+        }
+        if (lane == 0) { red[0][wave] = mx; red[1][wave] = mn; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) { mx = max(mx, red[0][w]); mn = min(mn, red[1][w]); }
+            atomicMax(a.stats + 2 * bs, mx);
+            atomicMin(a.stats + 2 * bs + 1, mn);
+        }
+        __syncthreads();  // red is reused by the next spectrum
+    }
+}
+
 // Loss mode: per (ray, phantom) over all spectra,
 //   r_s = ln(I_s / I_max,s) - ln t_s,  loss_b += sum_s r_s^2,
 //   w[ray][b][m] = sum_s (2 r_s / I_s) dI_s/dL_m,
@@ -1584,14 +1653,28 @@ vct_status vct_batch_spectral(vct_plan* p, const float* d_L, int B, int M, const
     a.rays = p->g.n_views * p->g.n_channels; a.W = p->g.n_channels; a.VT = p->VT;
     a.B = B; a.S = S; a.M = M; a.NE = NE;
     hipLaunchKernelGGL(k_batch_stats_init, dim3((B * S + 63) / 64), dim3(64), 0, s, d_stats, B * S);
-    const dim3 grid((a.rays + 255) / 256, S, B);
-    const size_t lds = sizeof(float) * (size_t)NE * 9;
-    switch (M) {
+    // all spectra of a phantom per block while their tables fit in 64 KB of
+    // LDS (two blocks per CU); else one (phantom, spectrum) per block
+    const size_t lds_all = sizeof(float) * (size_t)S * NE * 9 + sizeof(int) * 4 * S;
+    if (lds_all <= 64 * 1024) {
+        const dim3 grid((a.rays + 255) / 256, 1, B);
+        switch (M) {
+#define VCT_SPEC_CASE(MM) \
+    case MM: hipLaunchKernelGGL((k_batch_spectral_all<MM>), grid, dim3(256), lds_all, s, a); break;
+            VCT_SPEC_CASE(1) VCT_SPEC_CASE(2) VCT_SPEC_CASE(3) VCT_SPEC_CASE(4)
+            VCT_SPEC_CASE(5) VCT_SPEC_CASE(6) VCT_SPEC_CASE(7) VCT_SPEC_CASE(8)
+#undef VCT_SPEC_CASE
+        }
+    } else {
+        const dim3 grid((a.rays + 255) / 256, S, B);
+        const size_t lds = sizeof(float) * (size_t)NE * 9;
+        switch (M) {
 #define VCT_SPEC_CASE(MM) \
     case MM: hipLaunchKernelGGL((k_batch_spectral<MM>), grid, dim3(256), lds, s, a); break;
-        VCT_SPEC_CASE(1) VCT_SPEC_CASE(2) VCT_SPEC_CASE(3) VCT_SPEC_CASE(4)
-        VCT_SPEC_CASE(5) VCT_SPEC_CASE(6) VCT_SPEC_CASE(7) VCT_SPEC_CASE(8)
+            VCT_SPEC_CASE(1) VCT_SPEC_CASE(2) VCT_SPEC_CASE(3) VCT_SPEC_CASE(4)
+            VCT_SPEC_CASE(5) VCT_SPEC_CASE(6) VCT_SPEC_CASE(7) VCT_SPEC_CASE(8)
 #undef VCT_SPEC_CASE
+        }
     }
     VCT_HIP(hipGetLastError());
     return VCT_OK;
