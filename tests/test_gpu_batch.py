@@ -143,7 +143,8 @@ def test_batch_spectral_zero_tail_bins(gpu, small_setup):
     at each spectrum's last nonzero probability.  The padded 3-spectrum sum
     equals each spectrum's own unpadded sum bit for bit (an interior zero bin
     and a zeroed tail included), and the loss and its weights do not change
-    when the table is padded further."""
+    when the table is padded further.  700 bins take the per-spectrum form of
+    the sum (tables beyond 64 KB of LDS) with the same bits."""
     import torch
     ops, batch, _ = _mods()
     plan, sm = small_setup
@@ -161,9 +162,16 @@ def test_batch_spectral_zero_tail_bins(gpu, small_setup):
         r1, s1 = batch.batch_spectral(plan, L, 6, a1, p1)
         assert torch.equal(raw[:, s], r1[:, 0]), s
         assert torch.equal(stats[:, s], s1[:, 0]), s
+    # padded to 700 bins the tables exceed 64 KB of LDS and the sum takes one
+    # (phantom, spectrum) per block: the same bits
+    NE = att.shape[2]
+    attw = torch.zeros((3, 6, 700), device=gpu)
+    pdfw = torch.zeros((3, 700), device=gpu)
+    attw[:, :, :NE], pdfw[:, :NE] = att, pdf
+    rw, sw = batch.batch_spectral(plan, L, 6, attw, pdfw)
+    assert torch.equal(raw, rw) and torch.equal(stats, sw)
     tgt = (raw * 0.97 + 1e-3).contiguous()
     w0, l0 = batch.batch_loss(plan, L, 6, att, pdf, tgt)
-    NE = att.shape[2]
     att2 = torch.zeros((3, 6, NE + 37), device=gpu)
     pdf2 = torch.zeros((3, NE + 37), device=gpu)
     att2[:, :, :NE], pdf2[:, :NE] = att, pdf
