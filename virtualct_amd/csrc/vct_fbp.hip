@@ -17,6 +17,8 @@
 //                  workgroup, one lane per pixel, the view loop uniform over
 //                  the wave, FP64 geometry and the reference's float
 //                  accumulation order.
+//   k_backproject_multi  the same for 2..8 sinograms of one geometry per
+//                  lane, sharing each pixel-view's geometry.
 #include "vct_internal.h"
 
 #include <algorithm>
