@@ -214,7 +214,7 @@ vct_status vct_lengthproj(vct_plan* plan, const float* d_proj, double z_cm, doub
 
 /* Pixel-driven fan backprojection, replacing reconstructImage
  * (F/reconstructImageTOMO.cpp:11-105), n_sino images.  Several sinograms in
- * one call share the per-pixel geometry (up to 8 per pass), so one call with
+ * one call share the per-pixel geometry (up to 16 per pass), so one call with
  * n_sino = S is faster than S calls; the images are bit-identical either way. */
 vct_status vct_backproject(vct_plan* plan, const float* d_fltd, int n_sino, float* d_img,
                            void* stream);

@@ -106,12 +106,13 @@ def test_batched_fbp_equals_single(gpu, golden):
 
 
 @pytest.mark.parametrize("N,pix,V,S", [(512, 0.1, 800, 5), (512, 0.1, 200, 2), (512, 0.1, 200, 3),
-                                        (512, 0.1, 200, 10), (512, 0.1, 200, 17), (512, 0.15, 200, 6),
+                                        (512, 0.1, 200, 10), (512, 0.1, 200, 17), (512, 0.1, 200, 35),
+                                        (512, 0.15, 200, 6),
                                         (128, 0.6, 160, 6), (64, 0.7, 90, 3)])
 def test_multi_sinogram_backprojection_equals_single(gpu, oracle, N, pix, V, S):
     """S sinograms of one geometry share each lane's pixel-view geometry (up to
-    8 per lane, in even slices: 10 -> 5 + 5, 17 -> 6 + 6 + 6 with the last
-    slice one short); every image equals the single-sinogram backprojection
+    16 per lane, in even slices: 17 -> 9 + 8, 35 -> 12 + 12 + 11); every
+    image equals the single-sinogram backprojection
     bit for bit.  512 x 0.15 cm and 128 x 0.6 cm are wide fields (library atan
     fallback); at 128^2 and 64^2 the launch keeps 2^18 lanes by running one
     sinogram per lane."""

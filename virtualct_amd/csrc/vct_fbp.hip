@@ -17,7 +17,7 @@
 //                  workgroup, one lane per pixel, the view loop uniform over
 //                  the wave, FP64 geometry and the reference's float
 //                  accumulation order.
-//   k_backproject_multi  the same for 2..8 sinograms of one geometry per
+//   k_backproject_multi  the same for 2..16 sinograms of one geometry per
 //                  lane, sharing each pixel-view's geometry.
 #include "vct_internal.h"
 
@@ -30,7 +30,7 @@ namespace vct {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBpTile = 16;   // pixel tile edge
-constexpr int kBpSinos = 8;  // most sinograms per lane in k_backproject_multi
+constexpr int kBpSinos = 16;  // most sinograms per lane in k_backproject_multi
 constexpr long long kBpMinLanes = 1 << 18;  // lanes a backprojection launch keeps (4 waves per SIMD)
 
 __device__ __forceinline__ int trunc_i32_bp(double v)
@@ -350,7 +350,7 @@ template <bool WIDE>
 static void launch_bp_w(const BackprojArgs& a, int S, hipStream_t s)
 {
     // the fewest workgroup slices of at most kBpSinos sinograms, split evenly
-    // (S = 10: two slices of 5, not 8 + 2), but enough slices that the launch
+    // (S = 20: two slices of 10, not 16 + 4), but enough slices that the launch
     // has kBpMinLanes lanes: 256^2 pixels x 3 sinograms in one slice would
     // leave one wave per SIMD, and 3 slices of one sinogram run 2-3x faster
     const long long px = (long long)a.N * a.N;
