@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_F32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 SEGMENTS_512 = 282_824_404  # sum of per-ray segments at 512^2, 0.1 cm, 800x609
 # HBM bytes per tracer launch on this workload, from rocprofv3 --pmc passes
 # over `bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-batched
@@ -79,7 +80,8 @@ def cpu_baseline(dens_f32, att, pdf, geom_c, seconds_budget=20.0):
             "value": mbfpa, "unit": "ray·energy-bins/s", "cores": threads, "kind": "reference",
             "sample": (f"reference reprojection_TOMO (compiled from source, -O2 -fopenmp, {threads} threads) "
                        f"for {n_bins} of the 120 energy bins on the full 512^2 phantom / 800x609 geometry "
-                       f"({t_m:.2f} s, re-traced per bin as the reference does); reference FBP single-thread: "
+                       f"({t_m:.2f} s, re-traced per bin as the reference does); reference FBP single-thread "
+                       f"on a constant 0.5 sinogram (its time does not depend on the values): "
                        f"filter {t_f:.2f} s + backprojection {t_b:.2f} s"),
             "fbp_voxel_views_per_s": p_mask * g.V / (t_f + t_b),
             "fbp_cores": 1,
@@ -126,12 +128,12 @@ def bench_decomposition(B=16, iters=50, plain_iters=1):
     sm = batch.SystemMatrix(plan, transpose=True, blocked=True)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
-    # plain loops: per phantom, per spectrum forward + adjoint through autograd,
-    # with the ray-driven atomic adjoint (the plain path) and with the gather
-    # over the plan's cached system matrix
+    # plain loops: per phantom one forward of both spectra (one trace) and one
+    # adjoint through autograd, with the package's default adjoint (the gather
+    # over the plan's cached system matrix) and with the ray-driven atomic one
     plain_ms = {}
     mode0 = plan.adjoint_mode
-    for mode in ("atomic", "sysmat"):
+    for mode in ("sysmat", "atomic"):
         plan.adjoint_mode = mode
         plain = decompose.PlainLoop(plan, targets, att, pdf, init)
         plain.step()
@@ -142,7 +144,7 @@ def bench_decomposition(B=16, iters=50, plain_iters=1):
         torch.cuda.synchronize()
         plain_ms[mode] = (time.perf_counter() - t0) / plain_iters
     plan.adjoint_mode = mode0
-    plain_step = plain_ms["atomic"]
+    plain_step = plain_ms["sysmat"]
     # batched, graph-captured
     solver = batch.BatchedSolver(sm, targets, att, pdf, init)
     err0 = [decompose.reprojection_error(plan, init[b], targets[b], att, pdf) for b in range(2)]
@@ -160,17 +162,18 @@ def bench_decomposition(B=16, iters=50, plain_iters=1):
         "batched_steps_per_s": 1.0 / batched_step, "plain_steps_per_s": 1.0 / plain_step,
         "batched_vs_plain": plain_step / batched_step,
         "batched_ms_per_step": batched_step * 1e3, "plain_ms_per_step": plain_step * 1e3,
-        "plain_sysmat_steps_per_s": 1.0 / plain_ms["sysmat"],
-        "batched_vs_plain_sysmat": plain_ms["sysmat"] / batched_step,
+        "plain_atomic_steps_per_s": 1.0 / plain_ms["atomic"],
+        "batched_vs_plain_atomic": plain_ms["atomic"] / batched_step,
         "system_matrix_build_s": build_s, "system_matrix_segments": sm.segments,
         "system_matrix_bytes": sm.bytes,
         "reprojection_rel_rms_before": err0, "reprojection_rel_rms_after": err1,
         "definition": "one step = forward of both spectra, log-domain loss, adjoint and projected BB update "
-                      "for all phantoms; plain = per phantom and per spectrum autograd calls: on-the-fly trace "
-                      "+ spectral sum, adjoint by the ray-driven transposed march with atomics; plain_sysmat = "
-                      "the same with the adjoint gathered over the plan's cached system matrix; batched = cached "
-                      "blocked system matrix, per direction one two-stage gather (tile / view-block partial sums, "
-                      "then per-row sums) for all phantoms, HIP-graph replay",
+                      "for all phantoms; plain = per phantom one autograd forward of both spectra (one on-the-fly "
+                      "trace + spectral sums) and one backward, whose adjoint is the package default: the gather "
+                      "over the plan's cached voxel-major system matrix; plain_atomic = the same with the "
+                      "ray-driven transposed march with atomics; batched = cached blocked system matrix, per "
+                      "direction one two-stage gather (tile / view-block partial sums, then per-row sums) for all "
+                      "phantoms, HIP-graph replay",
     }
 
 
@@ -303,7 +306,7 @@ def main():
     def step(ev=None):
         if ev is not None:
             ev[0].record()
-        L = ops.trace_paths(plan, pk, dt, M)
+        L = ops.trace_paths(plan, pk, dt, M, check=False)
         if ev is not None:
             ev[1].record()
         raw, stats, _ = ops.spectral_sum(plan, L, M, d_att, d_pdf)
@@ -330,6 +333,7 @@ def main():
         img = step(events[k])
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
+    ops.check_errors()  # a ray that gave up in any timed step raises here
     st_ms = {s: 0.0 for s in stages}
     for ev in events:
         for i, s in enumerate(stages):
@@ -344,6 +348,10 @@ def main():
                   <= ((N - 1) / 2.0) ** 2).sum())
     trace_bytes = SEGMENTS_512 * M * 2 + V * W * 4
     achieved = trace_bytes / (st_ms["trace"] * 1e-3) / 1e9
+    filt_flop = 2.0 * V * W * W
+    filt_tflops = filt_flop / (st_ms["filter"] * 1e-3) / 1e12
+    bp_bytes = p_mask * V * 8 + N * N * 4
+    bp_gbs = bp_bytes / (st_ms["backproject"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -380,6 +388,17 @@ def main():
                     "densities are L2/MALL-resident (traffic = measured HBM bytes); the kernel is FP64 "
                     "VALU-issue bound (~76% VALU busy)",
         },
+        "rooflines": [
+            {"kernel": "k_trace_fwd (tracer)", "bound": "hbm", "unit": "GB/s", "achieved": achieved,
+             "peak": HBM_PEAK_GBS, "frac": achieved / HBM_PEAK_GBS,
+             "per_launch": "segments x M x 2 B + V x W x 4 B"},
+            {"kernel": "k_filter_prep + k_filter_gemm (filter stage)", "bound": "mfma", "unit": "TFLOP/s",
+             "achieved": filt_tflops, "peak": MFMA_F32_PEAK_TFLOPS, "frac": filt_tflops / MFMA_F32_PEAK_TFLOPS,
+             "per_launch": "2 V W^2 = %.1f MFLOP (stage time includes the pre-weight kernel)" % (filt_flop / 1e6)},
+            {"kernel": "k_backproject", "bound": "hbm", "unit": "GB/s", "achieved": bp_gbs,
+             "peak": HBM_PEAK_GBS, "frac": bp_gbs / HBM_PEAK_GBS,
+             "per_launch": "P_mask x V x 8 B + N^2 x 4 B = %.3f GB" % (bp_bytes / 1e9)},
+        ],
         "cpu_baseline": None,
     }
     if not args.no_batched:

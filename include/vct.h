@@ -16,6 +16,9 @@
  *     null stream, which is ordered with all blocking streams such as
  *     PyTorch's default stream).  Calls on one stream are ordered; calls
  *     that share the context scratch buffer must not overlap across streams.
+ *     The scratch only grows: a larger request allocates a new buffer and
+ *     the old one stays allocated until vct_terminate, so work recorded in a
+ *     HIP graph never writes into freed memory.
  *   - Inputs are never modified (the reference zeroes negative material-0
  *     voxels in place, M/reprojection_TOMO.cpp:151; here a negative density
  *     contributes nothing, which gives the same sums without the write).
@@ -96,6 +99,15 @@ vct_status vct_init(int device, vct_ctx** ctx);
 /* Replaces terminateGPU (M/main.cpp:670). */
 vct_status vct_terminate(vct_ctx* ctx);
 
+/* Device-side error flags (no counterpart in the reference, whose loops
+ * have no step bound).  The device-pointer entry points enqueue work and
+ * return without waiting, so a kernel that gives up (a ray that exceeds its
+ * marching step bound, a system-matrix row that does not match its count)
+ * only raises a flag in the context.  This call waits for `stream`, reads
+ * and clears the flag and returns VCT_ERR_INTERNAL if it was set (the
+ * results of the work since the last check are then invalid). */
+vct_status vct_check_errors(vct_ctx* ctx, void* stream);
+
 /* Device tables for one geometry (view cos/sin from glibc on the host, as
  * prepareTrigonometricTable M/misc_TOMO.cpp:81-100; per-channel detector
  * points M/reprojection_TOMO.cpp:60-71; filter matrices F/filterSinogramTOMO.cpp:9-138). */
@@ -105,8 +117,11 @@ vct_status vct_plan_destroy(vct_plan* plan);
 /* Plan options (no counterpart in the reference).
  * VCT_OPT_EXACT_TRACE = 1: march every ray with correctly rounded divisions
  * (the reference's operation sequence) instead of the default guarded
- * reciprocal march; both give the same segments, the default is faster. */
-enum { VCT_OPT_EXACT_TRACE = 1 };
+ * reciprocal march; both give the same segments, the default is faster.
+ * VCT_OPT_MAX_STEPS = 2: marching steps after which a ray gives up and
+ * raises the context's error flag (0, the default: 4N + 4/pixel + 64, more
+ * than any ray of the geometry needs). */
+enum { VCT_OPT_EXACT_TRACE = 1, VCT_OPT_MAX_STEPS = 2 };
 vct_status vct_plan_set_option(vct_plan* plan, int option, int value);
 
 /* ------------------------------------------------------- host-side tables */
@@ -183,6 +198,12 @@ vct_status vct_mbfpa_forward(vct_plan* plan, const void* d_packed, int dens_type
  * the "IN" threshold of :250-255. */
 vct_status vct_filter_sinogram(vct_plan* plan, const float* d_proj, int n_sino, int filter,
                                int in_mode, float* d_fltd, void* stream);
+/* The same with a caller-owned workspace (the packed GEMM operand) of
+ * vct_filter_workspace_bytes(plan, n_sino) bytes instead of the context
+ * scratch; callers that record the filter in a HIP graph pass their own. */
+size_t vct_filter_workspace_bytes(const vct_plan* plan, int n_sino);
+vct_status vct_filter_sinogram_ws(vct_plan* plan, const float* d_proj, int n_sino, int filter,
+                                  int in_mode, float* d_fltd, void* d_work, void* stream);
 /* Beam-hardening correction, replacing the standalone tool
  * M/beam_hardening_correction.cpp:21-69 (run between MBFPA and FBP):
  * y = -ln p, y' = |y| + alpha*|y|^beta, out = exp(-y'), elementwise over n
@@ -211,6 +232,16 @@ vct_status vct_minmax(const float* d_x, int n, float* h_maxmin, void* stream);
  * reference indexes its tables with negative views.  d_npf: float [N][N]. */
 vct_status vct_lengthproj(vct_plan* plan, const float* d_proj, double z_cm, double couch_speed_cm,
                           double iso_det_cm, float* d_npf, void* stream);
+/* The same for a window whose rows hold the first proj_width (<= the plan's
+ * n_channels) channels of the plan's detector: measured data keep 582 or
+ * fewer channels of the 609-channel detector, whose pitch and isocentre
+ * channel still place the hits (the reference's compile-time End_Channel and
+ * isocenter_channel, M/physParamsTOMO.h:43-47, with the measured width as
+ * the row stride, M/lengthproj_TOMO.cpp:152-166).  A hit past the window's
+ * last channel takes that channel's value; the reference reads the next
+ * row's channels there (past its buffer on the last rows). */
+vct_status vct_lengthproj_window(vct_plan* plan, const float* d_proj, int proj_width, double z_cm,
+                                 double couch_speed_cm, double iso_det_cm, float* d_npf, void* stream);
 
 /* Pixel-driven fan backprojection, replacing reconstructImage
  * (F/reconstructImageTOMO.cpp:11-105), n_sino images.  Several sinograms in
@@ -254,6 +285,14 @@ typedef struct vct_sysmat vct_sysmat;
 vct_status vct_sysmat_build(vct_plan* plan, int flags, vct_sysmat** out, void* stream);
 vct_status vct_sysmat_destroy(vct_sysmat* m);
 int64_t vct_sysmat_segments(const vct_sysmat* m);
+/* The cached segments of one ray (view * channels + channel) in march
+ * order, copied to the host (waits for `stream`): voxel index x + y*N and
+ * float length, i.e. the reference's reprojection_vnum /
+ * reprojection_systemMatrixfloat entries of the ray's in-image segment
+ * counters (M/reprojection_TOMO.cpp:160-163, :186-188).  *n_out receives
+ * the count; VCT_ERR_ARG if it exceeds capacity. */
+vct_status vct_sysmat_row(const vct_sysmat* m, int ray, uint32_t* h_voxel, float* h_length, int capacity,
+                          int* n_out, void* stream);
 size_t vct_sysmat_bytes(const vct_sysmat* m);
 /* Bytes of the partial-sum workspace a batched gather over this system
  * matrix needs for n_phantoms phantoms in the blocked form (0 if it has
@@ -339,6 +378,9 @@ vct_status vct_reconstruct_image_host(vct_ctx* ctx, const vct_geometry* g, const
  * host buffers proj [V/2*2][W] -> npf [N][N]. */
 vct_status vct_lengthproj_host(vct_ctx* ctx, const vct_geometry* g, const float* proj, double z_cm,
                                double couch_speed_cm, double iso_det_cm, float* npf);
+/* vct_lengthproj_window on host buffers, proj [V/2*2][proj_width]. */
+vct_status vct_lengthproj_window_host(vct_ctx* ctx, const vct_geometry* g, const float* proj, int proj_width,
+                                      double z_cm, double couch_speed_cm, double iso_det_cm, float* npf);
 
 /* ------------------------------------------- measured (real-scanner) data */
 

@@ -98,8 +98,8 @@ class Oracle:
         L.orc_backproject_ref.argtypes = [G, _f32p, _f32p]
         L.orc_backproject_f64.argtypes = [G, _f64p, _f64p]
         L.orc_adjoint_f64.argtypes = [G, _f32p, C.c_int, _f32p, _f32p, C.c_int, _f64p, _f64p]
-        for fn, out in (("orc_lengthproj_ref", _f32p), ("orc_lengthproj_f64", _f64p)):
-            getattr(L, fn).argtypes = [G, _f32p, C.c_double, C.c_double, C.c_double, out]
+        for fn, out in (("orc_lengthproj_window_ref", _f32p), ("orc_lengthproj_window_f64", _f64p)):
+            getattr(L, fn).argtypes = [G, _f32p, C.c_int, C.c_double, C.c_double, C.c_double, out]
             getattr(L, fn).restype = C.c_int
         L.orc_load_measured.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f32p, _f32p, _f32p, _u8p,
                                         C.POINTER(C.c_double)]
@@ -235,16 +235,18 @@ class Oracle:
 
 
     def lengthproj_ref(self, g: Geom, proj, z_cm: float, couch_cm: float, iso_det: float = 57.6):
-        """npf [N,N] float (reference rounding); None when the view window is invalid."""
+        """npf [N,N] float (reference rounding) of a window proj [V/2*2, w], whose
+        rows hold the first w <= g.W channels of the detector; None when the
+        view window is invalid."""
+        proj = np.ascontiguousarray(proj, np.float32)
         out = np.zeros((g.N, g.N), np.float32)
-        rc = self.lib.orc_lengthproj_ref(C.byref(g.c()), np.ascontiguousarray(proj, np.float32), z_cm, couch_cm,
-                                         iso_det, out)
+        rc = self.lib.orc_lengthproj_window_ref(C.byref(g.c()), proj, proj.shape[-1], z_cm, couch_cm, iso_det, out)
         return out if rc == 0 else None
 
     def lengthproj_f64(self, g: Geom, proj, z_cm: float, couch_cm: float, iso_det: float = 57.6):
+        proj = np.ascontiguousarray(proj, np.float32)
         out = np.zeros((g.N, g.N), np.float64)
-        rc = self.lib.orc_lengthproj_f64(C.byref(g.c()), np.ascontiguousarray(proj, np.float32), z_cm, couch_cm,
-                                         iso_det, out)
+        rc = self.lib.orc_lengthproj_window_f64(C.byref(g.c()), proj, proj.shape[-1], z_cm, couch_cm, iso_det, out)
         return out if rc == 0 else None
 
 

@@ -756,7 +756,7 @@ static int lp_window(int V, double z_cm, double couch_cm, int* base, int* count)
  * reference's operation order; returns 0 when the reference adds nothing
  * (a NaN channel position). */
 static int lp_term(const orc_geom* g, double rx, double ry, double rz, double c, double s,
-                   const float* row, double iso_det, double* out)
+                   const float* row, int pw, double iso_det, double* out)
 {
     const double pix = g->pix;
     const double sx = g->sid * s, sy = g->sid * c, sz = rz;
@@ -802,7 +802,11 @@ static int lp_term(const orc_geom* g, double rx, double ry, double rz, double c,
     const double rot_px = px * c - py * s;
     const double delta = g->max_alpha / ((g->W - 1) / 2);
     const double dj = rot_px > 0.0 ? g->iso_channel - angle / delta : g->iso_channel + angle / delta;
-    const int last = g->W - 1;
+    /* the row holds the first pw channels of the W-channel detector (the
+     * reference's End_Channel is fixed at 609 and the row stride is the
+     * measured width; hits past the row's last channel take that channel,
+     * where the reference reads into the next row) */
+    const int last = pw - 1;
     double np;
     if (0 <= dj && dj < last) {   /* :152-158 */
         int i0 = (int)dj;
@@ -821,6 +825,7 @@ static int lp_term(const orc_geom* g, double rx, double ry, double rz, double c,
 
 #define LP_BODY(ACC_T, ADD)                                                                 \
     int base, count;                                                                        \
+    if (pw < 2 || pw > g->W) return -1;                                                     \
     if (lp_window(g->V, z_cm, couch_cm, &base, &count) != 0) return -1;                     \
     const int N = g->N, V = g->V, W = g->W;                                                 \
     double* cosT = (double*)malloc(sizeof(double) * V);                                     \
@@ -839,8 +844,8 @@ static int lp_term(const orc_geom* g, double rx, double ry, double rz, double c,
                     int jr = base + j;                                                      \
                     if (jr >= V) jr -= V;                                                   \
                     double t;                                                               \
-                    if (lp_term(g, rx, ry, z_cm, cosT[jr], sinT[jr], proj + (size_t)j * W,  \
-                                iso_det, &t))                                               \
+                    if (lp_term(g, rx, ry, z_cm, cosT[jr], sinT[jr], proj + (size_t)j * pw, \
+                                pw, iso_det, &t))                                           \
                         ADD;                                                                \
                 }                                                                           \
             }                                                                               \
@@ -852,17 +857,29 @@ static int lp_term(const orc_geom* g, double rx, double ry, double rz, double c,
     free(mask);                                                                             \
     return 0;
 
-int orc_lengthproj_ref(const orc_geom* g, const float* proj, double z_cm, double couch_cm,
-                       double iso_det, float* npf)
+int orc_lengthproj_window_ref(const orc_geom* g, const float* proj, int pw, double z_cm, double couch_cm,
+                              double iso_det, float* npf)
 {
     /* M/lengthproj_TOMO.cpp:14-185: float accumulator, views ascending */
     LP_BODY(float, acc += (float)t)
 }
 
+int orc_lengthproj_window_f64(const orc_geom* g, const float* proj, int pw, double z_cm, double couch_cm,
+                              double iso_det, double* npf)
+{
+    LP_BODY(double, acc += t)
+}
+
+int orc_lengthproj_ref(const orc_geom* g, const float* proj, double z_cm, double couch_cm,
+                       double iso_det, float* npf)
+{
+    return orc_lengthproj_window_ref(g, proj, g->W, z_cm, couch_cm, iso_det, npf);
+}
+
 int orc_lengthproj_f64(const orc_geom* g, const float* proj, double z_cm, double couch_cm,
                        double iso_det, double* npf)
 {
-    LP_BODY(double, acc += t)
+    return orc_lengthproj_window_f64(g, proj, g->W, z_cm, couch_cm, iso_det, npf);
 }
 #undef LP_BODY
 

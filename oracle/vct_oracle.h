@@ -116,6 +116,14 @@ int orc_lengthproj_ref(const orc_geom* g, const float* proj, double z_cm, double
                        double iso_det, float* npf);
 int orc_lengthproj_f64(const orc_geom* g, const float* proj, double z_cm, double couch_cm,
                        double iso_det, double* npf);
+/* The same for a window whose rows hold the first pw <= W channels of the
+ * W-channel detector (row stride pw; measured data keep fewer channels than
+ * the detector has, M/lengthproj_TOMO.cpp:152-166 with End_Channel 609).
+ * Returns -1 also for pw outside 2..W. */
+int orc_lengthproj_window_ref(const orc_geom* g, const float* proj, int pw, double z_cm, double couch_cm,
+                              double iso_det, float* npf);
+int orc_lengthproj_window_f64(const orc_geom* g, const float* proj, int pw, double z_cm, double couch_cm,
+                              double iso_det, double* npf);
 
 /* Measured-data loader, M/projectionDataTOMO.cpp:323-524, on arrays holding
  * the file contents (detector of 609 channels, the first 27 skipped; air

@@ -129,3 +129,52 @@ def measured_inputs(inpr, H, rate, rot, seed):
     step = 5 if rate >= 200 else 1
     det = rng.uniform(0.0, 0.9, (rot * H * step, 609)).astype(np.float32)
     return air, det, det[::step].reshape(rot, H, 609).copy()
+
+
+# Measured files with couch motion through the reference's whole MBFPA program:
+# (name, class, rotations, views per rotation, channels, seed, z [cm], couch [cm/rotation]).
+# The image is 256^2 x 0.1 cm: every masked pixel's ray then meets the detector
+# at channels 164-445, inside the 500 measured channels.  (The reference's npf
+# interpolates channels up to its compile-time End_Channel, 608, with the
+# measured width as row stride, so for larger images it reads the next row and,
+# on the last rows, past its buffer.)
+HELICAL_N = 256
+HELICAL_CASES = [
+    ("pr_rot3", "PR", 3, 800, 500, 40, 1.3, 0.8),
+    ("in_rot2", "IN", 2, 800, 500, 42, 1.3, 0.8),
+]
+
+
+def helical_txt(template: str, inpr: str, width: int, height: int, z: float, couch: float, N: int = 512) -> str:
+    """The reference's MBFPA input txt with measured files det.raw / air.raw,
+    noise off, couch motion, the reconstruction start z and an N x N image."""
+    t = template
+    reps = [
+        ("// reconstruction size (square format) \n512 ", f"// reconstruction size (square format) \n{N} "),
+        ("// Energy Label (IN (for treatment beam) or PR (for CT))\nPR",
+         f"// Energy Label (IN (for treatment beam) or PR (for CT))\n{inpr}"),
+        ("// sinogram width \n609", f"// sinogram width \n{width}"),
+        ("// sinogram height \n800", f"// sinogram height \n{height}"),
+        ("// sinogram image data for input \n*", "// sinogram image data for input \ndet.raw"),
+        ("// Air scan image data for input // parameters are fixed\n*",
+         "// Air scan image data for input // parameters are fixed\nair.raw"),
+        ("//Couch speed [cm/rotation] // 0.4=fine, 0.8=normal, 1.2=coarse\n0.0",
+         f"//Couch speed [cm/rotation] // 0.4=fine, 0.8=normal, 1.2=coarse\n{couch}"),
+        ("// reconstruction start z [cm]\n0.0", f"// reconstruction start z [cm]\n{z}"),
+        ("0.0000537", "0.0"),
+    ]
+    for a, b in reps:
+        assert t.count(a) == 1, a
+        t = t.replace(a, b)
+    return t
+
+
+def fbp_in_input(seed: int = 61) -> np.ndarray:
+    """A seeded "IN" sinogram [800][609]: counts in [40000, 120000) with a
+    smooth bump, so that part of it falls below the reference's 50000 cut
+    (read as 1) and part of the filtered values pass its [1000, 300000] window."""
+    rng = np.random.default_rng(seed)
+    ch = np.arange(609)[None, :]
+    v = np.arange(800)[:, None]
+    base = 0.3 + 0.25 * np.exp(-((ch - 304.5 - 120 * np.sin(v * 2 * np.pi / 800)) / 90.0) ** 2)
+    return (1.0 / base * 20000.0 + rng.uniform(0, 8000, (800, 609))).astype(np.float32)

@@ -29,6 +29,15 @@ Outputs (all data, no code from the reference):
                                 the reference's measured-data loader on seeded
                                 synthetic detector / air-scan files
     (--measured-only regenerates these alone)
+  helical_<case>_npf_sha256 in kat_hashes.json
+                                npf_float.raw of the reference's whole mvct program
+                                (oracle/_ref/mvct_mbfpa_ref) on seeded measured files
+                                with couch motion: loader, helical window and npf
+    (--helical-only regenerates these alone)
+  fbp_in_mode_sha256 in kat_hashes.json
+                                the reference's FBP program (oracle/_ref/mvct_fbp_ref)
+                                on a seeded "IN" sinogram
+    (--fbp-in-only regenerates it alone)
 """
 from __future__ import annotations
 
@@ -48,6 +57,9 @@ sys.path.insert(0, ROOT)
 
 from oracle.oracle import Geom, Oracle, RefFBP, RefMBFPA  # noqa: E402
 from virtualct_amd import phantoms, spectra  # noqa: E402
+
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+from conftest import HELICAL_CASES, HELICAL_N, fbp_in_input, helical_txt  # noqa: E402
 
 SMALL_CASES = [
     # name, N, pix, V, phantom seed
@@ -306,12 +318,87 @@ def measured_goldens():
         json.dump(hashes, f, indent=1)
 
 
+def helical_goldens():
+    """SHA-256 of npf_float.raw written by the reference program compiled from
+    source, on HELICAL_CASES.  The energy loop after the npf pass runs with a
+    one-bin spectrum so that a case takes seconds."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "mvct_mbfpa_ref")
+    with open(os.path.join(HERE, "kat_hashes.json")) as f:
+        hashes = json.load(f)
+    template = open(os.path.join(HERE, "input_mbfpa_120kV.txt")).read()
+    # the phantom file the program reads after the npf pass: the central
+    # 256 x 256 of the ICRU110 phantom
+    icru = np.frombuffer(lzma.decompress(open(os.path.join(HERE, "icru110_phantom.i16.xz"), "rb").read()),
+                         np.int16).reshape(6, 512, 512)
+    c0 = (512 - HELICAL_N) // 2
+    icru = np.ascontiguousarray(icru[:, c0:c0 + HELICAL_N, c0:c0 + HELICAL_N]).tobytes()
+    for name, inpr, rot, H, W, seed, z, couch in HELICAL_CASES:
+        air, det, _ = measured_inputs(inpr, H, 80.0, rot, seed)
+        with tempfile.TemporaryDirectory() as d:
+            os.makedirs(os.path.join(d, "Spectrum"))
+            with open(os.path.join(d, "Spectrum", "PhaseSpace_photon_120kV_0cm.txt"), "w") as f:
+                f.write("Elow Ehigh weight\n0.059 0.061 1.0\n")
+            with open(os.path.join(d, "Weight_input_ICRU110.raw"), "wb") as f:
+                f.write(icru)
+            air.tofile(os.path.join(d, "air.raw"))
+            det.tofile(os.path.join(d, "det.raw"))
+            with open(os.path.join(d, "in.txt"), "w") as f:
+                f.write(helical_txt(template, inpr, W, rot * H, z, couch, HELICAL_N))
+            # The program's energy pass after the npf writes 609 channels per row
+            # (its compile-time End_Channel) into rows of the measured width, so
+            # it corrupts its heap and its exit status is not checked; the npf
+            # pass before it reads only inside its buffers at this image size
+            # (checked once with an AddressSanitizer build of the same sources).
+            r = subprocess.run([exe, "in.txt"], cwd=d, capture_output=True, text=True, timeout=1200)
+            assert "Iteration loop, 0 step" in r.stderr, r.stderr[-2000:]
+            npf = np.fromfile(os.path.join(d, "npf_float.raw"), np.float32)
+            assert npf.size == HELICAL_N ** 2 and np.count_nonzero(npf) > 1000, name
+            hashes[f"helical_{name}_npf_sha256"] = hashlib.sha256(npf.tobytes()).hexdigest()
+            print(name, "npf nonzero", int(np.count_nonzero(npf)), flush=True)
+    with open(os.path.join(HERE, "kat_hashes.json"), "w") as f:
+        json.dump(hashes, f, indent=1)
+
+
+def fbp_in_golden():
+    """SHA-256 of the image of the reference's FBP program (oracle/_ref/mvct_fbp_ref)
+    on fbp_in_input() with the shipped FBP txt switched to "IN"."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "mvct_fbp_ref")
+    with open(os.path.join(HERE, "kat_hashes.json")) as f:
+        hashes = json.load(f)
+    txt = open(os.path.join(HERE, "input_fbp_human.txt")).read()
+    a = "// Energy Label (IN : 6MV, PR : 3.5MV)\nPR"
+    assert txt.count(a) == 1
+    txt = txt.replace(a, "// Energy Label (IN : 6MV, PR : 3.5MV)\nIN")
+    with tempfile.TemporaryDirectory() as d:
+        fbp_in_input().tofile(os.path.join(d, "reprojection_float.raw"))
+        with open(os.path.join(d, "in.txt"), "w") as f:
+            f.write(txt)
+        r = subprocess.run([exe, "in.txt"], cwd=d, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        img = np.fromfile(os.path.join(d, "FBP_virtual_projection_512x512_human.raw"), np.float32)
+        assert img.size == 512 * 512
+        print("IN image nonzero", int(np.count_nonzero(img)), "max", float(img.max()), flush=True)
+    hashes["fbp_in_mode_sha256"] = hashlib.sha256(img.tobytes()).hexdigest()
+    with open(os.path.join(HERE, "kat_hashes.json"), "w") as f:
+        json.dump(hashes, f, indent=1)
+
+
 if __name__ == "__main__":
     if "--npf-only" in sys.argv:
         npf_goldens()
     elif "--measured-only" in sys.argv:
         measured_goldens()
+    elif "--helical-only" in sys.argv:
+        helical_goldens()
+    elif "--fbp-in-only" in sys.argv:
+        fbp_in_golden()
     else:
         main()
         npf_goldens()
         measured_goldens()
+        helical_goldens()
+        fbp_in_golden()

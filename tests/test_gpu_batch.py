@@ -483,3 +483,27 @@ def test_tile_stage1_equals_blocked_stage1(gpu, N, pix, B, M):
     assert torch.equal(smt.workspace(B, 0), smn.workspace(B, 0))
     smt.close()
     smn.close()
+
+
+def test_sysmat_rows_equal_reference_segment_lists(gpu, golden):
+    """The cached rows of the 56 rays whose segment caches the reference
+    dumped at the full 512^2 / 800 x 609 geometry (tests/golden/ref_segments.npz:
+    reprojection_vnum / reprojection_systemMatrixfloat per segment counter,
+    zero where a counter's segment is outside the image): the voxel sequence
+    bit for bit and every length within one float ulp."""
+    ops, batch, _ = _mods()
+    plan = ops.plan_for(_geom())
+    sm = batch.SystemMatrix(plan, transpose=False)
+    s = golden.npz("ref_segments.npz")
+    total = 0
+    for i, ray in enumerate(s["rays"]):
+        n = int(s["count"][i])
+        vox, ln = s["vox"][i, :n], s["length"][i, :n]
+        keep = (vox != 0) | (ln != 0)
+        gv, gl = sm.row(int(ray))
+        np.testing.assert_array_equal(gv.astype(np.int64), vox[keep].astype(np.int64), err_msg=f"ray {ray}")
+        ulp = np.spacing(np.abs(ln[keep]))
+        assert np.all(np.abs(gl.astype(np.float64) - ln[keep]) <= ulp), f"ray {ray}"
+        total += gv.size
+    assert total > 10000
+    sm.close()

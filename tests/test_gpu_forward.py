@@ -271,3 +271,33 @@ def test_odd_material_count_u16(gpu, golden, M):
     dirty[..., M] = 12345  # the high half of word M // 2
     L2 = ops.trace_paths(plan, dirty, dt, M)
     assert torch.equal(L2, L)
+
+
+def test_step_bound_overflow_is_reported(gpu, golden):
+    """A plan whose marching step bound is below what its rays need: the
+    tracer and the atomic adjoint give up on those rays and exit normally,
+    the device flag makes the call raise VCT_ERR_INTERNAL, and the flag is
+    clear afterwards."""
+    import torch
+    ops = _ops()
+    from virtualct_amd import VctError, _lib
+    plan = ops.Plan(_geom(128, 0.35, 160))  # own plan: the option is per plan
+    ph = golden.npz("ref_small.npz")["n128_phantom"]
+    pk, dt = ops.pack_densities(plan, _t(ph, gpu))
+    good = ops.trace_paths(plan, pk, dt, 6)
+    plan.set_option(_lib.OPT_MAX_STEPS, 8)
+    with pytest.raises(VctError) as e:
+        ops.trace_paths(plan, pk, dt, 6)
+    assert e.value.status == _lib.ERR_INTERNAL
+    with pytest.raises(VctError) as e:
+        ops.mbfpa_forward(plan, None, torch.ones((6, 2), device=gpu), torch.ones(2, device=gpu) / 2,
+                          packed=(pk, dt))
+    assert e.value.status == _lib.ERR_INTERNAL
+    w = torch.ones((160, 609, 8), device=gpu)
+    with pytest.raises(VctError) as e:
+        ops.trace_adjoint(plan, w, 6)
+    assert e.value.status == _lib.ERR_INTERNAL
+    ops.check_errors()  # cleared by the raising check
+    plan.set_option(_lib.OPT_MAX_STEPS, 0)
+    assert torch.equal(ops.trace_paths(plan, pk, dt, 6), good)
+    plan.close()

@@ -4,10 +4,13 @@ loadProjectionData in test_oracle.py), and the MBFPA driver reading the
 detector and air-scan files its input txt names."""
 from __future__ import annotations
 
+import os
+import shutil
+
 import numpy as np
 import pytest
 
-from conftest import MEASURED_CASES, measured_inputs, rel_max
+from conftest import HELICAL_CASES, HELICAL_N, MEASURED_CASES, helical_txt, measured_inputs, rel_max
 from oracle.oracle import Geom
 from test_drivers import _built, _mbfpa_dir, run
 
@@ -102,10 +105,38 @@ def test_mbfpa_driver_npf_from_measured_files(gpu, tmp_path, golden, oracle):
     vol, _, _ = oracle.load_measured(0, rot, H, W, 1, air, raw)
     np.testing.assert_array_equal(np.fromfile(d / "projections.raw", np.float32).reshape(rot * H, W), vol)
     win = oracle.helical_window(vol, rot, H, W)
-    ref = oracle.lengthproj_ref(Geom(W=W), win, 1.3, 0.8)
+    # the window holds 500 of the 609-channel detector's channels
+    ref = oracle.lengthproj_ref(Geom(), win, 1.3, 0.8)
     npf = np.fromfile(d / "npf_float.raw", np.float32).reshape(512, 512)
     assert rel_max(npf, ref) < 1e-5
     assert np.fromfile(d / "reprojection_float.raw", np.float32).size == H * W
+
+
+@pytest.mark.parametrize("case", HELICAL_CASES, ids=[c[0] for c in HELICAL_CASES])
+def test_mbfpa_driver_helical_npf_against_reference_program(gpu, tmp_path, golden, oracle, case):
+    """The driver on the measured files with couch motion that the reference's
+    whole MBFPA program was run on (256^2, 500 channels): npf_float.raw within
+    1e-5 of the oracle chain, which equals the reference program's output bit
+    for bit (test_oracle.py)."""
+    _built()
+    name, inpr, rot, H, W, seed, z, couch = case
+    d = tmp_path / name
+    os.makedirs(d / "Spectrum")
+    shutil.copyfile(golden.path("spectrum_120kV.txt"), d / "Spectrum" / "PhaseSpace_photon_120kV_0cm.txt")
+    c0 = (512 - HELICAL_N) // 2
+    golden.icru110()[:, c0:c0 + HELICAL_N, c0:c0 + HELICAL_N].copy().tofile(d / "Weight_input_ICRU110.raw")
+    air, det, raw = measured_inputs(inpr, H, 80.0, rot, seed)
+    air.tofile(d / "air.raw")
+    det.tofile(d / "det.raw")
+    (d / "in.txt").write_text(helical_txt(open(golden.path("input_mbfpa_120kV.txt")).read(), inpr, W, rot * H, z,
+                                          couch, HELICAL_N))
+    print(f"case {name}", flush=True)
+    r = run("mvct_mbfpa", ["in.txt"], d)
+    assert r.returncode == 0, r.stderr
+    vol, _, _ = oracle.load_measured(int(inpr == "IN"), rot, H, W, 1, air, raw)
+    ref = oracle.lengthproj_ref(Geom(N=HELICAL_N, pix=0.1), oracle.helical_window(vol, rot, H, W), z, couch)
+    npf = np.fromfile(d / "npf_float.raw", np.float32).reshape(HELICAL_N, HELICAL_N)
+    assert rel_max(npf, ref) < 1e-5
 
 
 def test_mbfpa_driver_measured_stop(gpu, tmp_path, golden):

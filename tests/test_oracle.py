@@ -17,7 +17,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import MEASURED_CASES, measured_inputs, rel_l2, rel_max
+from conftest import HELICAL_CASES, HELICAL_N, MEASURED_CASES, fbp_in_input, measured_inputs, rel_l2, rel_max
 from oracle.oracle import Geom
 
 SMALL = {"n64": (64, 0.7, 80), "n128": (128, 0.35, 160)}
@@ -339,3 +339,35 @@ def test_helical_window_closed_forms(oracle):
             else:
                 expect = c[0] + w * (c[1] - c[0]) if n < H // 2 else c[0]
             np.testing.assert_allclose(win[n], expect, rtol=2 ** -23, atol=0)
+
+
+@pytest.mark.parametrize("case", HELICAL_CASES, ids=[c[0] for c in HELICAL_CASES])
+def test_helical_npf_matches_reference_program(oracle, golden, case):
+    """Loader, helical window and npf of the oracle on seeded measured files
+    with couch motion (500 of the detector's 609 channels) reproduce
+    npf_float.raw of the reference's whole MBFPA program bit for bit
+    (SHA-256 written by make_golden.py from oracle/_ref/mvct_mbfpa_ref):
+    pins the helical blend of M/main.cpp:352-371 and the npf's use of the
+    detector's channel pitch for a narrower window."""
+    name, inpr, rot, H, W, seed, z, couch = case
+    air, det, raw = measured_inputs(inpr, H, 80.0, rot, seed)
+    vol, _, _ = oracle.load_measured(int(inpr == "IN"), rot, H, W, 1, air, raw)
+    win = oracle.helical_window(vol, rot, H, W)
+    npf = oracle.lengthproj_ref(Geom(N=HELICAL_N, pix=0.1), win, z, couch)
+    assert npf is not None and np.count_nonzero(npf) > 1000
+    assert hashlib.sha256(npf.tobytes()).hexdigest() == golden.hashes()[f"helical_{name}_npf_sha256"]
+    # a window as wide as the detector gives another image: the pitch is the detector's
+    wide = oracle.lengthproj_ref(Geom(N=HELICAL_N, pix=0.1, W=W), win, z, couch)
+    assert not np.array_equal(wide, npf)
+
+
+def test_fbp_in_mode_matches_reference_program(oracle, golden):
+    """The FBP program's "IN" path: values below 50000 read as 1
+    (F/projectionDataTOMO.cpp:274-292), then the filter keeps values in
+    [1000, 300000] (F/filterSinogramTOMO.cpp:250-255).  The oracle equals the
+    reference program's image bit for bit (SHA-256 from oracle/_ref/mvct_fbp_ref)."""
+    p = fbp_in_input()
+    cut = np.where(p < 50000.0, np.float32(1.0), p)
+    img = oracle.fbp_ref(Geom(), cut, 2, in_mode=True)
+    assert np.count_nonzero(img) > 100000
+    assert hashlib.sha256(img.tobytes()).hexdigest() == golden.hashes()["fbp_in_mode_sha256"]

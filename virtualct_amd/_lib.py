@@ -19,6 +19,8 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of mem
 DENS_U16, DENS_F32 = 0, 1
 FILTER_RAMLAK, FILTER_SHEPP_LOGAN = 1, 2
 OPT_EXACT_TRACE = 1
+OPT_MAX_STEPS = 2
+ERR_INTERNAL = 6
 SYSMAT_TRANSPOSE = 1
 SYSMAT_BLOCKED = 2
 SYSMAT_NO_TILE = 4
@@ -60,6 +62,7 @@ _SIGS = {
     "vct_abi_version": (C.c_int, []),
     "vct_init": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "vct_terminate": (C.c_int, [_P]),
+    "vct_check_errors": (C.c_int, [_P, _P]),
     "vct_plan_create": (C.c_int, [_P, C.POINTER(Geometry), C.POINTER(_P)]),
     "vct_plan_destroy": (C.c_int, [_P]),
     "vct_plan_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
@@ -75,8 +78,11 @@ _SIGS = {
     "vct_mbfpa_forward": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_double,
                                      C.c_uint64, C.c_uint32, _P, _P, _P]),
     "vct_filter_sinogram": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "vct_filter_workspace_bytes": (C.c_size_t, [_P, C.c_int]),
+    "vct_filter_sinogram_ws": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
     "vct_backproject": (C.c_int, [_P, _P, C.c_int, _P, _P]),
     "vct_lengthproj": (C.c_int, [_P, _P, C.c_double, C.c_double, C.c_double, _P, _P]),
+    "vct_lengthproj_window": (C.c_int, [_P, _P, C.c_int, C.c_double, C.c_double, C.c_double, _P, _P]),
     "vct_beam_hardening": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, _P, _P]),
     "vct_filter_sinogram_bhc": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P, _P]),
     "vct_minmax": (C.c_int, [_P, C.c_int, _P, _P]),
@@ -86,6 +92,8 @@ _SIGS = {
     "vct_filter_sinogram_host": (C.c_int, [_P, C.POINTER(Geometry), _P, C.c_int, C.c_int, _P]),
     "vct_reconstruct_image_host": (C.c_int, [_P, C.POINTER(Geometry), _P, _P]),
     "vct_lengthproj_host": (C.c_int, [_P, C.POINTER(Geometry), _P, C.c_double, C.c_double, C.c_double, _P]),
+    "vct_lengthproj_window_host": (C.c_int, [_P, C.POINTER(Geometry), _P, C.c_int, C.c_double, C.c_double,
+                                             C.c_double, _P]),
 }
 
 
@@ -116,6 +124,7 @@ _SIGS.update({
     "vct_sysmat_build": (C.c_int, [_P, C.c_int, C.POINTER(_P), _P]),
     "vct_sysmat_destroy": (C.c_int, [_P]),
     "vct_sysmat_segments": (C.c_int64, [_P]),
+    "vct_sysmat_row": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, C.POINTER(C.c_int), _P]),
     "vct_sysmat_bytes": (C.c_size_t, [_P]),
     "vct_sysmat_workspace": (C.c_size_t, [_P, C.c_int]),
     "vct_batch_bb_workspace": (C.c_size_t, [C.c_int]),

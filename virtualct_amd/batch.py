@@ -54,6 +54,17 @@ class SystemMatrix:
         self.blocked = blocked
         self._work: dict[tuple[int, int], torch.Tensor] = {}
 
+    def row(self, ray: int):
+        """(voxels uint32, lengths float32) of one ray's cached segments, in march order."""
+        import numpy as np
+        n = C.c_int(0)
+        _lib.lib().vct_sysmat_row(self.handle, ray, None, None, 0, C.byref(n), _stream(torch.device("cuda", self.plan.device)))
+        vox = np.zeros(max(n.value, 1), np.uint32)
+        ln = np.zeros(max(n.value, 1), np.float32)
+        call("vct_sysmat_row", self.handle, ray, C.c_void_p(vox.ctypes.data), C.c_void_p(ln.ctypes.data), vox.size,
+             C.byref(n), _stream(torch.device("cuda", self.plan.device)))
+        return vox[: n.value], ln[: n.value]
+
     def workspace(self, B: int, slot: int = 0) -> torch.Tensor | None:
         """Partial-sum workspace of the blocked gathers for B phantoms (one
         per slot: gathers that may overlap use different slots); None when
@@ -320,10 +331,24 @@ class BatchForwardFBP:
         blocked: gather through the blocked (tile-regrouped) system matrix."""
         self.att = att.contiguous()
         self.pdf = pdf.contiguous()
+        if att.dim() != 3 or pdf.dim() != 2 or pdf.shape != (att.shape[0], att.shape[2]):
+            raise ValueError(f"attenuation [S,M,NE] / pdf [S,NE] expected, got {tuple(att.shape)} / {tuple(pdf.shape)}")
         self.S = att.shape[0]
         self.M = att.shape[1]
         self.sigma, self.seed, self.filt = sigma, seed, filt
         self.groups: list[Group] = []
+        if not groups:
+            raise ValueError("no groups")
+        # one stacked filter GEMM serves every group, and the filter depends on
+        # the detector and source geometry: only the image grid may differ
+        fixed = [f for f, _ in groups[0][0]._fields_ if f not in ("recon_size", "pixel_cm", "reserved0")]
+        for geom, dens in groups:
+            diff = [f for f in fixed if getattr(geom, f) != getattr(groups[0][0], f)]
+            if diff:
+                raise ValueError(f"groups may differ only in recon_size and pixel_cm; {geom} differs in {diff}")
+            if dens.dim() != 4 or dens.shape[1] != self.M or dens.shape[2:] != (geom.recon_size, geom.recon_size):
+                raise ValueError(f"densities must be [B,{self.M},{geom.recon_size},{geom.recon_size}], "
+                                 f"got {tuple(dens.shape)}")
         for geom, dens in groups:
             plan = ops.plan_for(geom)
             sm = SystemMatrix(plan, transpose=False, blocked=blocked)
@@ -338,6 +363,11 @@ class BatchForwardFBP:
             raise ValueError("all groups must share the detector geometry (views x channels)")
         self.lanes = lanes or lanes_for(max(g.B for g in self.groups))
         self.blocks = blocks
+        # the filter's packed operand lives in a buffer of this run, not in the
+        # context scratch, so a replay of the captured graph shares nothing
+        # with later eager calls
+        self.n_sino = sum(g.B for g in self.groups) * self.S
+        self.filter_work = ops.filter_workspace(self.groups[0].plan, self.n_sino)
         self.graph = None
         self.out = None
 
@@ -354,7 +384,7 @@ class BatchForwardFBP:
             sid += B * S
             sinos.append(sino)
         stack = torch.cat(sinos, 0) if len(sinos) > 1 else sinos[0]
-        fltd = ops.filter_sinogram(gs[0].plan, stack, self.filt)
+        fltd = ops.filter_sinogram(gs[0].plan, stack, self.filt, work=self.filter_work)
         imgs = []
         off = 0
         for g, s in zip(gs, sinos):

@@ -26,6 +26,7 @@
 // soft failures (missing info file, usage), 1 on a library error.
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -253,7 +254,14 @@ int main(int argc, char* argv[])
         }
     }
     if (!win.empty()) {
-        st = vct_lengthproj_host(ctx, &g, win.data(), zstart, in.couch_speed, 57.6, npf.data());
+        // the window keeps W channels of the 609-channel detector, whose pitch
+        // and isocentre place the hits (End_Channel, isocenter_channel,
+        // M/physParamsTOMO.h:43-47)
+        vct_geometry gd = g;
+        vct_geometry det;
+        vct_geometry_default(&det);
+        gd.n_channels = std::max(W, det.n_channels);
+        st = vct_lengthproj_window_host(ctx, &gd, win.data(), W, zstart, in.couch_speed, 57.6, npf.data());
         if (st != VCT_OK) return stop("lengthproj", st);
     }
     if (FILE* fp = std::fopen("npf_float.raw", "wb")) {

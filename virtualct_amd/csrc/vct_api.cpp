@@ -71,6 +71,7 @@ TraceArgs make_trace_args(const vct_plan* p)
     // A ray crosses at most 2N/sqrt(2)+1 major planes plus the integer-cm
     // start plane; the bound below is generous and only guards the loop.
     a.max_steps = 4 * g.recon_size + (int)(4.0 / g.pixel_cm) + 64;
+    if (p->max_steps > 0) a.max_steps = p->max_steps;
     a.ray_offset = 0;
     a.n_rays = g.n_views * g.n_channels;
     a.exact = p->exact_trace;
@@ -119,7 +120,13 @@ static vct_status const_map(const vct_geometry* g, int filter, double* C)
     return VCT_OK;
 }
 
-vct_status plan_filter_matrix(vct_plan* p, int filter, float** d_bpk)
+// The filter operand B = C * dbeta, packed for the MFMA GEMM, and its
+// Toeplitz generator.  Rows m of C that prepareConstMap does not renormalise
+// (20 < m < W - 20) depend on m - k only, so toep[m - k + Kp - 1] =
+// (float)(C[m][k] * dbeta) for those rows (the same double expression as the
+// packed value); toep covers m - k in [-(Kp - 1), Wp - 1] and is 0 outside
+// the matrix (padded k, whose A operand is 0).
+vct_status plan_filter_matrix(vct_plan* p, int filter, float** d_bpk, float** d_toep)
 {
     if (filter != VCT_FILTER_RAMLAK && filter != VCT_FILTER_SHEPP_LOGAN)
         return set_error(VCT_ERR_ARG, "filter %d is not 1 (Ram-Lak) or 2 (Shepp-Logan)", filter);
@@ -134,11 +141,32 @@ vct_status plan_filter_matrix(vct_plan* p, int filter, float** d_bpk)
                 const size_t idx = ((((size_t)(m >> 5) * (p->Kp / 8) + ((k >> 1) >> 2)) * 64 + ((m & 31) + 32 * (k & 1))) * 4) + ((k >> 1) & 3);
                 bpk[idx] = (float)(C[(size_t)m * W + k] * scale);
             }
+        // generator from the inner rows: every difference d = m - k in
+        // [-(W-1), W-1] occurs in some inner row when W > 41
+        std::vector<float> toep((size_t)p->Wp + p->Kp, 0.f);
+        std::vector<char> seen(toep.size(), 0);
+        for (int m = 21; m < W - 20; m++)
+            for (int k = 0; k < W; k++) {
+                const size_t i = (size_t)(m - k + p->Kp - 1);
+                const float v = (float)(C[(size_t)m * W + k] * scale);
+                if (seen[i] && toep[i] != v)
+                    return set_error(VCT_ERR_INTERNAL, "filter matrix row %d is not Toeplitz", m);
+                toep[i] = v;
+                seen[i] = 1;
+            }
         void* d = nullptr;
+        void* dt = nullptr;
         VCT_CHECK(plan_upload(bpk.data(), bpk.size() * sizeof(float), &d));
+        vct_status st = plan_upload(toep.data(), toep.size() * sizeof(float), &dt);
+        if (st != VCT_OK) {
+            (void)hipFree(d);
+            return st;
+        }
         p->d_bpk[filter] = (float*)d;
+        p->d_toep[filter] = (float*)dt;
     }
     *d_bpk = p->d_bpk[filter];
+    *d_toep = p->d_toep[filter];
     return VCT_OK;
 }
 
@@ -158,7 +186,8 @@ static vct_status check_err_flag(vct_ctx* ctx, hipStream_t s)
     if (h) {
         VCT_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(int), s));
         VCT_HIP(hipStreamSynchronize(s));
-        return set_error(VCT_ERR_INTERNAL, "a ray exceeded its marching step bound");
+        return set_error(VCT_ERR_INTERNAL, "a kernel reported error flags %#x (1: a ray exceeded its marching "
+                                           "step bound; 2: a system-matrix row did not match its count)", h);
     }
     return VCT_OK;
 }
@@ -167,16 +196,16 @@ static vct_status check_err_flag(vct_ctx* ctx, hipStream_t s)
 
 using namespace vct;
 
+// Grow-only: a larger request allocates a new buffer and retires the old one
+// (freed at vct_terminate), so pointers already handed to kernels, including
+// kernels recorded in a HIP graph, stay valid.
 vct_status vct_ctx::ensure_scratch(size_t bytes)
 {
     if (bytes <= scratch_bytes) return VCT_OK;
-    if (scratch) {
-        VCT_HIP(hipStreamSynchronize(stream));
-        VCT_HIP(hipFree(scratch));
-        scratch = nullptr;
-        scratch_bytes = 0;
-    }
-    VCT_HIP(hipMalloc(&scratch, bytes));
+    void* fresh = nullptr;
+    VCT_HIP(hipMalloc(&fresh, bytes));
+    if (scratch) retired.push_back(scratch);
+    scratch = fresh;
     scratch_bytes = bytes;
     return VCT_OK;
 }
@@ -247,6 +276,14 @@ vct_status vct_init(int device, vct_ctx** out)
     return VCT_OK;
 }
 
+vct_status vct_check_errors(vct_ctx* c, void* stream)
+{
+    clear_error();
+    if (!c) return set_error(VCT_ERR_ARG, "ctx is NULL");
+    VCT_HIP(hipSetDevice(c->device));
+    return check_err_flag(c, (hipStream_t)stream);
+}
+
 vct_status vct_terminate(vct_ctx* c)
 {
     clear_error();
@@ -254,6 +291,7 @@ vct_status vct_terminate(vct_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->scratch) (void)hipFree(c->scratch);
+    for (void* r : c->retired) (void)hipFree(r);
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -317,6 +355,10 @@ vct_status vct_plan_set_option(vct_plan* p, int option, int value)
     if (!p) return set_error(VCT_ERR_ARG, "plan is NULL");
     switch (option) {
     case VCT_OPT_EXACT_TRACE: p->exact_trace = value ? 1 : 0; return VCT_OK;
+    case VCT_OPT_MAX_STEPS:
+        if (value < 0) return set_error(VCT_ERR_ARG, "step bound %d must be >= 0", value);
+        p->max_steps = value;
+        return VCT_OK;
     default: return set_error(VCT_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -330,6 +372,8 @@ vct_status vct_plan_destroy(vct_plan* p)
     for (double* d : dd)
         if (d) (void)hipFree(d);
     for (float* b : p->d_bpk)
+        if (b) (void)hipFree(b);
+    for (float* b : p->d_toep)
         if (b) (void)hipFree(b);
     delete p;
     return VCT_OK;
@@ -525,23 +569,31 @@ vct_status vct_mbfpa_forward(vct_plan* p, const void* d_packed, int dens_type, i
     return vct_detector(p, raw, stats, S, sigma, seed, stream_id, d_sino, s);
 }
 
+size_t vct_filter_workspace_bytes(const vct_plan* p, int n_sino)
+{
+    if (!p || n_sino < 1) return 0;
+    const size_t R = (size_t)n_sino * p->g.n_views;
+    return (R + 31) / 32 * 32 * p->Kp * sizeof(float);
+}
+
 static vct_status filter_impl(vct_plan* p, const float* d_proj, int n_sino, int filter, int in_mode, int bhc,
-                              double alpha, double beta, float* d_fltd, void* stream)
+                              double alpha, double beta, float* d_fltd, void* d_work, void* stream)
 {
     clear_error();
     if (!p || !d_proj || !d_fltd) return set_error(VCT_ERR_ARG, "NULL argument");
     if (n_sino < 1) return set_error(VCT_ERR_ARG, "n_sino must be >= 1");
-    float* bpk = nullptr;
-    VCT_CHECK(plan_filter_matrix(p, filter, &bpk));
+    float *bpk = nullptr, *toep = nullptr;
+    VCT_CHECK(plan_filter_matrix(p, filter, &bpk, &toep));
     const long long R = (long long)n_sino * p->g.n_views;
     if (R > (1LL << 26)) return set_error(VCT_ERR_ARG, "too many sinogram rows");
-    const size_t Rp = (size_t)(R + 31) / 32 * 32;
-    const size_t need = Rp * p->Kp * sizeof(float);
-    VCT_CHECK(p->ctx->ensure_scratch(need));
+    if (!d_work) {
+        VCT_CHECK(p->ctx->ensure_scratch(vct_filter_workspace_bytes(p, n_sino)));
+        d_work = p->ctx->scratch;
+    }
     FilterPrepArgs pa{};
     pa.proj = d_proj;
     pa.coeff = p->d_coeff;
-    pa.apk = (float*)p->ctx->scratch;
+    pa.apk = (float*)d_work;
     pa.R = (int)R;
     pa.W = p->g.n_channels;
     pa.Kp = p->Kp;
@@ -551,25 +603,36 @@ static vct_status filter_impl(vct_plan* p, const float* d_proj, int n_sino, int 
     FilterGemmArgs ga{};
     ga.apk = pa.apk;
     ga.bpk = bpk;
+    ga.toep = toep;
     ga.out = d_fltd;
     ga.R = (int)R;
     ga.W = p->g.n_channels;
     ga.Kp = p->Kp;
     ga.in_mode = in_mode ? 1 : 0;
+    // rows m <= 20 and m >= W - 20 are renormalised (F/filterSinogramTOMO.cpp:114-123)
+    ga.edge_lo = 20;
+    ga.edge_hi = p->g.n_channels - 20;
     return launch_filter(pa, ga, pick_stream(p, stream));
 }
 
 vct_status vct_filter_sinogram(vct_plan* p, const float* d_proj, int n_sino, int filter, int in_mode, float* d_fltd,
                                void* stream)
 {
-    return filter_impl(p, d_proj, n_sino, filter, in_mode, 0, 0.0, 0.0, d_fltd, stream);
+    return filter_impl(p, d_proj, n_sino, filter, in_mode, 0, 0.0, 0.0, d_fltd, nullptr, stream);
+}
+
+vct_status vct_filter_sinogram_ws(vct_plan* p, const float* d_proj, int n_sino, int filter, int in_mode,
+                                  float* d_fltd, void* d_work, void* stream)
+{
+    if (!d_work) return set_error(VCT_ERR_ARG, "workspace is NULL");
+    return filter_impl(p, d_proj, n_sino, filter, in_mode, 0, 0.0, 0.0, d_fltd, d_work, stream);
 }
 
 vct_status vct_filter_sinogram_bhc(vct_plan* p, const float* d_proj, int n_sino, int filter, int in_mode, double alpha,
                                    double beta, float* d_fltd, void* stream)
 {
     if (!std::isfinite(alpha) || !std::isfinite(beta)) return set_error(VCT_ERR_ARG, "alpha/beta must be finite");
-    return filter_impl(p, d_proj, n_sino, filter, in_mode, 1, alpha, beta, d_fltd, stream);
+    return filter_impl(p, d_proj, n_sino, filter, in_mode, 1, alpha, beta, d_fltd, nullptr, stream);
 }
 
 vct_status vct_beam_hardening(const float* d_in, int n, double alpha, double beta, float* d_out, void* stream)
@@ -629,8 +692,18 @@ vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* 
 vct_status vct_lengthproj(vct_plan* p, const float* d_proj, double z_cm, double couch_speed_cm, double iso_det_cm,
                           float* d_npf, void* stream)
 {
+    if (!p) return set_error(VCT_ERR_ARG, "NULL argument");
+    return vct_lengthproj_window(p, d_proj, p->g.n_channels, z_cm, couch_speed_cm, iso_det_cm, d_npf, stream);
+}
+
+vct_status vct_lengthproj_window(vct_plan* p, const float* d_proj, int proj_width, double z_cm, double couch_speed_cm,
+                                 double iso_det_cm, float* d_npf, void* stream)
+{
     clear_error();
     if (!p || !d_proj || !d_npf) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (proj_width < 2 || proj_width > p->g.n_channels)
+        return set_error(VCT_ERR_ARG, "window width %d outside 2..%d (the detector's channels)", proj_width,
+                         p->g.n_channels);
     const vct_geometry& g = p->g;
     const int V = g.n_views;
     // view window of M/lengthproj_TOMO.cpp:35-49 (UsedReconAngle = 2)
@@ -650,6 +723,7 @@ vct_status vct_lengthproj(vct_plan* p, const float* d_proj, double z_cm, double 
     a.npf = d_npf;
     a.N = g.recon_size;
     a.W = g.n_channels;
+    a.pw = proj_width;
     a.V = V;
     a.base = start - (start / V) * V;
     a.count = V / 2 * 2;
@@ -823,11 +897,21 @@ vct_status vct_reconstruct_image_host(vct_ctx* ctx, const vct_geometry* g, const
 vct_status vct_lengthproj_host(vct_ctx* ctx, const vct_geometry* g, const float* proj, double z_cm,
                                double couch_speed_cm, double iso_det_cm, float* npf)
 {
+    if (!g) return set_error(VCT_ERR_ARG, "NULL argument");
+    return vct_lengthproj_window_host(ctx, g, proj, g->n_channels, z_cm, couch_speed_cm, iso_det_cm, npf);
+}
+
+vct_status vct_lengthproj_window_host(vct_ctx* ctx, const vct_geometry* g, const float* proj, int proj_width,
+                                      double z_cm, double couch_speed_cm, double iso_det_cm, float* npf)
+{
     clear_error();
     if (!ctx || !g || !proj || !npf) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (proj_width < 2 || proj_width > g->n_channels)
+        return set_error(VCT_ERR_ARG, "window width %d outside 2..%d (the detector's channels)", proj_width,
+                         g->n_channels);
     vct_plan* p = nullptr;
     VCT_CHECK(vct_plan_create(ctx, g, &p));
-    const size_t nr = (size_t)(g->n_views / 2 * 2) * g->n_channels;
+    const size_t nr = (size_t)(g->n_views / 2 * 2) * proj_width;
     const size_t NN = (size_t)g->recon_size * g->recon_size;
     hipStream_t s = ctx->stream;
     float *d_p = nullptr, *d_n = nullptr;
@@ -838,7 +922,7 @@ vct_status vct_lengthproj_host(vct_ctx* ctx, const vct_geometry* g, const float*
     hip(hipMalloc(&d_p, nr * sizeof(float)), "hipMalloc");
     hip(hipMalloc(&d_n, NN * sizeof(float)), "hipMalloc");
     if (st == VCT_OK) hip(hipMemcpyAsync(d_p, proj, nr * sizeof(float), hipMemcpyHostToDevice, s), "copy");
-    if (st == VCT_OK) st = vct_lengthproj(p, d_p, z_cm, couch_speed_cm, iso_det_cm, d_n, s);
+    if (st == VCT_OK) st = vct_lengthproj_window(p, d_p, proj_width, z_cm, couch_speed_cm, iso_det_cm, d_n, s);
     if (st == VCT_OK) hip(hipMemcpyAsync(npf, d_n, NN * sizeof(float), hipMemcpyDeviceToHost, s), "copy");
     if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
     if (d_p) (void)hipFree(d_p);

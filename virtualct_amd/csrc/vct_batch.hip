@@ -60,7 +60,7 @@ struct vct_blocked {
 // length (longest first) in slices of 64, whose entries are interleaved so
 // that lane l of a wave reads entry k of run l at slice_ptr + 64 k + l.  An
 // entry holds its voxel's index inside the tile (ly * 64 + lx) and the
-// length; padding entries are (0, 0.0f).
+// length; padding entries are (kTilePadVoxel, 0.0f), which read zeroed LDS.
 struct vct_tiled {
     int ntiles = 0, nslices = 0, kmax = 0, ntx = 0;
     uint2* ell = nullptr;            // [sum slice_len * 64]
@@ -917,6 +917,16 @@ __global__ void k_zero_f32(float* p, int n)
 
 constexpr int kTileThreads = 1024;  // one workgroup per CU (LDS)
 constexpr int kTileUnroll = 8;      // entries in flight per lane
+// Tile-local voxel code of padding entries: "row 64", the zeroed LDS slots
+// after the tile, so padding adds exact zeros whatever the tile holds (an inf
+// or NaN density included).
+constexpr uint32_t kTilePadVoxel = 64u * 64u;
+
+__global__ __launch_bounds__(256) void k_fill_pad(uint2* __restrict__ p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_uint2(kTilePadVoxel, 0u);
+}
 
 // Build: per stage-1 run, sort key tile << 16 | (0xffff - length) (longest
 // first within a tile), and the first run of each tile (the runs are in tile
@@ -1016,13 +1026,14 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_tile(TileArgs a)
     constexpr int MP = (M + 1) & ~1;  // floats per voxel in LDS
     constexpr int ROW = 64 * MP + 2;  // floats per tile row
     constexpr int U = kTileUnroll;
-    __shared__ float lds[64 * ROW];
+    __shared__ float lds[64 * ROW + 8];  // the tile, then 8 zeros for padding entries
     const int g = blockIdx.x, xcd = g % kXcds, slot = g / kXcds;
     const int k = slot / a.B, b = slot % a.B;
     if (k >= a.kmax) return;  // the whole workgroup
     const int t = a.xcd_tiles[xcd * a.kmax + k];
     if (t < 0) return;
     const int tx0 = (t % a.ntx) * 64, ty0 = (t / a.ntx) * 64;
+    if (threadIdx.x < 8) lds[64 * ROW + threadIdx.x] = 0.f;
     for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
         const int ly = i >> 6, lx = i & 63, gy = ty0 + ly, gx = tx0 + lx;
         float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -1041,11 +1052,13 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_tile(TileArgs a)
         for (int m = 0; m < MP; m++) acc[m] = 0.0;
         uint2 q[U], nq[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) q[u] = u < len ? e[u * 64] : make_uint2(0u, 0u);
+        for (int u = 0; u < U; u++) q[u] = u < len ? e[u * 64] : make_uint2(kTilePadVoxel, 0u);
         for (int j = 0; j < len; j += U) {
-            // padding entries past len are (0, 0.0f): they add exact zeros
+            // padding entries (runs shorter than the slice, and past len) are
+            // (kTilePadVoxel, 0.0f): zero density times zero length
 #pragma unroll
-            for (int u = 0; u < U; u++) nq[u] = j + U + u < len ? e[(j + U + u) * 64] : make_uint2(0u, 0u);
+            for (int u = 0; u < U; u++)
+                nq[u] = j + U + u < len ? e[(j + U + u) * 64] : make_uint2(kTilePadVoxel, 0u);
             float2 f[U][MP / 2];
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -1313,7 +1326,7 @@ static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tile
         if (st == VCT_OK) {
             hip(hipMemcpyAsync(t.xcd_tiles, xt.data(), 4 * xt.size(), hipMemcpyHostToDevice, s), "copy");
             hip(hipMemcpyAsync(t.slice_ptr, sptr.data(), 4 * (size_t)nslices, hipMemcpyHostToDevice, s), "copy");
-            hip(hipMemsetAsync(t.ell, 0, 8 * (size_t)total + 16, s), "memset");
+            hipLaunchKernelGGL(k_fill_pad, dim3((unsigned)((total + 2 + 255) / 256)), blk, 0, s, t.ell, (size_t)total + 2);
             hip(hipMemsetAsync(t.slice_runs, 0xff, 4 * 64 * (size_t)nslices, s), "memset");
             hipLaunchKernelGGL(k_zero_f32, dim3(1), blk, 0, s, (float*)err, 1);
             hipLaunchKernelGGL(k_tile_fill, gr, blk, 0, s, f.ptr1, f.ent1, key_s, order, R, tile_begin, t.tile_slices,
@@ -1468,6 +1481,32 @@ vct_status vct_sysmat_destroy(vct_sysmat* m)
 }
 
 int64_t vct_sysmat_segments(const vct_sysmat* m) { return m ? m->nseg : -1; }
+
+vct_status vct_sysmat_row(const vct_sysmat* m, int ray, uint32_t* h_voxel, float* h_length, int capacity, int* n_out,
+                          void* stream)
+{
+    clear_error();
+    if (!m || !n_out) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (ray < 0 || ray >= m->nrays) return set_error(VCT_ERR_ARG, "ray %d outside 0..%d", ray, m->nrays - 1);
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t be[2] = {0u, 0u};
+    VCT_HIP(hipMemcpyAsync(be, m->row_ptr + ray, sizeof(be), hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipStreamSynchronize(s));
+    const int n = (int)(be[1] - be[0]);
+    *n_out = n;
+    if (n > capacity) return set_error(VCT_ERR_ARG, "row %d holds %d segments, capacity %d", ray, n, capacity);
+    if (n == 0) return VCT_OK;
+    if (!h_voxel || !h_length) return set_error(VCT_ERR_ARG, "NULL output");
+    std::vector<uint2> e(n);
+    VCT_HIP(hipMemcpyAsync(e.data(), m->csr + be[0], sizeof(uint2) * n, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++) {
+        h_voxel[i] = e[i].x;
+        uint32_t bits = e[i].y;
+        std::memcpy(&h_length[i], &bits, sizeof(float));
+    }
+    return VCT_OK;
+}
 
 size_t vct_sysmat_bytes(const vct_sysmat* m)
 {

@@ -5,13 +5,14 @@
 //                  written straight into the register layout of the MFMA A
 //                  operand (zero padded to Kp).
 //   k_filter_gemm  fltd = q . C'^T with v_mfma_f32_32x32x2f32, where
-//                  C'[m][k] = C[m][k] * dbeta is the reference's dense filter
+//                  C'[m][k] = C[m][k] * dbeta is the reference's filter
 //                  matrix (prepareConstMap, F/filterSinogramTOMO.cpp:9-138;
-//                  Shepp-Logan or Ram-Lak with edge-row renormalisation, so
-//                  it is stored dense rather than generated as a Toeplitz
-//                  stencil).  One wave owns a 32x32 output tile and half of
-//                  K; operands are pre-packed so each lane issues one 16-byte
-//                  load per four MFMAs; the two halves meet in LDS.
+//                  Shepp-Logan or Ram-Lak).  Its rows are Toeplitz except the
+//                  41 edge rows that prepareConstMap renormalises, so most
+//                  column tiles read C' from an LDS copy of its Toeplitz
+//                  generator and only edge tiles read the packed matrix.  A
+//                  workgroup owns a 32x32 output tile, one wave per quarter
+//                  of K; the quarters meet in LDS.
 //   k_backproject  pixel-driven fan backprojection
 //                  (F/reconstructImageTOMO.cpp:46-100): a 16x16 pixel tile per
 //                  workgroup, one lane per pixel, the view loop uniform over
@@ -76,30 +77,93 @@ __global__ __launch_bounds__(256) void k_filter_prep(FilterPrepArgs a)
     a.apk[pack_index(r >> 5, k, r & 31, a.Kp)] = v;
 }
 
-__global__ __launch_bounds__(128) void k_filter_gemm(FilterGemmArgs a)
+constexpr int kFiltWaves = 4;  // K quarters of one 32x32 output tile, one wave each
+constexpr int kFiltU = 10;     // quads (4 MFMA steps each) a lane loads one chunk ahead
+
+// One 32x32 output tile per workgroup; wave w sums the w-th quarter of K, and
+// the quarters meet in LDS in a fixed order.  Operands are the pre-packed
+// register order: lane l = i + 32*h holds row/column i at k = 8q + 2j + h in
+// element j of float4 q.  Every lane loads a whole chunk of kFiltU float4 of A
+// and B before the chunk's MFMAs run, so 20-40 loads are in flight per lane.
+//
+// B is the filter matrix.  For a column tile whose rows are not renormalised
+// (edge_lo < m < edge_hi) it is Toeplitz, B[k][m] = toep[m - k + Kp - 1]: the
+// workgroup copies the tile's window of the generator (Kp + 31 floats) into
+// LDS and reads B from there, so only the few edge tiles read the packed
+// matrix from memory.
+//
+// Tiles are dealt to XCDs in contiguous row-major ranges (workgroup g runs on
+// XCD g % 8), so an XCD reads about an eighth of A, once, into its L2.
+__global__ __launch_bounds__(256) void k_filter_gemm(FilterGemmArgs a, int ct, int ntiles, int per_xcd)
 {
-    __shared__ float part[16][64];
-    const int tr = blockIdx.x, tc = blockIdx.y;
+    extern __shared__ float tw[];  // Toeplitz window, Kp + 31 floats
+    __shared__ float part[kFiltWaves - 1][16][64];
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int tile = xcd * per_xcd + slot;
+    if (slot >= per_xcd || tile >= ntiles) return;  // the whole workgroup
+    const int tr = tile / ct, tc = tile - tr * ct;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int Q = a.Kp / 8;  // quads of 4 MFMA steps
-    const int q0 = wave * (Q / 2), q1 = (wave + 1) * (Q / 2);
+    const int Q = a.Kp / 8, Qw = Q / kFiltWaves;
+    const int qb = wave * Qw, qe = qb + Qw;
+    const bool toep = tc * 32 > a.edge_lo && tc * 32 + 31 < a.edge_hi;  // uniform over the workgroup
+    if (toep) {
+        for (int j = threadIdx.x; j < a.Kp + 31; j += blockDim.x) tw[j] = a.toep[32 * tc + j];
+        __syncthreads();
+    }
     const float4* A = reinterpret_cast<const float4*>(a.apk) + ((size_t)tr * Q) * 64 + lane;
     const float4* B = reinterpret_cast<const float4*>(a.bpk) + ((size_t)tc * Q) * 64 + lane;
+    // window index of B[k = 0] for this lane: column m = 32 tc + (lane & 31), k = lane >> 5
+    const int t0 = (lane & 31) - (lane >> 5) + a.Kp - 1;
     floatx16 acc;
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[i] = 0.f;
-#pragma unroll 2
-    for (int q = q0; q < q1; q++) {
-        const float4 av = A[(size_t)q * 64];
-        const float4 bv = B[(size_t)q * 64];
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bv.x, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bv.y, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bv.z, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bv.w, acc, 0, 0, 0);
-    }
-    if (wave == 1) {
+    float4 ca[kFiltU], cb[kFiltU];
 #pragma unroll
-        for (int i = 0; i < 16; i++) part[i][lane] = acc[i];
+    for (int u = 0; u < kFiltU; u++) {
+        const int q = qb + u;
+        if (q < qe) {
+            ca[u] = A[(size_t)q * 64];
+            if (toep) {
+                const int t = t0 - 8 * q;
+                cb[u] = make_float4(tw[t], tw[t - 2], tw[t - 4], tw[t - 6]);
+            } else {
+                cb[u] = B[(size_t)q * 64];
+            }
+        }
+    }
+    for (int qs = qb; qs < qe; qs += kFiltU) {
+        float4 na[kFiltU], nb[kFiltU];
+#pragma unroll
+        for (int u = 0; u < kFiltU; u++) {
+            const int q = qs + kFiltU + u;
+            if (q < qe) {
+                na[u] = A[(size_t)q * 64];
+                if (toep) {
+                    const int t = t0 - 8 * q;
+                    nb[u] = make_float4(tw[t], tw[t - 2], tw[t - 4], tw[t - 6]);
+                } else {
+                    nb[u] = B[(size_t)q * 64];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFiltU; u++) {
+            if (qs + u < qe) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ca[u].x, cb[u].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ca[u].y, cb[u].y, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ca[u].z, cb[u].z, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ca[u].w, cb[u].w, acc, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFiltU; u++) {
+            ca[u] = na[u];
+            cb[u] = nb[u];
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) part[wave - 1][i][lane] = acc[i];
     }
     __syncthreads();
     if (wave == 0) {
@@ -109,7 +173,9 @@ __global__ __launch_bounds__(128) void k_filter_gemm(FilterGemmArgs a)
         for (int i = 0; i < 16; i++) {
             const int row_in = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
             const int row = tr * 32 + row_in;
-            float v = acc[i] + part[i][lane];
+            float v = acc[i] + part[0][i][lane];
+#pragma unroll
+            for (int w = 1; w < kFiltWaves - 1; w++) v += part[w][i][lane];
             if (a.in_mode && (v < 1000.0f || v > 300000.0f)) v = 0.f;  // :250-255
             if (row < a.R && col < a.W) a.out[(size_t)row * a.W + col] = v;
         }
@@ -322,12 +388,19 @@ vct_status launch_beam_hardening(const float* in, float* out, int n, double alph
 
 vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipStream_t s)
 {
-    if (p.Kp % 16 != 0) return set_error(VCT_ERR_INTERNAL, "filter K padding %d not a multiple of 16", p.Kp);
+    if (p.Kp % (8 * kFiltWaves) != 0)
+        return set_error(VCT_ERR_INTERNAL, "filter K padding %d not a multiple of %d", p.Kp, 8 * kFiltWaves);
+    if ((size_t)(p.Kp + 31) * sizeof(float) > 48 * 1024) return set_error(VCT_ERR_ARG, "too many channels for the filter");
     const int Rp = (p.R + 31) / 32 * 32;
     hipLaunchKernelGGL(k_filter_prep, dim3((p.Kp + 255) / 256, Rp), dim3(256), 0, s, p);
     VCT_HIP(hipGetLastError());
     const int Wp = (g.W + 31) / 32 * 32;
-    hipLaunchKernelGGL(k_filter_gemm, dim3(Rp / 32, Wp / 32), dim3(128), 0, s, g);
+    const int ct = Wp / 32;
+    const long long ntiles = (long long)(Rp / 32) * ct;
+    if (ntiles > INT_MAX / 2) return set_error(VCT_ERR_ARG, "filter GEMM too large");
+    const int per_xcd = (int)((ntiles + 7) / 8);
+    const size_t lds = (size_t)(g.Kp + 31) * sizeof(float);
+    hipLaunchKernelGGL(k_filter_gemm, dim3(8 * per_xcd), dim3(64 * kFiltWaves), lds, s, g, ct, (int)ntiles, per_xcd);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "vct.h"
 
@@ -94,8 +95,10 @@ struct FilterPrepArgs {
 struct FilterGemmArgs {
     const float* apk;     // packed A [R/32][Kp/8][64][4]
     const float* bpk;     // packed B [Wp/32][Kp/8][64][4]
+    const float* toep;    // Toeplitz generator of the rows without edge renormalisation
     float* out;           // [R][W]
     int R, W, Kp, in_mode;
+    int edge_lo, edge_hi; // output columns m <= edge_lo or m >= edge_hi are renormalised rows
 };
 
 struct BackprojArgs {
@@ -112,7 +115,8 @@ struct LengthprojArgs {
     const double* cosT;   // [V]
     const double* sinT;   // [V]
     float* npf;           // [N][N]
-    int N, W, V;
+    int N, W, V;          // W: detector channels (the channel pitch)
+    int pw;               // channels per window row (<= W): channels 0 .. pw-1
     int base;             // view of window row 0
     int count;            // window rows (V/2*2)
     double pix, rz, sid, det_src_iso, det_radius, iso_det, iso_channel;
@@ -127,6 +131,10 @@ struct vct_ctx {
     int* d_err = nullptr;          // kernel error flag
     void* scratch = nullptr;       // grows on demand
     size_t scratch_bytes = 0;
+    // Buffers the scratch outgrew.  They stay allocated until vct_terminate,
+    // so a kernel recorded with an older scratch pointer (a captured HIP
+    // graph) never writes into freed memory.
+    std::vector<void*> retired;
     int num_cu = 0;
     vct_status ensure_scratch(size_t bytes);
 };
@@ -145,8 +153,10 @@ struct vct_plan {
     double* d_px0 = nullptr;
     double* d_py0 = nullptr;
     double* d_coeff = nullptr;
-    float* d_bpk[3] = {nullptr, nullptr, nullptr};  // packed filter matrix per type
+    float* d_bpk[3] = {nullptr, nullptr, nullptr};   // packed filter matrix per type
+    float* d_toep[3] = {nullptr, nullptr, nullptr};  // Toeplitz generator per type
     int exact_trace = 0;  // VCT_OPT_EXACT_TRACE
+    int max_steps = 0;    // VCT_OPT_MAX_STEPS (0: the automatic bound)
 };
 
 namespace vct {
@@ -162,6 +172,6 @@ vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipSt
 vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s);
 vct_status launch_beam_hardening(const float* in, float* out, int n, double alpha, double beta, hipStream_t s);
 vct_status launch_lengthproj(const LengthprojArgs& a, hipStream_t s);
-vct_status plan_filter_matrix(vct_plan* plan, int filter, float** d_bpk);
+vct_status plan_filter_matrix(vct_plan* plan, int filter, float** d_bpk, float** d_toep);
 TraceArgs make_trace_args(const vct_plan* plan);
 }  // namespace vct

@@ -9,7 +9,12 @@
 // interpolates the view's row of the sinogram and adds sample * chord in
 // float, views ascending (:152-170).  All geometry is FP64 in the
 // reference's operation order (contraction off); the sinogram window
-// (<= 2 MB) stays in L2/MALL.  The reference also fills two per-pixel,
+// (<= 2 MB) stays in L2/MALL.  The channel pitch and isocentre channel are
+// the detector's (the plan's geometry); a window row may hold fewer channels
+// (measured data keep 582 or fewer), and hits past its last channel take that
+// channel's value (the reference, whose pitch is fixed by its 609-channel
+// End_Channel, interpolates there with the narrower row as stride, i.e. reads
+// the next row's first channels).  The reference also fills two per-pixel,
 // per-view caches for its disabled iterative loop; they are not produced.
 #include "vct_internal.h"
 
@@ -33,7 +38,7 @@ __global__ __launch_bounds__(256) void k_lengthproj(LengthprojArgs a)
         const double ry = a.N * pix * 0.5 - (jy + 0.5) * pix;   // :56
         const double rx = -a.N * pix * 0.5 + (jx + 0.5) * pix;  // :61
         const double rz = a.rz, R = a.det_radius;
-        const int last = a.W - 1;
+        const int last = a.pw - 1;  // the window's last channel
         for (int j = 0; j < a.count; j++) {
             int jr = a.base + j;
             if (jr >= a.V) jr -= a.V;  // :70-71
@@ -73,7 +78,7 @@ __global__ __launch_bounds__(256) void k_lengthproj(LengthprojArgs a)
             const double angle = acos(1.0 - dis / (2.0 * R * R));
             const double rot_px = px * c - py * s;
             const double dj = rot_px > 0.0 ? a.iso_channel - angle / a.delta : a.iso_channel + angle / a.delta;
-            const float* row = a.proj + (size_t)j * a.W;
+            const float* row = a.proj + (size_t)j * a.pw;
             double np;
             if (0 <= dj && dj < last) {  // :152-158
                 const int i0 = (int)dj;

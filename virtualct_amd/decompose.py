@@ -12,8 +12,9 @@ hand-written adjoint.  Updates are projected Barzilai-Borwein steps,
 rho <- max(0, rho - alpha_b * g_b), alpha_b from the last two iterates.
 
 Two drivers share the mathematics:
-  * ``PlainLoop``: per phantom and per spectrum, the plain forward and the
-    plain adjoint (what a user of the single-sinogram API writes);
+  * ``PlainLoop``: per phantom, the plain forward of all spectra from one
+    trace and one plain adjoint for their combined gradient (what a user of
+    the single-phantom API writes);
   * ``BatchedSolver`` (virtualct_amd.batch): every phantom and spectrum in
     one launch per stage over the cached system matrix, captured in a HIP
     graph.
@@ -42,7 +43,9 @@ def reprojection_error(plan, dens: torch.Tensor, target: torch.Tensor, att: torc
 
 
 class PlainLoop:
-    """Per-phantom, per-spectrum projected-gradient loop over the plain API."""
+    """Per-phantom projected-gradient loop over the plain API: one
+    ``mbfpa_intensity`` call (one trace, every spectrum) and one backward
+    (one adjoint) per phantom and iteration."""
 
     def __init__(self, plan, targets, att, pdf, init):
         self.plan = plan
@@ -57,17 +60,12 @@ class PlainLoop:
         self.alpha = torch.full((self.B,), 1e-4, device=init.device)
 
     def gradient(self, b):
-        g = torch.zeros_like(self.x[b])
-        loss = torch.zeros((), device=self.x.device)
-        for s in range(self.S):
-            x = self.x[b].detach().requires_grad_(True)
-            I = ops.mbfpa_intensity(x, self.plan, self.att[s:s + 1], self.pdf[s:s + 1])
-            imax = I.detach().amax()
-            r, dI = log_residual(I.detach(), imax, self.t[b, s:s + 1])
-            I.backward(dI)
-            g += x.grad
-            loss += r.square().sum()
-        return g, loss
+        x = self.x[b].detach().requires_grad_(True)
+        I = ops.mbfpa_intensity(x, self.plan, self.att, self.pdf)  # [S,V,W]
+        imax = I.detach().amax(dim=(1, 2), keepdim=True)
+        r, dI = log_residual(I.detach(), imax, self.t[b])
+        I.backward(dI)
+        return x.grad, r.square().sum()
 
     def step(self):
         losses = []

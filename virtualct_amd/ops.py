@@ -65,6 +65,10 @@ class Plan:
     def set_option(self, option: int, value: int):
         call("vct_plan_set_option", self.handle, option, value)
 
+    def check_errors(self):
+        """``check_errors`` for this plan's device."""
+        check_errors(self.device)
+
     def system_matrix(self):
         """The cached system matrix of this geometry (ray- and voxel-major)."""
         if self._sysmat is None:
@@ -85,6 +89,15 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def check_errors(device: int = 0, stream=None):
+    """Wait for the stream (default: the current torch stream) and raise
+    ``VctError`` (status internal) if a kernel since the last check gave up
+    (a ray beyond its marching step bound, a system-matrix row off its
+    count); the flag is cleared either way."""
+    s = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream(device).cuda_stream)
+    call("vct_check_errors", context(device), s)
 
 
 def plan_for(geom: Geometry, device: int = 0) -> Plan:
@@ -131,11 +144,27 @@ def pack_densities(plan: Plan, dens: torch.Tensor):
     raise TypeError("densities must be int16 or float32")
 
 
-def trace_paths(plan: Plan, packed: torch.Tensor, dens_type: int, M: int, counts: bool = False):
-    """Per-ray elemental path integrals L [V,W,8] (and segment counters)."""
+def _check_packed(plan: Plan, packed: torch.Tensor, dens_type: int):
+    _require_gpu(packed)
+    want = torch.int16 if dens_type == DENS_U16 else torch.float32
+    if dens_type not in (DENS_U16, DENS_F32) or packed.dtype != want or packed.shape != (plan.N, plan.N, 8) \
+            or not packed.is_contiguous():
+        raise ValueError(f"packed densities must be contiguous {want} [{plan.N},{plan.N},8] "
+                         f"(got {packed.dtype} {tuple(packed.shape)}, type {dens_type})")
+
+
+def trace_paths(plan: Plan, packed: torch.Tensor, dens_type: int, M: int, counts: bool = False,
+                check: bool = True):
+    """Per-ray elemental path integrals L [V,W,8] (and segment counters).
+
+    check: wait for the tracer and raise if a ray gave up (``check_errors``);
+    pipelines that enqueue many launches pass False and check once."""
+    _check_packed(plan, packed, dens_type)
     L = torch.empty((plan.V, plan.W, 8), dtype=torch.float32, device=_dev(plan))
     cnt = torch.empty((plan.V, plan.W), dtype=torch.int32, device=_dev(plan)) if counts else None
     call("vct_trace_paths", plan.handle, _ptr(packed), dens_type, M, _ptr(L), _ptr(cnt), _stream(plan.device))
+    if check:
+        check_errors(plan.device)
     return (L, cnt) if counts else L
 
 
@@ -151,6 +180,10 @@ def spectral_sum(plan: Plan, L: torch.Tensor, M: int, att: torch.Tensor, pdf: to
     S, Ma, NE = att.shape
     if Ma != M or pdf.shape != (S, NE):
         raise ValueError(f"attenuation {tuple(att.shape)} / pdf {tuple(pdf.shape)} do not match M={M}")
+    _require_gpu(L)
+    if L.dtype != torch.float32 or L.shape != (plan.V, plan.W, 8) or not L.is_contiguous():
+        raise ValueError(f"path integrals must be contiguous float32 [{plan.V},{plan.W},8], got "
+                         f"{L.dtype} {tuple(L.shape)}")
     raw = torch.empty((S, plan.V, plan.W), dtype=torch.float32, device=_dev(plan))
     stats = torch.empty((S, 2), dtype=torch.int32, device=_dev(plan))
     dIdL = torch.empty((S, plan.V, plan.W, 8), dtype=torch.float32, device=_dev(plan)) if want_dIdL else None
@@ -166,7 +199,12 @@ def stats_values(stats: torch.Tensor) -> torch.Tensor:
 
 def detector(plan: Plan, raw: torch.Tensor, stats: torch.Tensor, sigma: float = 0.0,
              seed: int = 0x5EED, stream_id: int = 0):
+    _require_gpu(raw)
     S = raw.shape[0]
+    if raw.dtype != torch.float32 or raw.shape != (S, plan.V, plan.W) or not raw.is_contiguous():
+        raise ValueError(f"raw sinograms must be contiguous float32 [S,{plan.V},{plan.W}]")
+    if stats.shape != (S, 2) or stats.element_size() != 4 or not stats.is_contiguous():
+        raise ValueError("stats must be contiguous 32-bit [S,2]")
     out = torch.empty_like(raw)
     call("vct_detector", plan.handle, _ptr(raw), _ptr(stats), S, float(sigma), seed & (2**64 - 1),
          stream_id & 0xFFFFFFFF, _ptr(out), _stream(plan.device))
@@ -175,11 +213,12 @@ def detector(plan: Plan, raw: torch.Tensor, stats: torch.Tensor, sigma: float = 
 
 def mbfpa_forward(plan: Plan, dens: torch.Tensor, att: torch.Tensor, pdf: torch.Tensor,
                   sigma: float = 0.0, seed: int = 0x5EED, stream_id: int = 0,
-                  packed=None):
+                  packed=None, check: bool = True):
     """Normalised (and optionally noisy) sinograms [S,V,W] of one phantom.
 
     dens: [M,N,N] int16 or float32 (ignored if ``packed=(tensor, type)``);
-    att: [S,M,NE] or [M,NE]; pdf: [S,NE] or [NE]."""
+    att: [S,M,NE] or [M,NE]; pdf: [S,NE] or [NE].  check: wait and raise if
+    a ray gave up (``check_errors``)."""
     if packed is None:
         packed = pack_densities(plan, dens)
     pk, dt = packed
@@ -190,33 +229,60 @@ def mbfpa_forward(plan: Plan, dens: torch.Tensor, att: torch.Tensor, pdf: torch.
         att = att.unsqueeze(0)
     if pdf.dim() == 1:
         pdf = pdf.unsqueeze(0)
-    S, _, NE = att.shape
+    S, Ma, NE = att.shape
+    if Ma != M or pdf.shape != (S, NE):
+        raise ValueError(f"attenuation {tuple(att.shape)} / pdf {tuple(pdf.shape)} do not match M={M}")
+    _check_packed(plan, pk, dt)
     sino = torch.empty((S, plan.V, plan.W), dtype=torch.float32, device=_dev(plan))
     call("vct_mbfpa_forward", plan.handle, _ptr(pk), dt, M, _ptr(att), _ptr(pdf), NE, S, float(sigma),
          seed & (2**64 - 1), stream_id & 0xFFFFFFFF, _ptr(sino), C.c_void_p(0), _stream(plan.device))
+    if check:
+        check_errors(plan.device)
     return sino
 
 
 # --------------------------------------------------------------------- FBP
 
-def filter_sinogram(plan: Plan, proj: torch.Tensor, filt: int = FILTER_SHEPP_LOGAN, in_mode: bool = False):
-    _require_gpu(proj)
-    proj = proj.contiguous()
-    if proj.dim() == 2:
-        proj = proj.unsqueeze(0)
+def _sinograms(plan: Plan, x: torch.Tensor, what: str) -> torch.Tensor:
+    """[V,W] or [S,V,W] float32 on the GPU -> contiguous [S,V,W]."""
+    _require_gpu(x)
+    if x.dtype != torch.float32:
+        raise ValueError(f"{what} must be float32, got {x.dtype}")
+    x = x.contiguous()
+    if x.dim() == 2:
+        x = x.unsqueeze(0)
+    if x.dim() != 3 or x.shape[1:] != (plan.V, plan.W) or x.shape[0] < 1:
+        raise ValueError(f"{what} shape {tuple(x.shape)} does not match geometry {plan.V}x{plan.W}")
+    return x
+
+
+def filter_workspace(plan: Plan, n_sino: int) -> torch.Tensor:
+    """A workspace for ``filter_sinogram(..., work=)`` of n_sino sinograms."""
+    n = int(_lib.lib().vct_filter_workspace_bytes(plan.handle, n_sino))
+    return torch.empty(max(n // 4, 1), dtype=torch.float32, device=_dev(plan))
+
+
+def filter_sinogram(plan: Plan, proj: torch.Tensor, filt: int = FILTER_SHEPP_LOGAN, in_mode: bool = False,
+                    work: torch.Tensor | None = None):
+    """Filtered sinograms [S,V,W].  work: a caller-owned workspace
+    (``filter_workspace``) instead of the context scratch, for callers that
+    record the filter in a HIP graph."""
+    proj = _sinograms(plan, proj, "sinogram")
     S = proj.shape[0]
-    if proj.shape[1:] != (plan.V, plan.W):
-        raise ValueError(f"sinogram shape {tuple(proj.shape)} does not match geometry {plan.V}x{plan.W}")
     out = torch.empty_like(proj, dtype=torch.float32)
-    call("vct_filter_sinogram", plan.handle, _ptr(proj), S, filt, int(in_mode), _ptr(out), _stream(plan.device))
+    if work is None:
+        call("vct_filter_sinogram", plan.handle, _ptr(proj), S, filt, int(in_mode), _ptr(out), _stream(plan.device))
+    else:
+        need = int(_lib.lib().vct_filter_workspace_bytes(plan.handle, S))
+        if not work.is_cuda or not work.is_contiguous() or work.numel() * work.element_size() < need:
+            raise ValueError(f"filter workspace must be a contiguous device buffer of >= {need} bytes")
+        call("vct_filter_sinogram_ws", plan.handle, _ptr(proj), S, filt, int(in_mode), _ptr(out), _ptr(work),
+             _stream(plan.device))
     return out
 
 
 def backproject(plan: Plan, fltd: torch.Tensor):
-    _require_gpu(fltd)
-    fltd = fltd.contiguous()
-    if fltd.dim() == 2:
-        fltd = fltd.unsqueeze(0)
+    fltd = _sinograms(plan, fltd, "filtered sinogram")
     S = fltd.shape[0]
     img = torch.empty((S, plan.N, plan.N), dtype=torch.float32, device=fltd.device)
     call("vct_backproject", plan.handle, _ptr(fltd), S, _ptr(img), _stream(plan.device))
@@ -229,14 +295,17 @@ ISO_TO_DETECTOR_CM = 57.6
 def lengthproj(plan: Plan, proj: torch.Tensor, z_cm: float, couch_speed_cm: float,
                iso_det_cm: float = ISO_TO_DETECTOR_CM) -> torch.Tensor:
     """npf [N,N]: chord-weighted pixel backprojection of the sinogram window
-    proj [V,W] for slice z (the MBFPA program's npf_float.raw)."""
+    proj [V, w] for slice z (the MBFPA program's npf_float.raw).  The window's
+    rows hold the first w <= W channels of the plan's detector (measured data
+    keep fewer channels; the detector's pitch still places the hits)."""
     _require_gpu(proj)
     proj = proj.contiguous()
-    if proj.dtype != torch.float32 or proj.shape != (plan.V // 2 * 2, plan.W):
-        raise ValueError(f"proj must be float32 [{plan.V // 2 * 2}, {plan.W}]")
+    if proj.dtype != torch.float32 or proj.dim() != 2 or proj.shape[0] != plan.V // 2 * 2 \
+            or not 2 <= proj.shape[1] <= plan.W:
+        raise ValueError(f"proj must be float32 [{plan.V // 2 * 2}, w <= {plan.W}], got {tuple(proj.shape)}")
     npf = torch.empty((plan.N, plan.N), dtype=torch.float32, device=proj.device)
-    call("vct_lengthproj", plan.handle, _ptr(proj), float(z_cm), float(couch_speed_cm), float(iso_det_cm),
-         _ptr(npf), _stream(plan.device))
+    call("vct_lengthproj_window", plan.handle, _ptr(proj), proj.shape[1], float(z_cm), float(couch_speed_cm),
+         float(iso_det_cm), _ptr(npf), _stream(plan.device))
     return npf
 
 
@@ -286,10 +355,7 @@ def fbp(plan: Plan, proj: torch.Tensor, filt: int = FILTER_SHEPP_LOGAN, in_mode:
 
 def filter_sinogram_bhc(plan: Plan, proj: torch.Tensor, alpha: float, beta: float,
                         filt: int = FILTER_SHEPP_LOGAN, in_mode: bool = False):
-    _require_gpu(proj)
-    proj = proj.contiguous()
-    if proj.dim() == 2:
-        proj = proj.unsqueeze(0)
+    proj = _sinograms(plan, proj, "sinogram")
     S = proj.shape[0]
     out = torch.empty_like(proj, dtype=torch.float32)
     call("vct_filter_sinogram_bhc", plan.handle, _ptr(proj), S, filt, int(in_mode), float(alpha), float(beta),
@@ -320,12 +386,27 @@ def minmax(x: torch.Tensor):
 
 def trace_adjoint(plan: Plan, weights: torch.Tensor, M: int, g: torch.Tensor | None = None,
                   dens_packed_f32: torch.Tensor | None = None, grad: torch.Tensor | None = None,
-                  dens_type: int = DENS_F32):
-    """grad [N,N,8] += sum_rays weights[ray,m] * g[ray] * length(ray, voxel)."""
+                  dens_type: int = DENS_F32, check: bool = True):
+    """grad [N,N,8] += sum_rays weights[ray,m] * g[ray] * length(ray, voxel).
+
+    weights: float32 [V,W,8]; g: float32 [V,W] or None; dens_packed_f32:
+    float32 [N,N,8] (voxels with a negative density get no gradient) or None;
+    grad: float32 [N,N,8] to accumulate into (zeros if None)."""
+    _require_gpu(weights)
+    if weights.dtype != torch.float32 or weights.shape != (plan.V, plan.W, 8):
+        raise ValueError(f"weights must be float32 [{plan.V},{plan.W},8], got {weights.dtype} {tuple(weights.shape)}")
+    if g is not None and (g.dtype != torch.float32 or g.shape != (plan.V, plan.W)):
+        raise ValueError(f"g must be float32 [{plan.V},{plan.W}]")
+    if dens_packed_f32 is not None:
+        _check_packed(plan, dens_packed_f32, DENS_F32)
     if grad is None:
         grad = torch.zeros((plan.N, plan.N, 8), dtype=torch.float32, device=_dev(plan))
+    elif grad.dtype != torch.float32 or grad.shape != (plan.N, plan.N, 8) or not grad.is_contiguous():
+        raise ValueError(f"grad must be contiguous float32 [{plan.N},{plan.N},8]")
     call("vct_trace_adjoint", plan.handle, _ptr(dens_packed_f32), dens_type, M, _ptr(weights.contiguous()),
          _ptr(g.contiguous() if g is not None else None), _ptr(grad), _stream(plan.device))
+    if check:
+        check_errors(plan.device)
     return grad
 
 
