@@ -120,8 +120,30 @@ vct_status vct_plan_destroy(vct_plan* plan);
  * reciprocal march; both give the same segments, the default is faster.
  * VCT_OPT_MAX_STEPS = 2: marching steps after which a ray gives up and
  * raises the context's error flag (0, the default: 4N + 4/pixel + 64, more
- * than any ray of the geometry needs). */
-enum { VCT_OPT_EXACT_TRACE = 1, VCT_OPT_MAX_STEPS = 2 };
+ * than any ray of the geometry needs).
+ * VCT_OPT_EXACT_BACKPROJECT = 4: 1 keeps, for a single sinogram, the
+ * backprojection that adds each pixel's views in the reference's order
+ * (>= 99% of pixels bit-identical to it for the same filtered input).  By
+ * default, when the view count is a multiple of 4 and N is even, one
+ * sinogram is backprojected by orbits of the 90-degree rotation (pixel p
+ * at view v + V/4 sees the geometry of pixel (y, N-1-x) at view v), which
+ * computes each pixel-view geometry once for four pixels; its sums are in
+ * another order (within 1e-5 relative of the reference's image).
+ * VCT_OPT_TRACE_ORBIT = 5: rays per march of vct_trace_paths: 4 (default)
+ * marches the ray of view v once for the rays of views v + kV/4 (the same
+ * segments turned by quarter turns, when V is a multiple of 4 and N even),
+ * 2 for views v and v + V/2 (V even), 1 marches every ray by itself.  The
+ * segments are the reference's either way: every decision of the shared
+ * march must clear the guarded march's margin for the turned rays too, or
+ * those rays are marched by themselves; lengths may differ in the last
+ * bits.  The exact option and geometries where the reference's r >= rmax
+ * test can fire use one ray per march. */
+enum {
+    VCT_OPT_EXACT_TRACE = 1,
+    VCT_OPT_MAX_STEPS = 2,
+    VCT_OPT_EXACT_BACKPROJECT = 4,
+    VCT_OPT_TRACE_ORBIT = 5
+};
 vct_status vct_plan_set_option(vct_plan* plan, int option, int value);
 
 /* ------------------------------------------------------- host-side tables */

@@ -228,13 +228,20 @@ def test_config5_gradient_central_differences(gpu, oracle, config5):
 
 
 def test_config5_iterations_lower_the_loss(gpu, config5):
-    """Ten projected Barzilai-Borwein iterations of the 16-phantom solver lower
-    every phantom's loss and keep the densities non-negative and finite."""
+    """Ten projected Barzilai-Borwein iterations of the 16-phantom solver: the
+    total loss falls below 0.6 of the start, every phantom's loss falls below
+    its start at some iteration (the method is not monotone), and the
+    densities stay non-negative and finite."""
     c = config5
     s = c["solver"]
-    l0 = s.step().clone()
-    for _ in range(9):
-        s.step()
-    l10 = s.step().clone()
-    assert (l10 < l0).all(), (l0, l10)
+    hist = [s.step().clone() for _ in range(11)]
+    l0 = hist[0]
+    best = torch_min(hist[1:])
+    assert hist[-1].sum() < 0.6 * l0.sum(), (l0, hist[-1])
+    assert (best < l0).all(), (l0, best)
     assert (s.x >= 0).all() and s.x.isfinite().all()
+
+
+def torch_min(ts):
+    import torch
+    return torch.stack(ts).amin(0)

@@ -27,6 +27,15 @@ def _t(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
 
+def _exact_plan(geom):
+    """A plan whose single-sinogram backprojection adds each pixel's views in
+    the reference's order (not the quarter-turn orbits)."""
+    from virtualct_amd import _lib
+    plan = _ops().Plan(geom)
+    plan.set_option(_lib.OPT_EXACT_BACKPROJECT, 1)
+    return plan
+
+
 @pytest.mark.parametrize("filt", [2, 1])
 def test_filter_matches_oracle(gpu, golden, oracle, filt):
     ops = _ops()
@@ -55,17 +64,46 @@ def test_filter_in_mode_threshold(gpu, golden, oracle):
 
 
 def test_backprojection_matches_oracle(gpu, golden, oracle):
-    """Same filtered input: the BP reproduces the reference's FP64 geometry and
-    float accumulation order (atan from the device math library)."""
+    """Same filtered input: the exact-order BP reproduces the reference's FP64
+    geometry and float accumulation order (atan from the device math library)."""
     ops = _ops()
     fl = oracle.filter_ref(Geom(), golden.kat_f_input(), 2)
-    plan = ops.plan_for(_geom())
+    plan = _exact_plan(_geom())
     img = ops.backproject(plan, _t(fl, gpu))[0].cpu().numpy()
     ref = oracle.backproject_ref(Geom(), fl)
     assert rel_max(img, ref) < 1e-6
     assert np.count_nonzero(img != ref) < 0.01 * img.size
     mask = oracle.mask(512).astype(bool)
     assert np.all(img[~mask] == 0)
+    plan.close()
+
+
+@pytest.mark.parametrize("N,pix,V", [(512, 0.1, 800), (128, 0.6, 160), (64, 0.7, 2100), (96, 0.5, 84)])
+def test_quarter_turn_backprojection_matches_oracle(gpu, golden, oracle, N, pix, V):
+    """The default single-sinogram BP shares each pixel-view geometry over the
+    4 pixels of a quarter-turn orbit (views a multiple of 4, N even): within
+    1e-5 of the reference-order image and of FP64 (512^2 on the KAT-F input;
+    a wide field with the library atan; 2100 views; 84 views, 96^2), zero
+    outside the mask, and within 1e-5 of the exact-order kernel (the float
+    sums run in another order: measured 1.2e-6 at 800 views, 2.2e-6 at 2100)."""
+    ops = _ops()
+    g = Geom(N=N, pix=pix, V=V)
+    if (N, V) == (512, 800):
+        fl = oracle.filter_ref(Geom(), golden.kat_f_input(), 2)
+    else:
+        fl = np.random.default_rng(N + V).normal(size=(V, 609)).astype(np.float32) * 1e3
+    img = ops.backproject(ops.plan_for(_geom(N, pix, V)), _t(fl, gpu))[0].cpu().numpy()
+    ref = oracle.backproject_ref(g, fl)
+    f64 = oracle.backproject_f64(g, fl.astype(np.float64))
+    assert rel_max(img, ref) < TOL
+    assert rel_max(img, f64) < TOL
+    assert rel_l2(img, f64) < 1e-6
+    mask = oracle.mask(N).astype(bool)
+    assert np.all(img[~mask] == 0)
+    plan = _exact_plan(_geom(N, pix, V))
+    exact = ops.backproject(plan, _t(fl, gpu))[0].cpu().numpy()
+    assert rel_max(img, exact) < TOL
+    plan.close()
 
 
 def test_kat_f_end_to_end(gpu, golden, oracle):
@@ -97,7 +135,7 @@ def test_batched_fbp_equals_single(gpu, golden):
     import torch
     ops = _ops()
     p = golden.kat_f_input()
-    plan = ops.plan_for(_geom())
+    plan = _exact_plan(_geom())
     stack = torch.stack([_t(p, gpu), _t(p * np.float32(0.9), gpu), _t(np.sqrt(p), gpu)])
     many = ops.fbp(plan, stack, 2)
     for i in range(3):
@@ -146,13 +184,13 @@ def test_filter_scratch_growth_keeps_captured_graph_valid(gpu):
     run.capture()
     plan = ops.plan_for(_geom())
     big = torch.rand((24, 800, 609), device=gpu) * 0.5 + 0.25
-    many = ops.fbp(plan, big, 2)  # 24 sinograms: the context scratch grows
+    many = ops.filter_sinogram(plan, big, 2)  # 24 sinograms: the context scratch grows
     for _ in range(2):
         _, i2 = run.replay()
         torch.cuda.synchronize()
         for a, b in zip(i2, eager):
             assert torch.equal(a, b)
-    assert torch.equal(many[5], ops.fbp(plan, big[5], 2)[0])
+    assert torch.equal(many[5], ops.filter_sinogram(plan, big[5], 2)[0])
 
 
 @pytest.mark.parametrize("N,pix,V,S", [(512, 0.1, 800, 16), (512, 0.1, 800, 48),
@@ -172,7 +210,7 @@ def test_multi_sinogram_backprojection_equals_single(gpu, oracle, N, pix, V, S):
     ops = _ops()
     rng = np.random.default_rng(40 + S)
     fl = rng.normal(size=(S, V, 609)).astype(np.float32) * 1e3
-    plan = ops.plan_for(_geom(N, pix, V))
+    plan = _exact_plan(_geom(N, pix, V))
     many = ops.backproject(plan, _t(fl, gpu))
     for i in range(S):
         one = ops.backproject(plan, _t(fl[i], gpu))[0]
@@ -180,6 +218,7 @@ def test_multi_sinogram_backprojection_equals_single(gpu, oracle, N, pix, V, S):
     if N <= 128:
         ref = oracle.backproject_ref(Geom(N=N, pix=pix, V=V), fl[S - 1])
         assert rel_max(many[S - 1].cpu().numpy(), ref) < 1e-5
+    plan.close()
 
 
 def test_mbfpa_then_fbp_recovers_mu_map(gpu, golden):
@@ -251,12 +290,12 @@ def test_wide_field_backprojection_takes_library_atan(gpu, oracle):
 
 
 def test_backprojection_more_views_than_one_trig_block(gpu, oracle):
-    """2100 views: a view loop longer than 2048 views."""
+    """2100 views: a view loop longer than 2048 views (exact-order kernel)."""
     ops = _ops()
     N, pix, V = 64, 0.7, 2100
     g = Geom(N=N, pix=pix, V=V)
     fl = np.random.default_rng(32).normal(size=(V, 609)).astype(np.float32)
-    plan = ops.plan_for(_geom(N, pix, V))
+    plan = _exact_plan(_geom(N, pix, V))
     img = ops.backproject(plan, _t(fl, gpu))[0].cpu().numpy()
     ref = oracle.backproject_ref(g, fl)
     assert rel_max(img, ref) < 1e-5

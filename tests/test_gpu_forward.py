@@ -301,3 +301,32 @@ def test_step_bound_overflow_is_reported(gpu, golden):
     plan.set_option(_lib.OPT_MAX_STEPS, 0)
     assert torch.equal(ops.trace_paths(plan, pk, dt, 6), good)
     plan.close()
+
+
+@pytest.mark.parametrize("N,pix,V", [(512, 0.1, 800), (768, 50.0 / 768, 800), (128, 0.35, 162)])
+def test_orbit_trace_equals_single_ray_march(gpu, golden, N, pix, V):
+    """The default tracer marches the ray of view v once for the rays of views
+    v + kV/4 (V a multiple of 4; 162 views take half turns): its segment
+    counters equal marching every ray by itself on every ray, and the path
+    integrals agree to 1e-7 (the orbit shares the base ray's lengths), for
+    integer and float densities."""
+    import torch
+    ops = _ops()
+    from virtualct_amd import _lib, phantoms
+    ph = phantoms.body_phantom(17, N=N, pix_cm=pix)
+    out = {}
+    for orbit in (4, 2, 1):
+        plan = ops.Plan(_geom(N, pix, V))
+        plan.set_option(_lib.OPT_TRACE_ORBIT, orbit)
+        for name, d in (("u16", _t(ph, gpu)), ("f32", _t(phantoms.densities_f32(ph), gpu))):
+            pk, dt = ops.pack_densities(plan, d)
+            L, cnt = ops.trace_paths(plan, pk, dt, 6, counts=True)
+            out[(orbit, name)] = (L.cpu().numpy(), cnt.cpu().numpy())
+        plan.close()
+    for name in ("u16", "f32"):
+        L1, c1 = out[(1, name)]
+        assert c1.sum() > 0
+        for orbit in (4, 2):
+            L, c = out[(orbit, name)]
+            np.testing.assert_array_equal(c, c1, err_msg=f"orbit {orbit} {name}")
+            assert rel_max(L, L1) < 1e-7, (orbit, name)

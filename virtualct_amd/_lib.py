@@ -20,6 +20,8 @@ DENS_U16, DENS_F32 = 0, 1
 FILTER_RAMLAK, FILTER_SHEPP_LOGAN = 1, 2
 OPT_EXACT_TRACE = 1
 OPT_MAX_STEPS = 2
+OPT_EXACT_BACKPROJECT = 4
+OPT_TRACE_ORBIT = 5
 ERR_INTERNAL = 6
 SYSMAT_TRANSPOSE = 1
 SYSMAT_BLOCKED = 2

@@ -75,6 +75,14 @@ TraceArgs make_trace_args(const vct_plan* p)
     a.ray_offset = 0;
     a.n_rays = g.n_views * g.n_channels;
     a.exact = p->exact_trace;
+    // orbits of the half or quarter turn: every view traced (V even), a grid
+    // centred on a voxel corner (N even), the guarded march (no exact option,
+    // no r >= rmax test)
+    a.rot = 1;
+    if (!a.exact && !a.rcheck && g.recon_size % 2 == 0 && a.VT == g.n_views) {
+        if (p->trace_orbit >= 4 && g.n_views % 4 == 0) a.rot = 4;
+        else if (p->trace_orbit >= 2 && g.n_views % 2 == 0) a.rot = 2;
+    }
     return a;
 }
 
@@ -355,6 +363,11 @@ vct_status vct_plan_set_option(vct_plan* p, int option, int value)
     if (!p) return set_error(VCT_ERR_ARG, "plan is NULL");
     switch (option) {
     case VCT_OPT_EXACT_TRACE: p->exact_trace = value ? 1 : 0; return VCT_OK;
+    case VCT_OPT_EXACT_BACKPROJECT: p->exact_bp = value ? 1 : 0; return VCT_OK;
+    case VCT_OPT_TRACE_ORBIT:
+        if (value != 1 && value != 2 && value != 4) return set_error(VCT_ERR_ARG, "rays per orbit %d is not 1, 2 or 4", value);
+        p->trace_orbit = value;
+        return VCT_OK;
     case VCT_OPT_MAX_STEPS:
         if (value < 0) return set_error(VCT_ERR_ARG, "step bound %d must be >= 0", value);
         p->max_steps = value;
@@ -686,6 +699,7 @@ vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* 
     a.center = (double)g.recon_size / 2.0 - 0.5;
     a.mask_r = (double)(g.recon_size - 1) / 2.0;
     a.mask_r2 = a.mask_r * a.mask_r;
+    a.exact = p->exact_bp;
     return launch_backproject(a, n_sino, pick_stream(p, stream));
 }
 
@@ -749,8 +763,8 @@ vct_status vct_trace_adjoint(vct_plan* p, const void* d_packed, int dens_type, i
     a.dens = d_packed;
     a.wts = d_dIdL;
     a.g = d_g;
-    a.grad = d_grad;
     a.out_scale = (dens_type == VCT_DENS_U16 ? 1.0 / 1000.0 : 1.0) / p->g.xfactor;
+    a.grad = d_grad;
     return launch_trace(a, VCT_DENS_F32, M, true, pick_stream(p, stream));
 }
 

@@ -20,6 +20,9 @@
 //                  accumulation order.
 //   k_backproject_multi  the same for 2..16 sinograms of one geometry per
 //                  lane, sharing each pixel-view's geometry.
+//   k_backproject_sym  one sinogram by orbits of the quarter turn: each
+//                  pixel-view geometry serves four pixels (views a multiple
+//                  of 4, N even; sums in another order).
 #include "vct_internal.h"
 
 #include <algorithm>
@@ -270,10 +273,12 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject(BackprojArgs 
 }
 
 // The sinogram-independent part of one pixel-view: detector weights, 1/L^2,
-// whether the bin pair is on the detector, and the row offset of the pair.
+// whether the bin pair is on the detector, and the row offset of the pair
+// (col: its offset within the row).
 struct BpView {
     double w, w0, rL2;
     size_t off;
+    int col;
     bool ok;
 };
 
@@ -295,7 +300,8 @@ __device__ __forceinline__ BpView bp_view(const BackprojArgs& a, double xx, doub
     g.w0 = 1 - g.w;
     g.rL2 = rcp_f64(L2);
     g.ok = GAM >= 0 && GAM < a.W - 1;
-    g.off = (size_t)v * a.W + (g.ok ? GAM : 0);
+    g.col = g.ok ? GAM : 0;
+    g.off = (size_t)v * a.W + g.col;
     return g;
 }
 
@@ -365,6 +371,88 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_multi(Backpro
     }
 }
 
+// One sinogram by orbits of the quarter turn.  With V views over 360 degrees
+// (V a multiple of 4, Q = V/4) and a centred N x N grid (N even), pixel p at
+// view v + Q has the geometry of pixel T(p) = (y, N-1-x) at view v, so the
+// geometry G(p, v) of a base pixel p (x, y < N/2) serves the four pixels
+// T^-k(p), k = 0..3, at views v + kQ (mod V):
+//   T^0 = (x, y), T^-1 = (N-1-y, x), T^-2 = (N-1-x, N-1-y), T^-3 = (y, N-1-x).
+// A workgroup holds 64 base pixels (an 8 x 8 tile of the quadrant); wave w
+// runs views v in [wQ, (w+1)Q) with the software pipeline of
+// k_backproject_multi, so each pixel's 4 partial sums (wave 0 .. 3) are added
+// in that order through LDS.  Each term is the single-sinogram expression;
+// the order of the float sum differs from the reference's ascending views.
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_backproject_sym(BackprojArgs a)
+{
+    __shared__ float part[3][4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int h = a.N / 2, Q = a.V / 4;
+    const int x = blockIdx.x * 8 + (lane & 7), y = blockIdx.y * 8 + (lane >> 3);
+    const bool in_q = x < h && y < h;
+    // prepareMaskData, F/misc.cpp:20-36 (the circle is invariant under T)
+    const double dyc = (double)y - a.mask_r, dxc = (double)x - a.mask_r;
+    const bool masked = in_q && (dyc * dyc + dxc * dxc <= a.mask_r2);
+    const double xx = -(x - a.center) * a.pix;  // :52-53
+    const double yy = -(y - a.center) * a.pix;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    const int vb = wave * Q, ve = vb + Q;
+    if (masked && Q > 0) {
+        // rows of view v + kQ (mod V); vb + kQ < V + 3Q, one wrap at most
+        auto row_of = [&](int v, int k) {
+            const int r = v + k * Q;
+            return a.fltd + (size_t)(r >= a.V ? r - a.V : r) * a.W;
+        };
+        BpView g = bp_view<WIDE>(a, xx, yy, vb);
+        float r0[4], r1[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float* row = row_of(vb, k) + g.col;
+            r0[k] = row[0];
+            r1[k] = row[1];
+        }
+#pragma unroll 1
+        for (int v = vb; v < ve; v++) {
+            const int vn = min(v + 1, ve - 1);
+            const BpView gn = bp_view<WIDE>(a, xx, yy, vn);
+            float n0[4], n1[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float* row = row_of(vn, k) + gn.col;
+                n0[k] = row[0];
+                n1[k] = row[1];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double sum = g.w0 * r0[k] + g.w * r1[k];  // + 0.0 left out as in k_backproject
+                const float term = (float)(g.rL2 * sum * a.dbeta_view * a.xfactor);
+                acc[k] = g.ok ? acc[k] + term : acc[k];
+            }
+            g = gn;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                r0[k] = n0[k];
+                r1[k] = n1[k];
+            }
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) part[wave - 1][k][lane] = acc[k];
+    }
+    __syncthreads();
+    if (wave == 0 && in_q) {
+        const int n1 = a.N - 1;
+        const int px[4] = {x, n1 - y, n1 - x, y};
+        const int py[4] = {y, x, n1 - y, n1 - x};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float v = ((acc[k] + part[0][k][lane]) + part[1][k][lane]) + part[2][k][lane];
+            a.img[(size_t)py[k] * a.N + px[k]] = v;
+        }
+    }
+}
+
 // The beam-hardening tool M/beam_hardening_correction.cpp:47-60 elementwise:
 // y = -ln p (single-precision log, as the tool's C++ overload resolution
 // gives), y' = |y| + alpha |y|^beta, out = exp(-y') in FP64, stored as float.
@@ -431,6 +519,12 @@ static void launch_bp_w(const BackprojArgs& a, int S, hipStream_t s)
     int slices = (S + kBpSinos - 1) / kBpSinos;
     if (slices < floor_slices) slices = (int)std::min<long long>(S, floor_slices);
     const int K = (S + slices - 1) / slices;
+    if (S == 1 && !a.exact && a.V % 4 == 0 && a.N % 2 == 0 && a.use_views == a.V && a.N >= 2) {
+        const int h = a.N / 2;
+        const dim3 grid((h + 7) / 8, (h + 7) / 8);
+        hipLaunchKernelGGL(k_backproject_sym<WIDE>, grid, dim3(256), 0, s, a);
+        return;
+    }
     if (K == 1) {
         const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, S);
         hipLaunchKernelGGL(k_backproject<WIDE>, grid, dim3(kBpTile * kBpTile), 0, s, a);

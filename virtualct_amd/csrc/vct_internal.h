@@ -53,6 +53,7 @@ struct TraceArgs {
     int max_steps;        // hard bound on marching steps per ray
     int rcheck;           // 1 when the reference's r >= rmax test can fire
     int exact;            // 1 forces the exact-division march
+    int rot;              // forward: rays per orbit of k_trace_fwd_rot (1: one ray per lane)
     int ray_offset;       // first ray handled by this launch
     int n_rays;           // rays handled by this launch
     double D;             // pixel pitch
@@ -107,6 +108,7 @@ struct BackprojArgs {
     const double* sinT;   // [V]
     float* img;           // [S][N][N]
     int N, W, V, use_views, cent, half;
+    int exact;            // 1: views in the reference's order per pixel, no 4-fold sharing
     double pix, sid, dbeta_ch, inv_dbeta_ch, dbeta_view, xfactor, center, mask_r2, mask_r;
 };
 
@@ -157,6 +159,8 @@ struct vct_plan {
     float* d_toep[3] = {nullptr, nullptr, nullptr};  // Toeplitz generator per type
     int exact_trace = 0;  // VCT_OPT_EXACT_TRACE
     int max_steps = 0;    // VCT_OPT_MAX_STEPS (0: the automatic bound)
+    int exact_bp = 0;     // VCT_OPT_EXACT_BACKPROJECT
+    int trace_orbit = 4;  // VCT_OPT_TRACE_ORBIT
 };
 
 namespace vct {

@@ -27,10 +27,23 @@
 // makes the points involved exact.  The general step, given the same fast
 // values and no failed test, takes the same decisions, so both routes give
 // the same segments.
+//
+// Orbit sinks (kRot): with V views over 360 degrees, the ray of view v + kV/R
+// is the ray of view v turned by k/R of a turn about the centre, so its
+// segments are the turned voxels of view v's segments with the same lengths,
+// provided every discrete decision of the march holds for the turned ray
+// too.  Its own trig values differ from the exact turn by a few ulp, which
+// moves its coordinates by ~1e-13; the march therefore reports to such a
+// sink every decision that does not clear kMarchMargin (the general step,
+// and in the set-up the clip tests, the entry choice, the major axis, the
+// integer-cm start plane and the direction signs), and the caller then
+// traces the turned rays one by one.  Such sinks take voxels as (major,
+// minor) index pairs.
 #pragma once
 
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 #include "vct_internal.h"
 
@@ -38,9 +51,24 @@ namespace vct {
 
 constexpr double kMarchMargin = 1e-9;
 
+// Sink::kRot, false when the sink does not declare it.
+template <class S, class = void>
+struct sink_rot : std::false_type {};
+template <class S>
+struct sink_rot<S, std::void_t<decltype(S::kRot)>> : std::integral_constant<bool, S::kRot> {};
+
+// |v| and its distance to the nearest integer both clear the margin: trunc(v)
+// and the sign of v do not change under a perturbation below the margin.
+__host__ __device__ __forceinline__ bool trunc_safe(double v)
+{
+    const double u = fabs(v);
+    const double f = u - floor(u);
+    return u >= kMarchMargin && f >= kMarchMargin && f <= 1.0 - kMarchMargin;
+}
+
 // x86-64 cvttsd2si semantics (the reference's (int) casts): NaN and values
 // outside int range give INT_MIN.
-__device__ __forceinline__ int trunc_i32(double v)
+__host__ __device__ __forceinline__ int trunc_i32(double v)
 {
     return (v > -2147483649.0 && v < 2147483648.0) ? (int)v : INT_MIN;
 }
@@ -51,7 +79,7 @@ __device__ __forceinline__ int trunc_i32(double v)
 // so its truncation can only differ within a few ulp of an integer; such
 // values take the exact division.
 template <bool FAST>
-__device__ __forceinline__ int index_of(double h, double D, double invD, double c)
+__host__ __device__ __forceinline__ int index_of(double h, double D, double invD, double c)
 {
     if (FAST) {
         const double q = __builtin_fma(h, invD, c);
@@ -70,7 +98,7 @@ __device__ __forceinline__ int index_of(double h, double D, double invD, double 
 // A NaN or infinite q fails that test (f is NaN), and every index whose value
 // is used comes from a point within a few image widths of the image, far
 // inside the int range, so no separate range test is needed.
-__device__ __forceinline__ bool index_checked(double h, double invD, double c, int& out)
+__host__ __device__ __forceinline__ bool index_checked(double h, double invD, double c, int& out)
 {
     const double q = __builtin_fma(h, invD, c);
     const double t = trunc(q);
@@ -79,10 +107,23 @@ __device__ __forceinline__ bool index_checked(double h, double invD, double c, i
     return fabs(f - 0.5) < 0.5 - kMarchMargin;
 }
 
+// The same, also giving floor(q) (for orbit sinks: the reference's index is
+// trunc(q), which is floor(q) + 1 for q in (-1, 0), and a turned ray's index
+// value is N - q, so the turned indices follow from the floor).
+__host__ __device__ __forceinline__ bool index_checked_fl(double h, double invD, double c, int& out, int& fl)
+{
+    const double q = __builtin_fma(h, invD, c);
+    const double t = trunc(q);
+    const double f = fabs(q - t);
+    out = (int)t;
+    fl = (int)t - (q < t ? 1 : 0);
+    return fabs(f - 0.5) < 0.5 - kMarchMargin;
+}
+
 // (u/fabs(u) != v/fabs(v)) of M/reprojection_TOMO.cpp:134,221 for finite
 // operands, without the divisions: true if either is zero or NaN (0/0 is
 // NaN, which compares unequal) or the signs differ.
-__device__ __forceinline__ bool sign_split(double u, double v)
+__host__ __device__ __forceinline__ bool sign_split(double u, double v)
 {
     const bool up = u > 0.0, un = u < 0.0, vp = v > 0.0, vn = v < 0.0;
     return !(up || un) || !(vp || vn) || (up != vp);
@@ -112,9 +153,10 @@ struct RayConst {
 // go to Sink::add2_sync, which every lane calls once per step (see
 // march_ray_sync); alive_in = false makes a lane march nothing.
 template <bool FAST, bool SYNC = false, class Sink>
-__device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r, Sink& sink, bool& ovf,
+__host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r, Sink& sink, bool& ovf,
                                           bool alive_in = true)
 {
+    constexpr bool kRot = sink_rot<Sink>::value;
     const bool xmajor = r.xmajor;
     const double ai = r.ai, af = r.af, bi = r.bi, bf = r.bf;
     const double sa = r.sa, sb = r.sb, dA = r.dA, dB = r.dB, Lb = r.Lb;
@@ -162,43 +204,90 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
     // fast midpoint index: fma(p0 + p1, +-0.5/D, nxm) is the fma of the
     // reference's +-(p0 + p1) * 0.5 with 1/D (power-of-two scalings are exact)
     const double hA = 0.5 * sA * a.invD, hB = -0.5 * sA * a.invD;
+    // a point whose major coordinate a turned ray reproduces exactly (negated
+    // or equal): a plane, or an entry on a major-axis side of the square
+    auto a_sym = [&](const MarchPoint& p) { return p.kind == 1 || (p.kind == 0 && fabs(p.a) == Lb); };
     auto emit = [&](MarchPoint& p0, MarchPoint& p1) {
-        int ia, ib;
-        bool decided = false;
-        if (FAST) {
-            decided = index_checked(p0.a + p1.a, hA, (double)a.nxm, ia) &&
-                      index_checked(p0.b + p1.b, hB, (double)a.nxm, ib);
-            if (!decided) {
-                make_exact(p0);
-                make_exact(p1);
-            }
-        }
-        if (!decided) {
-            ia = index_of<FAST>(sA * ((p0.a + p1.a) * 0.5), D, a.invD, (double)a.nxm);
-            ib = index_of<FAST>(-sA * ((p0.b + p1.b) * 0.5), D, a.invD, (double)a.nxm);
-        }
-        if (ia >= 0 && ib >= 0 && ia <= a.N - 1 && ib <= a.N - 1) {
-            if (a.rcheck) {
-                // x^2 + y^2 of the reference with the terms in either order
-                const double r = sqrt((p0.a + p1.a) * (p0.a + p1.a) + (p0.b + p1.b) * (p0.b + p1.b)) * 0.5;
-                if (r >= a.rmax) return;
-            }
+        if constexpr (kRot) {
+            // index values: the fast ones when both clear the margin, else the
+            // reference's exact expressions.  A tie of the major index between
+            // two points with exactly reproduced major coordinates resolves the
+            // same way for every turned ray (their sums are exact negations or
+            // equal); any other decision within the margin is left to the
+            // turned rays' own marches.
+            double qa = __builtin_fma(p0.a + p1.a, hA, (double)a.nxm);
+            double qb = __builtin_fma(p0.b + p1.b, hB, (double)a.nxm);
+            const bool ca = fabs(fabs(qa - trunc(qa)) - 0.5) < 0.5 - kMarchMargin;
+            const bool cb = fabs(fabs(qb - trunc(qb)) - 0.5) < 0.5 - kMarchMargin;
             const double rr = FAST ? fabs(p1.a - p0.a) * sec
                                    : sqrt((p0.a - p1.a) * (p0.a - p1.a) + (p0.b - p1.b) * (p0.b - p1.b));
-            sink.add(ia * stA + ib * stB, rr);
+            if (!FAST || !(ca && cb)) {
+                make_exact(p0);
+                make_exact(p1);
+                // the reference's index values are t + nxm with the signed
+                // quotients t below; a turned ray whose coordinates on this
+                // axis are these negated or equal computes +-t exactly (the
+                // division is odd), so the sink gives it fl(+-t + nxm), its own
+                // value: exact for the major axis between planes and major-side
+                // entries, any other tie is left to the turned rays' marches
+                const double ta = sA * ((p0.a + p1.a) * 0.5) / D;
+                const double tb = -sA * ((p0.b + p1.b) * 0.5) / D;
+                if (FAST && (!cb || !(ca || (a_sym(p0) && a_sym(p1))))) sink.unsafe = true;
+                sink.add_t(ta, tb, rr);
+            } else {
+                sink.add_fl((int)floor(qa), (int)floor(qb), rr);
+            }
+        } else {
+            int ia, ib;
+            bool decided = false;
+            if (FAST) {
+                decided = index_checked(p0.a + p1.a, hA, (double)a.nxm, ia) &&
+                          index_checked(p0.b + p1.b, hB, (double)a.nxm, ib);
+                if (!decided) {
+                    make_exact(p0);
+                    make_exact(p1);
+                }
+            }
+            if (!decided) {
+                ia = index_of<FAST>(sA * ((p0.a + p1.a) * 0.5), D, a.invD, (double)a.nxm);
+                ib = index_of<FAST>(-sA * ((p0.b + p1.b) * 0.5), D, a.invD, (double)a.nxm);
+            }
+            if (ia >= 0 && ib >= 0 && ia <= a.N - 1 && ib <= a.N - 1) {
+                if (a.rcheck) {
+                    // x^2 + y^2 of the reference with the terms in either order
+                    const double r = sqrt((p0.a + p1.a) * (p0.a + p1.a) + (p0.b + p1.b) * (p0.b + p1.b)) * 0.5;
+                    if (r >= a.rmax) return;
+                }
+                const double rr = FAST ? fabs(p1.a - p0.a) * sec
+                                       : sqrt((p0.a - p1.a) * (p0.a - p1.a) + (p0.b - p1.b) * (p0.b - p1.b));
+                sink.add(ia * stA + ib * stB, rr);
+            }
         }
     };
 
+    if constexpr (kRot) {
+        // set-up decisions of this function: the integer-cm start plane and the
+        // signs of the entry coordinate and of both directions (an entry on a
+        // major-axis side is exactly +-Lb for every turned ray)
+        if (!((fabs(ai) == Lb || trunc_safe(ai)) && fabs(af - ai) >= kMarchMargin && fabs(bf - bi) >= kMarchMargin))
+            sink.unsafe = true;
+    }
     MarchPoint p0{ai, bi, 0, true};
     MarchPoint pc{(double)ain, 0.0, 1, !FAST};
     pc.b = b_calc(pc.a);
-    if (FAST && fabs(fabs(pc.b) - Lb) < kMarchMargin) make_exact(pc);
+    if (FAST && fabs(fabs(pc.b) - Lb) < kMarchMargin) {
+        make_exact(pc);
+        if constexpr (kRot) sink.unsafe = true;
+    }
     int bma = 0;
     int iks = 0;
 
     const double nxm = (double)a.nxm;
     // 0 <= ia, ib < N in one unsigned comparison (negatives wrap above N)
-    auto in_image = [&](int ia, int ib) { return max((uint32_t)ia, (uint32_t)ib) < (uint32_t)a.N; };
+    auto in_image = [&](int ia, int ib) {
+        const uint32_t ua = (uint32_t)ia, ub = (uint32_t)ib;
+        return (ua > ub ? ua : ub) < (uint32_t)a.N;
+    };
 
     bool alive = alive_in;
     double pik = 0.0;  // pha * ik, exact
@@ -242,10 +331,12 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
                 // chy != 0 of :141: when the step is committed, cand implies
                 // |dxa| or |dxb| >= margin, so chy > 0
                 const bool cross = cand;
-                int ia1, ib1, ia2, ib2;
-                const bool ok1 = index_checked(p0.a + xa, hA, nxm, ia1) & index_checked(p0.b + xb, hB, nxm, ib1);
+                int ia1, ib1, ia2, ib2, fa1, fb1, fa2, fb2;
+                const bool ok1 = index_checked_fl(p0.a + xa, hA, nxm, ia1, fa1) &
+                                 index_checked_fl(p0.b + xb, hB, nxm, ib1, fb1);
                 const double s_a = cross ? xa : p0.a, s_b = cross ? xb : p0.b;
-                const bool ok2 = index_checked(s_a + na, hA, nxm, ia2) & index_checked(s_b + nb, hB, nxm, ib2);
+                const bool ok2 = index_checked_fl(s_a + na, hA, nxm, ia2, fa2) &
+                                 index_checked_fl(s_b + nb, hB, nxm, ib2, fb2);
                 sure &= (ok1 | !cross) & ok2;
                 if (sure) {
                     const bool v1 = cross && in_image(ia1, ib1);
@@ -256,6 +347,10 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
                         er1 = fabs(xa - p0.a) * sec;
                         ev2 = v2 ? ia2 * stA + ib2 * stB : -1;
                         er2 = fabs(na - s_a) * sec;
+                    } else if constexpr (kRot) {
+                        // every candidate segment with the floors of its index values; the
+                        // sink applies each ray's own in-image test (no crossing: length 0)
+                        sink.add2_fl(fa1, fb1, cross ? fabs(xa - p0.a) * sec : 0.0, fa2, fb2, fabs(na - s_a) * sec);
                     } else if constexpr (Sink::kAdd2) {
                         // empty segments: length 0 at voxel 0, which adds nothing
                         sink.add2(v1 ? ia1 * stA + ib1 * stB : 0, v1 ? fabs(xa - p0.a) * sec : 0.0,
@@ -276,10 +371,14 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
                 pc.exact = !FAST;
                 if (FAST) {
                     const double ab = fabs(pc.b);
-                    if (ab < kMarchMargin || fabs(ab - Lb) < kMarchMargin) make_exact(pc);
+                    if (ab < kMarchMargin || fabs(ab - Lb) < kMarchMargin) {
+                        make_exact(pc);
+                        if constexpr (kRot) sink.unsafe = true;
+                    }
                     if (!index_checked(pc.b, a.invD, 0.0, bma)) {
                         make_exact(pc);
                         bma = trunc_i32(pc.b / D);  // :130
+                        if constexpr (kRot) sink.unsafe = true;
                     }
                 } else {
                     bma = index_of<false>(pc.b, D, a.invD, 0.0);
@@ -295,6 +394,7 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
                     if (FAST && fabs(pc.a - px.a) < kMarchMargin && fabs(pc.b - px.b) < kMarchMargin) {
                         make_exact(pc);
                         make_exact(px);
+                        if constexpr (kRot) sink.unsafe = true;
                     }
                     const double chy = (pc.a - px.a) * (pc.a - px.a) + (pc.b - px.b) * (pc.b - px.b);
                     if (chy != 0) {  // :141
@@ -315,7 +415,8 @@ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r,
 
 // Entry, exit and march constants of ray (view, ch) (M/reprojection_TOMO.cpp:53-113);
 // false when the ray misses the image.
-__device__ __forceinline__ bool setup_ray(const TraceArgs& a, int view, int ch, RayConst& r)
+__host__ __device__ __forceinline__ bool setup_ray(const TraceArgs& a, int view, int ch, RayConst& r,
+                                                  bool* safe = nullptr)
 {
     // source and detector point, M/reprojection_TOMO.cpp:53-74
     const double c = a.cosT[view], s = a.sinT[view];
@@ -331,14 +432,19 @@ __device__ __forceinline__ bool setup_ray(const TraceArgs& a, int view, int ch, 
     // intersections are used)
     int m = 0;
     double h0x = 0.0, h0y = 0.0, h1x = 0.0, h1y = 0.0;
+    // every decision clears the margin (for orbit sinks; NaN fails)
+    bool ok = true;
+    auto clear = [&](double d) { ok = ok && fabs(d) >= kMarchMargin; };
     {
         const double y = Lb;
         const double x = (y - sy) / (py - sy) * (px - sx) + sx;
+        clear(fabs(x) - Lb);
         if (fabs(x) < Lb) { h0x = x; h0y = y; m = 1; }
     }
     {
         const double y = -Lb;
         const double x = (y - sy) / (py - sy) * (px - sx) + sx;
+        clear(fabs(x) - Lb);
         if (fabs(x) < Lb) {
             if (m == 0) { h0x = x; h0y = y; } else { h1x = x; h1y = y; }
             m++;
@@ -347,6 +453,7 @@ __device__ __forceinline__ bool setup_ray(const TraceArgs& a, int view, int ch, 
     {
         const double x = Lb;
         const double y = (x - sx) / (px - sx) * (py - sy) + sy;
+        clear(fabs(y) - Lb);
         if (fabs(y) < Lb) {
             if (m == 0) { h0x = x; h0y = y; } else if (m == 1) { h1x = x; h1y = y; }
             m++;
@@ -355,23 +462,28 @@ __device__ __forceinline__ bool setup_ray(const TraceArgs& a, int view, int ch, 
     {
         const double x = -Lb;
         const double y = (x - sx) / (px - sx) * (py - sy) + sy;
+        clear(fabs(y) - Lb);
         if (fabs(y) < Lb) {
             if (m == 0) { h0x = x; h0y = y; } else if (m == 1) { h1x = x; h1y = y; }
             m++;
         }
     }
+    if (safe) *safe = ok;
     if (m < 2) return false;  // the ray misses the image: no segment, transmission 1
 
     // entry = the intersection closer to the source, :103-108
     const double d0 = (h0x - sx) * (h0x - sx) + (h0y - sy) * (h0y - sy);
     const double d1 = (h1x - sx) * (h1x - sx) + (h1y - sy) * (h1y - sy);
     const bool first = d0 < d1;
+    clear(d0 - d1);
     const double xi = first ? h0x : h1x, yi = first ? h0y : h1y;
     const double xf = first ? h1x : h0x, yf = first ? h1y : h0y;
 
     // dominant axis, :113.  a = major axis, b = minor axis; the reference's
     // two branches (:113-200, :201-283) are this march with x and y exchanged.
     const bool xmajor = fabs(xi - xf) > fabs(yi - yf);
+    clear(fabs(xi - xf) - fabs(yi - yf));
+    if (safe) *safe = ok;
     const double ai = xmajor ? xi : yi, af = xmajor ? xf : yf;
     const double bi = xmajor ? yi : xi, bf = xmajor ? yf : xf;
     const double sa = xmajor ? sx : sy, sb = xmajor ? sy : sx;
@@ -389,7 +501,7 @@ __device__ __forceinline__ bool setup_ray(const TraceArgs& a, int view, int ch, 
 // returns the reference's per-ray segment counter (iks).  ovf is set when
 // the step bound is hit (never for valid geometry).
 template <bool FAST_IN, class Sink>
-__device__ __forceinline__ int march_ray(const TraceArgs& a, int view, int ch, Sink& sink, bool& ovf)
+__host__ __device__ __forceinline__ int march_ray(const TraceArgs& a, int view, int ch, Sink& sink, bool& ovf)
 {
     RayConst r;
     if (!setup_ray(a, view, ch, r)) return 0;

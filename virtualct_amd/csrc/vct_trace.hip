@@ -22,7 +22,9 @@
 #include "vct_internal.h"
 #include "vct_march.h"
 
+#include <algorithm>
 #include <climits>
+#include <cstring>
 #include <cmath>
 
 namespace vct {
@@ -122,6 +124,174 @@ struct SinkF32 {
         const Vox q1 = load(v1), q2 = load(v2);
         accumulate(q1, r1);
         accumulate(q2, r2);
+    }
+};
+
+// Orbit sink: the segments of the ray of view v, taken also as the segments
+// of the rays of views v + kV/R (k = 1..R-1, turned by k/R of a turn), each
+// at the turned voxel.  Ray k accumulates rho(voxel) * length as the
+// single-ray sinks do.  The march hands over, for every candidate segment,
+// the floors (fa, fb) of its major and minor index values; the x and y floors
+// (fx, fy) of ray k are those of ray 0 turned: a quarter turn maps (fx, fy)
+// to (N-1-fy, fx).  Each ray then takes the reference's own rule: index
+// trunc(q) = f + (f < 0), in the image when both lie in 0..N-1, i.e.
+// f in -1..N-1 (a midpoint less than a pixel beyond the low edge counts into
+// the edge voxel, one beyond the high edge does not).
+// unsafe: a decision of the march that a turned ray might take otherwise did
+// not clear the margin (the caller then traces rays 1..R-1 by themselves;
+// ray 0 is right in any case).
+template <int M, int DT, int R>
+struct SinkRot {
+    static_assert(R == 1 || R == 2 || R == 4, "orbits of 1, 2 or 4 rays");
+    static constexpr bool kAdd2 = true;
+    static constexpr bool kRot = true;
+    static constexpr int NW = (M + 1) / 2;  // u16: 32-bit words of two materials
+    const char* __restrict__ vox;
+    bool xmajor;
+    int N;
+    bool unsafe;
+    double accw[R][NW], acch[R][M / 2];  // u16: word sums and high halves
+    double acc[R][M];                    // f32: sums; u16: after finish()
+    __host__ __device__ void init(const void* p, bool xm, int n)
+    {
+        vox = reinterpret_cast<const char*>(p);
+        xmajor = xm;
+        N = n;
+        unsafe = false;
+        #pragma unroll
+        for (int k = 0; k < R; k++) {
+            #pragma unroll
+            for (int w = 0; w < NW; w++) accw[k][w] = 0.0;
+            #pragma unroll
+            for (int w = 0; w < M / 2; w++) acch[k][w] = 0.0;
+            #pragma unroll
+            for (int m = 0; m < M; m++) acc[k][m] = 0.0;
+        }
+    }
+    // voxel of ray k (0 with rr = 0 when outside the image)
+    __host__ __device__ __forceinline__ void voxels(int fa, int fb, double rr, int (&v)[R], double (&r)[R]) const
+    {
+        const int fx = xmajor ? fa : fb, fy = xmajor ? fb : fa;
+        const int n1 = N - 1;
+        #pragma unroll
+        for (int k = 0; k < R; k++) {
+            int gx, gy;
+            if (R == 1 || k == 0) { gx = fx; gy = fy; }
+            else if (R == 2 || k == 2) { gx = n1 - fx; gy = n1 - fy; }
+            else if (k == 1) { gx = n1 - fy; gy = fx; }
+            else { gx = fy; gy = n1 - fx; }
+            const bool in = gx >= -1 && gx <= n1 && gy >= -1 && gy <= n1;
+            v[k] = in ? (gx + (gx < 0)) + N * (gy + (gy < 0)) : 0;
+            r[k] = in ? rr : 0.0;
+        }
+    }
+    struct Vox {
+        uint4 lo, hi;  // u16: lo = the 8 materials; f32: lo = materials 0-3, hi = 4-7
+    };
+    __host__ __device__ __forceinline__ Vox load(int v) const
+    {
+        Vox q;
+        if constexpr (DT == VCT_DENS_U16) {
+            q.lo = *reinterpret_cast<const uint4*>(vox + (uint32_t)v * 16u);
+            q.hi = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            const uint4* f = reinterpret_cast<const uint4*>(vox + (size_t)(uint32_t)v * 32u);
+            q.lo = f[0];
+            q.hi = M > 4 ? f[1] : make_uint4(0u, 0u, 0u, 0u);
+        }
+        return q;
+    }
+    __host__ __device__ __forceinline__ void accumulate(int k, const Vox& q, double rr)
+    {
+        if constexpr (DT == VCT_DENS_U16) {
+            const uint32_t w[4] = {q.lo.x, q.lo.y, q.lo.z, q.lo.w};
+            #pragma unroll
+            for (int j = 0; j < NW; j++) {
+                const uint32_t wj = (2 * j + 1 < M) ? w[j] : (w[j] & 0xffffu);
+                accw[k][j] = __builtin_fma((double)wj, rr, accw[k][j]);
+            }
+            #pragma unroll
+            for (int j = 0; j < M / 2; j++) acch[k][j] = __builtin_fma((double)(w[j] >> 16), rr, acch[k][j]);
+        } else {
+            const uint32_t b[8] = {q.lo.x, q.lo.y, q.lo.z, q.lo.w, q.hi.x, q.hi.y, q.hi.z, q.hi.w};
+            #pragma unroll
+            for (int m = 0; m < M; m++) {
+                float rho;
+                memcpy(&rho, &b[m], sizeof(float));
+                acc[k][m] = __builtin_fma((double)fmaxf(rho, 0.f), rr, acc[k][m]);  // as SinkF32
+            }
+        }
+    }
+    // one candidate segment whose index values clear the margin (floors)
+    __host__ __device__ __forceinline__ void add_fl(int fa, int fb, double rr)
+    {
+        int v[R];
+        double r[R];
+        voxels(fa, fb, rr, v, r);
+        Vox q[R];
+        #pragma unroll
+        for (int k = 0; k < R; k++) q[k] = load(v[k]);
+        #pragma unroll
+        for (int k = 0; k < R; k++) accumulate(k, q[k], r[k]);
+    }
+    // one candidate segment of a general march step from the signed quotients
+    // (ta, tb) of its index values t + N/2: ray k takes fl(+-t + N/2) on each
+    // of its axes (minus where the turn flips the axis) and the reference's
+    // trunc and in-image test
+    __host__ __device__ __forceinline__ void add_t(double ta, double tb, double rr)
+    {
+        int v[R];
+        double r[R];
+        voxels_t(ta, tb, rr, v, r);
+        Vox q[R];
+        #pragma unroll
+        for (int k = 0; k < R; k++) q[k] = load(v[k]);
+        #pragma unroll
+        for (int k = 0; k < R; k++) accumulate(k, q[k], r[k]);
+    }
+    __host__ __device__ __forceinline__ void voxels_t(double ta, double tb, double rr, int (&v)[R], double (&r)[R]) const
+    {
+        const double tx = xmajor ? ta : tb, ty = xmajor ? tb : ta;
+        const double c = (double)(N / 2);
+        #pragma unroll
+        for (int k = 0; k < R; k++) {
+            double gx, gy;
+            if (R == 1 || k == 0) { gx = tx + c; gy = ty + c; }
+            else if (R == 2 || k == 2) { gx = -tx + c; gy = -ty + c; }
+            else if (k == 1) { gx = -ty + c; gy = tx + c; }
+            else { gx = ty + c; gy = -tx + c; }
+            const int ix = trunc_i32(gx), iy = trunc_i32(gy);
+            const bool in = ix >= 0 && ix <= N - 1 && iy >= 0 && iy <= N - 1;
+            v[k] = in ? ix + N * iy : 0;
+            r[k] = in ? rr : 0.0;
+        }
+    }
+    __host__ __device__ __forceinline__ void add2_fl(int fa1, int fb1, double r1, int fa2, int fb2, double r2)
+    {
+        int v1[R], v2[R];
+        double q1r[R], q2r[R];
+        voxels(fa1, fb1, r1, v1, q1r);
+        voxels(fa2, fb2, r2, v2, q2r);
+        Vox q1[R], q2[R];
+        #pragma unroll
+        for (int k = 0; k < R; k++) q1[k] = load(v1[k]);
+        #pragma unroll
+        for (int k = 0; k < R; k++) q2[k] = load(v2[k]);
+        #pragma unroll
+        for (int k = 0; k < R; k++) accumulate(k, q1[k], q1r[k]);
+        #pragma unroll
+        for (int k = 0; k < R; k++) accumulate(k, q2[k], q2r[k]);
+    }
+    __host__ __device__ __forceinline__ void finish()
+    {
+        if constexpr (DT == VCT_DENS_U16) {
+            #pragma unroll
+            for (int k = 0; k < R; k++)
+                #pragma unroll
+                for (int m = 0; m < M; m++)
+                    acc[k][m] = (m & 1) ? acch[k][m >> 1]
+                                        : accw[k][m >> 1] - (m / 2 < M / 2 ? 65536.0 * acch[k][m >> 1] : 0.0);
+        }
     }
 };
 
@@ -237,6 +407,96 @@ __global__ __launch_bounds__(256) void k_trace_fwd(TraceArgs a)
     dst[0] = make_float4(out[0], out[1], out[2], out[3]);
     dst[1] = make_float4(out[4], out[5], out[6], out[7]);
     if (a.counts) a.counts[ray] = iks;
+}
+
+// One ray of k_trace_fwd, written to L (and counts).
+template <int M, int DT>
+__device__ __forceinline__ void trace_one(const TraceArgs& a, int view, int ch)
+{
+    const int ray = view * a.W + ch;
+    float out[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int iks = 0;
+    bool ovf = false;
+    if (DT == VCT_DENS_U16) {
+        SinkU16<M> sink;
+        sink.init(a.dens);
+        iks = march_ray<true>(a, view, ch, sink, ovf);
+        sink.finish();
+#pragma unroll
+        for (int m = 0; m < M; m++) out[m] = (float)(sink.acc[m] * a.out_scale);
+    } else {
+        SinkF32<M> sink;
+        sink.init(a.dens);
+        iks = march_ray<true>(a, view, ch, sink, ovf);
+#pragma unroll
+        for (int m = 0; m < M; m++) out[m] = (float)(sink.acc[m] * a.out_scale);
+    }
+    if (ovf) atomicOr(a.err, 1);
+    float4* dst = reinterpret_cast<float4*>(a.L) + (size_t)ray * 2;
+    dst[0] = make_float4(out[0], out[1], out[2], out[3]);
+    dst[1] = make_float4(out[4], out[5], out[6], out[7]);
+    if (a.counts) a.counts[ray] = iks;
+}
+
+// Orbits of R rays: lane (view v < V/R, channel) marches its ray once and
+// gives the rays of views v + kV/R their path integrals from the turned
+// voxels (SinkRot).  When the march reports a decision within the margin,
+// or the ray needs the exact march, the lane traces rays 1..R-1 (and its
+// own, unless marched) one by one.  Every ray's segments are the
+// reference's; the orbit shares the base ray's lengths, which differ from a
+// ray's own |da| * sec by rounding.
+template <int M, int DT, int R>
+__global__ __launch_bounds__(256) void k_trace_fwd_rot(TraceArgs a)
+{
+    const int local = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Vq = a.V / R;
+    if (local >= Vq * a.W) return;
+    const int view = local / a.W;
+    const int ch = local - view * a.W;
+    RayConst r;
+    bool safe = false;
+    const bool hit = setup_ray(a, view, ch, r, &safe);
+    bool done = false;
+    if (safe && !hit) {
+        // the whole orbit misses the image: no segment, path integrals 0
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int ray = (view + k * Vq) * a.W + ch;
+            float4* dst = reinterpret_cast<float4*>(a.L) + (size_t)ray * 2;
+            dst[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+            dst[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (a.counts) a.counts[ray] = 0;
+        }
+        done = true;
+    } else if (safe && r.dB != 0.0 && !a.rcheck && !a.exact) {
+        SinkRot<M, DT, R> sink;
+        sink.init(a.dens, r.xmajor, a.N);
+        bool ovf = false;
+        const int iks = march_loop<true>(a, r, sink, ovf);
+        if (ovf) atomicOr(a.err, 1);
+        sink.finish();
+        // ray 0 is right in any case (its own march); the turned rays only
+        // when no decision was within the margin
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            if (k > 0 && sink.unsafe) break;
+            const int ray = (view + k * Vq) * a.W + ch;
+            float out[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int m = 0; m < M; m++) out[m] = (float)(sink.acc[k][m] * a.out_scale);
+            float4* dst = reinterpret_cast<float4*>(a.L) + (size_t)ray * 2;
+            dst[0] = make_float4(out[0], out[1], out[2], out[3]);
+            dst[1] = make_float4(out[4], out[5], out[6], out[7]);
+            if (a.counts) a.counts[ray] = iks;
+        }
+        if (sink.unsafe) {
+            for (int k = 1; k < R; k++) trace_one<M, DT>(a, view + k * Vq, ch);
+        }
+        done = true;
+    }
+    if (!done) {
+        for (int k = 0; k < R; k++) trace_one<M, DT>(a, view + k * Vq, ch);
+    }
 }
 
 template <int M, bool FAST>
@@ -482,12 +742,30 @@ __global__ __launch_bounds__(256) void k_pack_f32(const float* __restrict__ src,
 
 // --------------------------------------------------------------- launchers
 
+template <int M, int DT, int R>
+static void launch_trace_rot(const TraceArgs& a, hipStream_t s)
+{
+    const long long lanes = (long long)(a.V / R) * a.W;
+    hipLaunchKernelGGL((k_trace_fwd_rot<M, DT, R>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, a);
+}
+
 template <int M>
 static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint, hipStream_t s)
 {
     const dim3 block(256);
     const dim3 grid((a.n_rays + 255) / 256);
-    if (adjoint) {
+    const bool whole = a.ray_offset == 0 && a.n_rays == a.V * a.W;
+    if (!adjoint && whole && (a.rot == 4 || a.rot == 2)) {
+        if (a.V % a.rot != 0 || a.N % 2 != 0 || a.VT != a.V || a.exact || a.rcheck)
+            return set_error(VCT_ERR_INTERNAL, "orbit trace with V %d, N %d", a.V, a.N);
+        if (dens_type == VCT_DENS_U16) {
+            if (a.rot == 4) launch_trace_rot<M, VCT_DENS_U16, 4>(a, s);
+            else launch_trace_rot<M, VCT_DENS_U16, 2>(a, s);
+        } else {
+            if (a.rot == 4) launch_trace_rot<M, VCT_DENS_F32, 4>(a, s);
+            else launch_trace_rot<M, VCT_DENS_F32, 2>(a, s);
+        }
+    } else if (adjoint) {
         hipLaunchKernelGGL((k_trace_adj<M, true>), grid, block, 0, s, a);
     } else if (dens_type == VCT_DENS_U16) {
         hipLaunchKernelGGL((k_trace_fwd<M, VCT_DENS_U16, true>), grid, block, 0, s, a);
