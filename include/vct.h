@@ -214,6 +214,22 @@ vct_status vct_mbfpa_forward(vct_plan* plan, const void* d_packed, int dens_type
                              double sigma, uint64_t seed, uint32_t stream_id,
                              float* d_sino, void* d_work, void* stream);
 
+/* The same forward without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * work buffer is one static device array per GPU of
+ * vct_noalloc_scratch_bytes() bytes (64 MiB: the path integrals, V*W*32 B,
+ * plus n_spectra*V*W*4 B of raw sums, e.g. up to 26 spectra at 800 x 609),
+ * resolved by vct_init and held until the process ends; larger requests
+ * return VCT_ERR_ARG.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (other streams, other threads, other contexts) race;
+ * each call overwrites what it reads. */
+size_t vct_noalloc_scratch_bytes(void);
+vct_status vct_mbfpa_forward_noalloc(vct_plan* plan, const void* d_packed, int dens_type, int M,
+                                     const float* d_att, const float* d_pdf, int NE, int n_spectra,
+                                     double sigma, uint64_t seed, uint32_t stream_id, float* d_sino,
+                                     void* stream);
+
 /* FBP filter, replacing filterSinogram (F/filterSinogramTOMO.cpp:205-275):
  * fltd[v][m] = sum_k C[m][k] * (-ln proj[v][k] * sid * cos((k-W/2) dbeta)) * dbeta
  * as an FP32 MFMA GEMM over n_sino stacked sinograms.  in_mode != 0 applies

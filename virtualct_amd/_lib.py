@@ -77,6 +77,9 @@ _SIGS = {
     "vct_spectral_sum": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
     "vct_detector": (C.c_int, [_P, _P, _P, C.c_int, C.c_double, C.c_uint64, C.c_uint32, _P, _P]),
     "vct_forward_workspace_bytes": (C.c_size_t, [_P, C.c_int]),
+    "vct_noalloc_scratch_bytes": (C.c_size_t, []),
+    "vct_mbfpa_forward_noalloc": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_double,
+                                             C.c_uint64, C.c_uint32, _P, _P]),
     "vct_mbfpa_forward": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_double,
                                      C.c_uint64, C.c_uint32, _P, _P, _P]),
     "vct_filter_sinogram": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -13,9 +14,37 @@
 
 #include "vct_internal.h"
 
+// Work buffer of vct_mbfpa_forward_noalloc (path integrals, raw sums and
+// their statistics), one copy per GPU, held until the process ends.  Host
+// code never uses the name itself (on the host it is a stand-in, not a device
+// address): vct_init resolves each device's copy with hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace vct {
 
 static thread_local std::string g_last_error;
+
+// Device address of `scratch` per device, resolved once with that device
+// current (vct_init).
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static vct_status noalloc_scratch_for(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices) return set_error(VCT_ERR_UNSUPPORTED, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(VCT_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return VCT_OK;
+}
 
 vct_status set_error(vct_status s, const char* fmt, ...)
 {
@@ -269,9 +298,12 @@ vct_status vct_init(int device, vct_ctx** out)
     VCT_HIP(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return set_error(VCT_ERR_UNSUPPORTED, "device %d is %s; this build targets gfx950", device, prop.gcnArchName);
+    void* noalloc = nullptr;
+    VCT_CHECK(noalloc_scratch_for(device, &noalloc));  // with the device current
     vct_ctx* c = new vct_ctx();
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
+    c->noalloc_scratch = noalloc;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_err, sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->d_err, 0, sizeof(int));
@@ -587,6 +619,26 @@ size_t vct_filter_workspace_bytes(const vct_plan* p, int n_sino)
     if (!p || n_sino < 1) return 0;
     const size_t R = (size_t)n_sino * p->g.n_views;
     return (R + 31) / 32 * 32 * p->Kp * sizeof(float);
+}
+
+size_t vct_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+vct_status vct_mbfpa_forward_noalloc(vct_plan* p, const void* d_packed, int dens_type, int M, const float* d_att,
+                                     const float* d_pdf, int NE, int S, double sigma, uint64_t seed,
+                                     uint32_t stream_id, float* d_sino, void* stream)
+{
+    clear_error();
+    if (!p) return set_error(VCT_ERR_ARG, "plan is NULL");
+    if (S < 1 || S > 32) return set_error(VCT_ERR_ARG, "n_spectra %d outside 1..32", S);
+    const size_t need = vct_forward_workspace_bytes(p, S);
+    if (need > kNoAllocScratchBytes)
+        return set_error(VCT_ERR_ARG,
+                         "%d spectra of %d x %d rays need %zu bytes of work space; the no-allocation forward "
+                         "holds %zu (use vct_mbfpa_forward with a workspace)",
+                         S, p->g.n_views, p->g.n_channels, need, kNoAllocScratchBytes);
+    if (!p->ctx->noalloc_scratch) return set_error(VCT_ERR_INTERNAL, "no-allocation scratch not resolved");
+    return vct_mbfpa_forward(p, d_packed, dens_type, M, d_att, d_pdf, NE, S, sigma, seed, stream_id, d_sino,
+                             p->ctx->noalloc_scratch, stream);
 }
 
 static vct_status filter_impl(vct_plan* p, const float* d_proj, int n_sino, int filter, int in_mode, int bhc,

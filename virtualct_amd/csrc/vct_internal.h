@@ -137,6 +137,7 @@ struct vct_ctx {
     // so a kernel recorded with an older scratch pointer (a captured HIP
     // graph) never writes into freed memory.
     std::vector<void*> retired;
+    void* noalloc_scratch = nullptr;  // this device's copy of the no-allocation forward's buffer
     int num_cu = 0;
     vct_status ensure_scratch(size_t bytes);
 };
