@@ -35,12 +35,12 @@ MFMA_F32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 SEGMENTS_512 = 282_824_404  # sum of per-ray segments at 512^2, 0.1 cm, 800x609
 # HBM bytes per tracer launch on this workload, from rocprofv3 --pmc passes
 # over `bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-batched
-# --no-batch-fbp --copy-probe 256`: FETCH_SIZE 27535.3 KiB x 2.00 + WRITE_SIZE 15225.0 KiB
+# --no-batch-fbp --copy-probe 256`: FETCH_SIZE 23131.1 KiB x 2.00 + WRITE_SIZE 15225.1 KiB
 # x 1.00, the factors being what the same counters read for the 256 MiB
-# device copy of the same run (131087.8 KiB fetched, 262144 KiB written).
-# The densities (4 MB) stay in L2/MALL: each XCD fetches them about twice,
-# and the write is the 15.6 MB of path integrals.
-TRACE_TRAFFIC_BYTES = int((27535.3 * 2.0 + 15225.0 * 1.0) * 1024)
+# device copy of the same run (131087.9 KiB fetched, 262144 KiB written).
+# The densities (4 MB) stay in L2/MALL: each XCD fetches them about once or
+# twice, and the write is the 15.6 MB of path integrals.
+TRACE_TRAFFIC_BYTES = int((23131.1 * 2.0 + 15225.1 * 1.0) * 1024)
 METRIC = "MBFPA ray·energy-bins/s, FBP voxel·views/s; batched grad steps/s vs plain"
 
 
@@ -376,7 +376,7 @@ def main():
         "mbfpa_ray_energy_bins_per_s": rays_bins / (mbfpa_ms * 1e-3),
         "fbp_voxel_views_per_s": p_mask * V / (fbp_ms * 1e-3),
         "roofline": {
-            "kernel": "k_trace_fwd (tracer)",
+            "kernel": "k_trace_fwd_rot (tracer, quarter-turn orbits)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -385,11 +385,12 @@ def main():
             "traffic": TRACE_TRAFFIC_BYTES,
             "algorithmic_bytes_per_launch": trace_bytes,
             "note": "algorithmic bytes = segments x 6 materials x 2 B (u16 densities) + V x W x 4 B; the "
-                    "densities are L2/MALL-resident (traffic = measured HBM bytes); the kernel is FP64 "
-                    "VALU-issue bound (~76% VALU busy)",
+                    "densities are L2/MALL-resident (traffic = measured HBM bytes); the kernel is bound by "
+                    "FP64 VALU issue and its latency (68 VALU instructions per segment, 57% VALU busy "
+                    "with under two waves per SIMD)",
         },
         "rooflines": [
-            {"kernel": "k_trace_fwd (tracer)", "bound": "hbm", "unit": "GB/s", "achieved": achieved,
+            {"kernel": "k_trace_fwd_rot (tracer, quarter-turn orbits)", "bound": "hbm", "unit": "GB/s", "achieved": achieved,
              "peak": HBM_PEAK_GBS, "frac": achieved / HBM_PEAK_GBS,
              "per_launch": "segments x M x 2 B + V x W x 4 B"},
             {"kernel": "k_filter_prep + k_filter_gemm (filter stage)", "bound": "mfma", "unit": "TFLOP/s",
