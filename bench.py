@@ -206,6 +206,45 @@ def run_gathers(pairs=3, B=16):
     sm.close()
 
 
+def bench_adjoint(calls=5, announce=False):
+    """One per-ray-weighted adjoint at 512^2 x 800 x 609 (6 materials, the
+    sign test on): the ray-driven tile form (no extra memory) and the gather
+    over the plan's cached system matrix, each timed with HIP events over
+    `calls` calls after one untimed call."""
+    import torch
+
+    from virtualct_amd import default_geometry, ops, phantoms
+
+    dev = torch.device("cuda", 0)
+    plan = ops.plan_for(default_geometry())
+    M = 6
+    g = torch.Generator(device=dev).manual_seed(9)
+    w = torch.zeros((plan.V, plan.W, 8), device=dev)
+    w[..., :M] = torch.randn((plan.V, plan.W, M), generator=g, device=dev)
+    dens = torch.from_numpy(phantoms.densities_f32(phantoms.body_phantom(5000))).to(dev)
+    pk, _ = ops.pack_densities(plan, dens)
+    grad = torch.zeros((plan.N, plan.N, 8), device=dev)
+    out = {}
+    for name, fn in (("tile", lambda: ops.trace_adjoint(plan, w, M, dens_packed_f32=pk, grad=grad, check=False)),
+                     ("sysmat_gather", lambda: ops.sysmat_adjoint(plan, w, M, dens_packed_f32=pk))):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(calls):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name + "_ms"] = e0.elapsed_time(e1) / calls
+        if announce:
+            print(f"adjoint {name}: {calls} calls done", flush=True)
+    ops.check_errors()
+    out["segments"] = SEGMENTS_512
+    out["note"] = "tile = k_trace_adj_tile (32x32 image tiles x view chunks, windowed marches, LDS atomics); " \
+                  "sysmat_gather = atomic-free gather over the cached voxel-major system matrix (4.5 GB)"
+    return out
+
+
 def bench_batch_fbp(per_size=16):
     """64 phantoms of uneven size x 3 spectra: one graph replay vs a loop of
     plain per-phantom GPU calls (forward with 3 spectra + FBP)."""
@@ -263,6 +302,8 @@ def main():
     ap.add_argument("--gathers-only", action="store_true",
                     help="run only the batched gather workload (16 phantoms, blocked matrix, 3 forward + adjoint "
                          "pairs) for a profiler; prints no JSON line")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint timing (512^2, tile form and system-matrix gather) and print it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="run one device-to-device copy of MB MiB first (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -272,6 +313,9 @@ def main():
         sys.exit(2)
     if args.gathers_only:
         run_gathers()
+        return
+    if args.adjoint_only:
+        print(json.dumps(bench_adjoint(announce=True)), flush=True)
         return
 
     import numpy as np
@@ -407,6 +451,11 @@ def main():
             out["batched_decomposition"] = bench_decomposition()
         except Exception as e:
             log("batched decomposition skipped:", repr(e))
+    if not args.no_batched:
+        try:
+            out["adjoint_512"] = bench_adjoint()
+        except Exception as e:
+            log("adjoint timing skipped:", repr(e))
     if not args.no_batch_fbp:
         try:
             out["batched_forward_fbp"] = bench_batch_fbp()

@@ -203,3 +203,58 @@ def test_sysmat_adjoint_mode_matches_fp64_and_is_deterministic(gpu, golden, orac
     assert rel_l2(grads[0], ref) < 1e-4
     assert rel_max(grads[0], ref) < 1e-4
     assert np.all(grads[0][2, 60:64, 60:64] == 0)
+
+
+@pytest.mark.parametrize("N,pix,V,exact", [
+    (96, 50.0 / 96, 801, False),   # odd view count (the last view is not traced), N not a multiple of 32
+    (100, 0.3, 50, False),         # few views, N not a multiple of 32
+    (64, 0.78125, 200, False),     # image corners beyond N/2 cm: the r >= rmax test, exact march
+    (128, 0.35, 160, True),        # the exact-division march forced on every ray
+])
+def test_tile_adjoint_geometries_match_fp64(gpu, golden, oracle, N, pix, V, exact):
+    """The atomic adjoint (tile-resident: each workgroup marches only the
+    steps of the rays that can land in its 32x32 tile) against the FP64
+    adjoint on geometries where the tiling meets its edge cases."""
+    from virtualct_amd import _lib
+    ops = _ops()
+    ph, tabs = _problem(golden, gpu, N, pix, V)
+    dens = ph.astype(np.float32) / np.float32(1000)
+    att, pdf, _ = tabs[1]
+    plan = ops.Plan(_geom(N, pix, V))
+    plan.adjoint_mode = "atomic"
+    if exact:
+        plan.set_option(_lib.OPT_EXACT_TRACE, 1)
+    rng = np.random.default_rng(31 + N)
+    gI = rng.normal(size=(V, 609))
+    print(f"tile adjoint N={N} pix={pix:.4f} V={V} exact={exact}", flush=True)
+    x = _t(dens, gpu).requires_grad_(True)
+    I = ops.mbfpa_intensity(x, plan, _t(att, gpu), _t(pdf, gpu))
+    (I[0] * _t(gI.astype(np.float32), gpu)).sum().backward()
+    got = x.grad.cpu().numpy()
+    ref = oracle.adjoint_f64(Geom(N=N, pix=pix, V=V), dens, att, pdf, gI)
+    assert rel_l2(got, ref) < 1e-4
+    assert rel_max(got, ref) < 1e-4
+
+
+def test_tile_adjoint_equals_sysmat_gather_at_512(gpu):
+    """At 512^2 x 800 x 609 the atomic (tile) adjoint and the gather over the
+    system matrix (pinned to the FP64 adjoint at 128^2) sum the same segments
+    in different orders: equal to float rounding, with the sign test."""
+    import torch
+    ops = _ops()
+    plan = ops.plan_for(_geom())
+    g = torch.Generator(device=gpu).manual_seed(5)
+    M = 6
+    w = torch.zeros((800, 609, 8), device=gpu)
+    w[..., :M] = torch.rand((800, 609, M), generator=g, device=gpu) - 0.25
+    dens = torch.rand((512, 512, 8), generator=g, device=gpu)
+    dens[100:140, 200:260, 3] = -1.0
+    tile = ops.trace_adjoint(plan, w, M, dens_packed_f32=dens)
+    gather = ops.sysmat_adjoint(plan, w, M, dens_packed_f32=dens)
+    torch.cuda.synchronize()
+    a = tile[..., :M].cpu().numpy().astype(np.float64)
+    b = gather[..., :M].cpu().numpy().astype(np.float64)
+    assert rel_l2(a, b) < 1e-6
+    assert np.abs(a - b).max() < 1e-5 * np.abs(b).max()
+    assert np.all(a[100:140, 200:260, 3] == 0)
+    assert np.all(tile[..., M:].cpu().numpy() == 0)

@@ -57,6 +57,13 @@ struct sink_rot : std::false_type {};
 template <class S>
 struct sink_rot<S, std::void_t<decltype(S::kRot)>> : std::integral_constant<bool, S::kRot> {};
 
+// Sink::kAB: the sink takes in-image segments as add_ab(major index, minor
+// index, length) instead of add(voxel, length).
+template <class S, class = void>
+struct sink_ab : std::false_type {};
+template <class S>
+struct sink_ab<S, std::void_t<decltype(S::kAB)>> : std::integral_constant<bool, S::kAB> {};
+
 // |v| and its distance to the nearest integer both clear the margin: trunc(v)
 // and the sign of v do not change under a perturbation below the margin.
 __host__ __device__ __forceinline__ bool trunc_safe(double v)
@@ -152,11 +159,20 @@ struct RayConst {
 // SYNC: the loop is wave-uniform and the segments of a committed fast step
 // go to Sink::add2_sync, which every lane calls once per step (see
 // march_ray_sync); alive_in = false makes a lane march nothing.
+// Window: only steps k_begin..k_end (the loop counter ik of :115) run.  Step
+// k's segments lie between major planes k - 1 and k (step 0: from the entry
+// to plane 0), and the state the march carries into step k is plane point
+// k - 1 with its minor cell, so a window starts from that point, built as the
+// general step builds it; consecutive windows then give the segments of the
+// whole march (lengths equal, or within an ulp where a plane point was made
+// exact for a decision of the step before), and the returned counter is
+// meaningful for the whole march only.
 template <bool FAST, bool SYNC = false, class Sink>
 __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r, Sink& sink, bool& ovf,
-                                          bool alive_in = true)
+                                          bool alive_in = true, int k_begin = 0, int k_end = INT_MAX)
 {
     constexpr bool kRot = sink_rot<Sink>::value;
+    constexpr bool kTakesAB = sink_ab<Sink>::value;
     const bool xmajor = r.xmajor;
     const double ai = r.ai, af = r.af, bi = r.bi, bf = r.bf;
     const double sa = r.sa, sb = r.sb, dA = r.dA, dB = r.dB, Lb = r.Lb;
@@ -260,7 +276,8 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
                 }
                 const double rr = FAST ? fabs(p1.a - p0.a) * sec
                                        : sqrt((p0.a - p1.a) * (p0.a - p1.a) + (p0.b - p1.b) * (p0.b - p1.b));
-                sink.add(ia * stA + ib * stB, rr);
+                if constexpr (kTakesAB) sink.add_ab(ia, ib, rr);
+                else sink.add(ia * stA + ib * stB, rr);
             }
         }
     };
@@ -289,9 +306,35 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
         return (ua > ub ? ua : ub) < (uint32_t)a.N;
     };
 
+    int ik_first = 0;
+    if (k_begin > 0) {
+        // windowed start: plane point k_begin - 1 as the general step makes it
+        // (a committed fast step leaves the same point and cell: it commits
+        // only when none of these tests fires)
+        pc.a = ain + (pha * (double)(k_begin - 1)) * D;
+        pc.b = b_calc(pc.a);
+        pc.exact = !FAST;
+        if (FAST) {
+            const double ab = fabs(pc.b);
+            if (ab < kMarchMargin || fabs(ab - Lb) < kMarchMargin) {
+                make_exact(pc);
+                if constexpr (kRot) sink.unsafe = true;
+            }
+            if (!index_checked(pc.b, a.invD, 0.0, bma)) {
+                make_exact(pc);
+                bma = trunc_i32(pc.b / D);
+                if constexpr (kRot) sink.unsafe = true;
+            }
+        } else {
+            bma = index_of<false>(pc.b, D, a.invD, 0.0);
+        }
+        p0 = pc;
+        ik_first = k_begin;
+    }
+
     bool alive = alive_in;
-    double pik = 0.0;  // pha * ik, exact
-    for (int ik = 0;; ik++, pik += pha) {
+    double pik = pha * (double)ik_first;  // pha * ik, exact
+    for (int ik = ik_first;; ik++, pik += pha) {
         if constexpr (SYNC) {
             // wave-uniform loop: finished lanes idle until every lane is done
             if (alive && !(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) alive = false;
@@ -303,6 +346,7 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
         } else {
             if (!(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) break;
             if (ik >= a.max_steps) { ovf = true; break; }
+            if (ik > k_end) break;
         }
         int ev1 = -1, ev2 = -1;  // SYNC: this step's committed segments (-1: none)
         double er1 = 0.0, er2 = 0.0;
@@ -347,6 +391,9 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
                         er1 = fabs(xa - p0.a) * sec;
                         ev2 = v2 ? ia2 * stA + ib2 * stB : -1;
                         er2 = fabs(na - s_a) * sec;
+                    } else if constexpr (kTakesAB) {
+                        if (v1) sink.add_ab(ia1, ib1, fabs(xa - p0.a) * sec);
+                        if (v2) sink.add_ab(ia2, ib2, fabs(na - s_a) * sec);
                     } else if constexpr (kRot) {
                         // every candidate segment with the floors of its index values; the
                         // sink applies each ray's own in-image test (no crossing: length 0)
@@ -494,6 +541,112 @@ __host__ __device__ __forceinline__ bool setup_ray(const TraceArgs& a, int view,
     r.sa = sa; r.sb = sb; r.dA = pa - sa; r.dB = pb - sb;
     r.Lb = Lb;
     return true;
+}
+
+// Steps k0..k1 of ray r whose segments can have their midpoint voxel in the
+// box x_lo..x_hi, y_lo..y_hi (cm; the caller pads it by two voxels, which
+// covers the reference's truncation of negative index values and the
+// midpoint's distance from the box).  Step k's segments lie between major
+// planes k - 1 and k at ain + pha * k * D (step 0 from the entry), so a
+// midpoint at major coordinate u belongs to step ceil(pha * (u - ain) / D);
+// the range of u where the ray is inside the box gives the steps, widened by
+// one for rounding.  false: the ray has no such step.
+__host__ __device__ __forceinline__ bool box_steps(const TraceArgs& a, const RayConst& r, double x_lo, double x_hi,
+                                                   double y_lo, double y_hi, int& k0, int& k1)
+{
+    const double df = r.af - r.ai;
+    if (!(df != 0.0)) {  // no major extent (or NaN): the whole march
+        k0 = 0;
+        k1 = INT_MAX;
+        return true;
+    }
+    double lo = r.xmajor ? x_lo : y_lo, hi = r.xmajor ? x_hi : y_hi;
+    const double b_lo = r.xmajor ? y_lo : x_lo, b_hi = r.xmajor ? y_hi : x_hi;
+    if (r.dB != 0.0) {
+        const double g = r.dA / r.dB;
+        const double u = r.sa + (b_lo - r.sb) * g, w = r.sa + (b_hi - r.sb) * g;
+        lo = fmax(lo, fmin(u, w));
+        hi = fmin(hi, fmax(u, w));
+    } else if (!(r.sb >= b_lo && r.sb <= b_hi)) {
+        return false;
+    }
+    if (!(lo <= hi)) return false;
+    // start plane of march_loop (M/reprojection_TOMO.cpp:115-119)
+    const double pha = df / fabs(df);
+    const int ain = ((r.ai < 0 && df < 0) || (r.ai > 0 && df > 0)) ? trunc_i32(trunc_i32(r.ai) + pha)
+                                                                  : trunc_i32(r.ai);
+    const double t1 = pha * (lo - ain) / a.D, t2 = pha * (hi - ain) / a.D;
+    const double tmin = fmin(t1, t2), tmax = fmax(t1, t2);
+    if (!(tmax < 1.0e9 && tmin > -1.0e9)) {
+        k0 = 0;
+        k1 = INT_MAX;
+        return true;
+    }
+    k0 = (int)ceil(tmin) - 1;
+    k1 = (int)ceil(tmax) + 1;
+    if (k1 < 0) k1 = 0;  // step 0 runs from the entry, before plane 0
+    if (k0 < 0) k0 = 0;
+    return true;
+}
+
+// About the number of march steps of ray r: planes from the start plane to
+// the exit, plus one (a split of the march into windows only needs the
+// count roughly: the last window runs to the end).
+__host__ __device__ __forceinline__ int march_step_estimate(const TraceArgs& a, const RayConst& r)
+{
+    const double df = r.af - r.ai;
+    if (!(df != 0.0)) return 1;
+    const int ain = ((r.ai < 0 && df < 0) || (r.ai > 0 && df > 0)) ? trunc_i32(trunc_i32(r.ai) + df / fabs(df))
+                                                                  : trunc_i32(r.ai);
+    const double n = fabs(r.af - (double)ain) / a.D;
+    return n < 1.0e8 ? (int)n + 2 : 1;
+}
+
+// Channels lo..hi of view v whose rays can cross the box x_lo..x_hi,
+// y_lo..y_hi (cm): in the view-0 frame the ray of channel ch leaves the
+// source (0, sid) with tan = px0 / (sid - py0), monotone in ch, and a ray
+// crosses a convex box seen from outside only between the tangents of its
+// corners; one channel of margin on each side.  lo > hi: none.
+__host__ __device__ __forceinline__ void box_channels(const TraceArgs& a, int v, double x_lo, double x_hi,
+                                                      double y_lo, double y_hi, int& lo, int& hi)
+{
+    const double c = a.cosT[v], s = a.sinT[v];
+    double tmin = 1.0e300, tmax = -1.0e300;
+    bool ok = true;
+    const double xs[2] = {x_lo, x_hi}, ys[2] = {y_lo, y_hi};
+    for (int i = 0; i < 4; i++) {
+        const double x = xs[i & 1], y = ys[i >> 1];
+        const double x0 = c * x - s * y, y0 = s * x + c * y;  // rotated back to view 0
+        const double den = a.sid - y0;
+        if (!(den > 1.0e-6)) ok = false;
+        const double t = x0 / den;
+        tmin = fmin(tmin, t);
+        tmax = fmax(tmax, t);
+    }
+    const int W = a.W;
+    if (!ok) {
+        lo = 0;
+        hi = W - 1;
+        return;
+    }
+    auto T = [&](int ch) { return a.px0[ch] / (a.sid - a.py0[ch]); };
+    const bool dec = T(0) > T(W - 1);
+    // U increasing in ch: first channel with U >= umin, and with U > umax
+    const double umin = dec ? -tmax : tmin, umax = dec ? -tmin : tmax;
+    auto first = [&](double u, bool strict) {
+        int l = 0, h = W;  // answer in [l, h]
+        while (l < h) {
+            const int m = (l + h) >> 1;
+            const double um = dec ? -T(m) : T(m);
+            if (strict ? (um > u) : (um >= u)) h = m;
+            else l = m + 1;
+        }
+        return l;
+    };
+    lo = first(umin, false) - 1;
+    hi = first(umax, true);  // (last channel with U <= umax) + 1
+    if (lo < 0) lo = 0;
+    if (hi > W - 1) hi = W - 1;
 }
 
 // Trace ray (view, ch).  Calls sink.add(voxel, length) for every segment

@@ -373,6 +373,36 @@ struct SinkAdj {
     }
 };
 
+// Adjoint sink of one image tile: a segment whose voxel lies in the tile adds
+// w[m] * length to the tile's accumulator in LDS, [m][row][col] with a row
+// pitch of kAdjTile + 1 floats (a wave's lanes are adjacent channels of one
+// view, which meet a column of the tile at one step when x is the major axis).
+constexpr int kAdjTile = 32;
+constexpr int kAdjPitch = kAdjTile + 1;
+constexpr int kAdjPlane = kAdjTile * kAdjPitch;
+constexpr int kAdjMaxViews = 256;  // views per workgroup
+
+template <int M>
+struct SinkTile {
+    static constexpr bool kAdd2 = false;
+    static constexpr bool kAB = true;
+    float* acc;
+    int x0, y0;
+    bool xmajor;
+    float w[M];
+    __device__ __forceinline__ void add_ab(int ia, int ib, double rr)
+    {
+        const int x = xmajor ? ia : ib, y = xmajor ? ib : ia;
+        const uint32_t lx = (uint32_t)(x - x0), ly = (uint32_t)(y - y0);
+        if (lx < (uint32_t)kAdjTile && ly < (uint32_t)kAdjTile) {
+            const float r = (float)rr;
+            float* p = acc + ly * kAdjPitch + lx;
+#pragma unroll
+            for (int m = 0; m < M; m++) atomicAdd(p + m * kAdjPlane, w[m] * r);
+        }
+    }
+};
+
 // ---------------------------------------------------------------- kernels
 
 template <int M, int DT, bool FAST>
@@ -438,29 +468,52 @@ __device__ __forceinline__ void trace_one(const TraceArgs& a, int view, int ch)
     if (a.counts) a.counts[ray] = iks;
 }
 
-// Orbits of R rays: lane (view v < V/R, channel) marches its ray once and
+// Sum over the P lanes of a group (P a power of two dividing 64, groups
+// aligned): every lane of the group gets the same sum.
+template <int P>
+__device__ __forceinline__ double group_sum(double x)
+{
+#pragma unroll
+    for (int d = 1; d < P; d <<= 1) x += __shfl_xor(x, d);
+    return x;
+}
+template <int P>
+__device__ __forceinline__ int group_sum(int x)
+{
+#pragma unroll
+    for (int d = 1; d < P; d <<= 1) x += __shfl_xor(x, d);
+    return x;
+}
+
+// Orbits of R rays: the base ray (view v < V/R, channel) is marched once and
 // gives the rays of views v + kV/R their path integrals from the turned
-// voxels (SinkRot).  When the march reports a decision within the margin,
-// or the ray needs the exact march, the lane traces rays 1..R-1 (and its
-// own, unless marched) one by one.  Every ray's segments are the
-// reference's; the orbit shares the base ray's lengths, which differ from a
-// ray's own |da| * sec by rounding.
-template <int M, int DT, int R>
+// voxels (SinkRot).  The march of one orbit is split over a group of P
+// adjacent lanes: lane p runs steps [p len, (p + 1) len) of the windowed
+// march (the last lane to the end), len = ceil(steps / P), and the group adds
+// its partial sums and segment counts with cross-lane shuffles.  When a march
+// piece reports a decision within the margin, or the ray needs the exact
+// march, the group traces rays 1..R-1 (and the base ray, unless marched) one
+// by one, one ray per lane.  Every ray's segments are the reference's; the
+// orbit shares the base ray's lengths, which differ from a ray's own
+// |da| * sec by rounding.  Every branch before a shuffle depends only on the
+// orbit, so the P lanes of a group stay together through the shuffles.
+template <int M, int DT, int R, int P>
 __global__ __launch_bounds__(256) void k_trace_fwd_rot(TraceArgs a)
 {
-    const int local = blockIdx.x * blockDim.x + threadIdx.x;
+    static_assert(P >= 1 && P <= 64 && (P & (P - 1)) == 0, "P: a power of two");
+    const int gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const int orbit = gl / P, piece = gl % P;
     const int Vq = a.V / R;
-    if (local >= Vq * a.W) return;
-    const int view = local / a.W;
-    const int ch = local - view * a.W;
+    if (orbit >= Vq * a.W) return;  // whole groups
+    const int view = orbit / a.W;
+    const int ch = orbit - view * a.W;
     RayConst r;
     bool safe = false;
     const bool hit = setup_ray(a, view, ch, r, &safe);
     bool done = false;
     if (safe && !hit) {
         // the whole orbit misses the image: no segment, path integrals 0
-#pragma unroll
-        for (int k = 0; k < R; k++) {
+        for (int k = piece; k < R; k += P) {
             const int ray = (view + k * Vq) * a.W + ch;
             float4* dst = reinterpret_cast<float4*>(a.L) + (size_t)ray * 2;
             dst[0] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -472,14 +525,38 @@ __global__ __launch_bounds__(256) void k_trace_fwd_rot(TraceArgs a)
         SinkRot<M, DT, R> sink;
         sink.init(a.dens, r.xmajor, a.N);
         bool ovf = false;
-        const int iks = march_loop<true>(a, r, sink, ovf);
+        int k0 = 0, k1 = INT_MAX;
+        if (P > 1) {
+            const int len = (march_step_estimate(a, r) + P - 1) / P;
+            k0 = piece * len;
+            k1 = piece == P - 1 ? INT_MAX : k0 + len - 1;
+        }
+        int iks = march_loop<true>(a, r, sink, ovf, true, k0, k1);
         if (ovf) atomicOr(a.err, 1);
+        if (P > 1) {
+            iks = group_sum<P>(iks);
+            sink.unsafe = group_sum<P>(sink.unsafe ? 1 : 0) != 0;
+            if constexpr (DT == VCT_DENS_U16) {
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+#pragma unroll
+                    for (int w = 0; w < SinkRot<M, DT, R>::NW; w++) sink.accw[k][w] = group_sum<P>(sink.accw[k][w]);
+#pragma unroll
+                    for (int w = 0; w < M / 2; w++) sink.acch[k][w] = group_sum<P>(sink.acch[k][w]);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < R; k++)
+#pragma unroll
+                    for (int m = 0; m < M; m++) sink.acc[k][m] = group_sum<P>(sink.acc[k][m]);
+            }
+        }
         sink.finish();
         // ray 0 is right in any case (its own march); the turned rays only
         // when no decision was within the margin
 #pragma unroll
         for (int k = 0; k < R; k++) {
-            if (k > 0 && sink.unsafe) break;
+            if (k % P != piece || (k > 0 && sink.unsafe)) continue;
             const int ray = (view + k * Vq) * a.W + ch;
             float out[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -490,12 +567,12 @@ __global__ __launch_bounds__(256) void k_trace_fwd_rot(TraceArgs a)
             if (a.counts) a.counts[ray] = iks;
         }
         if (sink.unsafe) {
-            for (int k = 1; k < R; k++) trace_one<M, DT>(a, view + k * Vq, ch);
+            for (int k = 1 + piece; k < R; k += P) trace_one<M, DT>(a, view + k * Vq, ch);
         }
         done = true;
     }
     if (!done) {
-        for (int k = 0; k < R; k++) trace_one<M, DT>(a, view + k * Vq, ch);
+        for (int k = piece; k < R; k += P) trace_one<M, DT>(a, view + k * Vq, ch);
     }
 }
 
@@ -537,6 +614,103 @@ __global__ __launch_bounds__(256) void k_trace_adj(TraceArgs a)
     else if (active)
         march_ray<false>(a, view, ch, sink, ovf);
     if (ovf) atomicOr(a.err, 1);
+}
+
+// Adjoint by image tiles.  Workgroup (chunk, tile) takes the views of one
+// chunk and one kAdjTile^2 tile of the image; for each view it finds the
+// channels whose rays can cross the tile (box_channels), and each such ray
+// runs only the march steps that can land in the tile (box_steps, the
+// windowed march_loop), adding w[m] * length into the tile in LDS.  Each
+// segment of each ray lands in exactly one tile, so the tiles of one chunk
+// together add every segment once.  At the end the tile is added to grad
+// with one float atomic per voxel and material (zero where the voxel's
+// density of that material is negative, the tracer's sign test).
+// Work items are dealt to XCDs (workgroup g runs on XCD g % 8) in contiguous
+// runs, tile fastest, so the workgroups of one XCD share view chunks and
+// read their rays' weights through one L2.
+template <int M>
+__global__ __launch_bounds__(256) void k_trace_adj_tile(TraceArgs a, int ntx, int ntiles, int cv, int per_xcd)
+{
+    __shared__ float acc[M * kAdjPlane];
+    __shared__ int pref[kAdjMaxViews + 1];
+    __shared__ int clo[kAdjMaxViews];
+    const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    const int nchunk = (a.VT + cv - 1) / cv;
+    if (item >= ntiles * nchunk) return;  // whole workgroup, before any barrier
+    const int chunk = item / ntiles, tile = item - chunk * ntiles;
+    const int x0 = (tile % ntx) * kAdjTile, y0 = (tile / ntx) * kAdjTile;
+    const int v0 = chunk * cv;
+    const int nv = min(cv, a.VT - v0);
+    for (int i = threadIdx.x; i < M * kAdjPlane; i += blockDim.x) acc[i] = 0.f;
+    // the tile plus two voxels, in cm (x index i covers [(i - nxm) D, (i + 1 - nxm) D),
+    // y index j covers (-(j + 1 - nxm) D, -(j - nxm) D])
+    const double D = a.D;
+    const double x_lo = (double)(x0 - 2 - a.nxm) * D, x_hi = (double)(x0 + kAdjTile + 2 - a.nxm) * D;
+    const double y_lo = -(double)(y0 + kAdjTile + 2 - a.nxm) * D, y_hi = -(double)(y0 - 2 - a.nxm) * D;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        int lo, hi;
+        box_channels(a, v0 + i, x_lo, x_hi, y_lo, y_hi, lo, hi);
+        clo[i] = lo;
+        pref[i + 1] = hi >= lo ? hi - lo + 1 : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        pref[0] = 0;
+        for (int i = 0; i < nv; i++) pref[i + 1] += pref[i];
+    }
+    __syncthreads();
+    const int total = pref[nv];
+    bool ovf = false;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        // view slot j with pref[j] <= i < pref[j + 1]
+        int lo = 0, hi = nv - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pref[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const int view = v0 + lo;
+        const int ch = clo[lo] + (i - pref[lo]);
+        const int ray = view * a.W + ch;
+        SinkTile<M> sink;
+        const float gscale = (a.g ? a.g[ray] : 1.f) * (float)a.out_scale;
+        const float4* wv = reinterpret_cast<const float4*>(a.wts) + (size_t)ray * 2;
+        const float4 w0 = wv[0];
+        const float4 w1 = M > 4 ? wv[1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float wa[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+        bool any = false;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            sink.w[m] = wa[m] * gscale;
+            any = any || (sink.w[m] != 0.f);
+        }
+        if (!any) continue;
+        RayConst r;
+        if (!setup_ray(a, view, ch, r)) continue;
+        int k0, k1;
+        if (!box_steps(a, r, x_lo, x_hi, y_lo, y_hi, k0, k1)) continue;
+        sink.acc = acc;
+        sink.x0 = x0;
+        sink.y0 = y0;
+        sink.xmajor = r.xmajor;
+        if (r.dB != 0.0 && !a.rcheck && !a.exact) march_loop<true>(a, r, sink, ovf, true, k0, k1);
+        else march_loop<false>(a, r, sink, ovf, true, k0, k1);
+    }
+    if (ovf) atomicOr(a.err, 1);
+    __syncthreads();
+    const float* dens = reinterpret_cast<const float*>(a.dens);
+    for (int i = threadIdx.x; i < kAdjTile * kAdjTile; i += blockDim.x) {
+        const int lx = i % kAdjTile, ly = i / kAdjTile;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x >= a.N || y >= a.N) continue;
+        const size_t v = (size_t)y * a.N + x;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            float val = acc[m * kAdjPlane + ly * kAdjPitch + lx];
+            if (dens && !(dens[v * 8 + m] >= 0.f)) val = 0.f;
+            if (val != 0.f) unsafeAtomicAdd(a.grad + v * 8 + m, val);
+        }
+    }
 }
 
 // Polyenergetic sum: one spectrum per blockIdx.y; tables in LDS as
@@ -742,11 +916,16 @@ __global__ __launch_bounds__(256) void k_pack_f32(const float* __restrict__ src,
 
 // --------------------------------------------------------------- launchers
 
+// Lanes per orbit: four, so that the launch holds about four times as many
+// waves as orbits/64 (121,800 orbits at 800 x 609 give ~7,600 waves).
+constexpr int kOrbitLanes = 4;
+
 template <int M, int DT, int R>
 static void launch_trace_rot(const TraceArgs& a, hipStream_t s)
 {
-    const long long lanes = (long long)(a.V / R) * a.W;
-    hipLaunchKernelGGL((k_trace_fwd_rot<M, DT, R>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, a);
+    const long long lanes = (long long)(a.V / R) * a.W * kOrbitLanes;
+    hipLaunchKernelGGL((k_trace_fwd_rot<M, DT, R, kOrbitLanes>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
+                       s, a);
 }
 
 template <int M>
@@ -766,7 +945,17 @@ static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint
             else launch_trace_rot<M, VCT_DENS_F32, 2>(a, s);
         }
     } else if (adjoint) {
-        hipLaunchKernelGGL((k_trace_adj<M, true>), grid, block, 0, s, a);
+        if (a.VT > 0) {
+            // about 2048 workgroups: enough view chunks to fill the chip twice over
+            const int ntx = (a.N + kAdjTile - 1) / kAdjTile;
+            const int ntiles = ntx * ntx;
+            int nchunk = std::max(1, std::min(a.VT, (2048 + ntiles - 1) / ntiles));
+            int cv = std::min((a.VT + nchunk - 1) / nchunk, kAdjMaxViews);
+            nchunk = (a.VT + cv - 1) / cv;
+            const int per_xcd = (ntiles * nchunk + 7) / 8;
+            hipLaunchKernelGGL((k_trace_adj_tile<M>), dim3(per_xcd * 8), dim3(256), 0, s, a, ntx, ntiles, cv,
+                               per_xcd);
+        }
     } else if (dens_type == VCT_DENS_U16) {
         hipLaunchKernelGGL((k_trace_fwd<M, VCT_DENS_U16, true>), grid, block, 0, s, a);
     } else {
