@@ -289,6 +289,19 @@ def bench_batch_fbp(per_size=16):
             "batched_ms": batched * 1e3, "plain_loop_ms": plain * 1e3, "batched_vs_plain": plain / batched}
 
 
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB (0: none), whose FETCH_SIZE and
+    WRITE_SIZE calibrate the byte counters of a profiler run."""
+    if mib <= 0:
+        return
+    import torch
+    src = torch.ones(mib * (1 << 20) // 4, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +328,7 @@ def main():
         run_gathers()
         return
     if args.adjoint_only:
+        copy_probe(args.copy_probe)
         print(json.dumps(bench_adjoint(announce=True)), flush=True)
         return
 
@@ -336,13 +350,7 @@ def main():
     d_pdf = torch.from_numpy(pdf).to(dev)
     sigma = 5.37e-5
     pk, dt = ops.pack_densities(plan, d_ph)
-    if args.copy_probe > 0:
-        src = torch.ones(args.copy_probe * (1 << 20) // 4, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    copy_probe(args.copy_probe)
     torch.cuda.synchronize()
 
     stages = ["trace", "spectral", "detector", "filter", "backproject"]

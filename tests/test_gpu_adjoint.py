@@ -258,3 +258,29 @@ def test_tile_adjoint_equals_sysmat_gather_at_512(gpu):
     assert np.abs(a - b).max() < 1e-5 * np.abs(b).max()
     assert np.all(a[100:140, 200:260, 3] == 0)
     assert np.all(tile[..., M:].cpu().numpy() == 0)
+
+
+def test_tile_adjoint_nonfinite_weight_reaches_its_voxels(gpu):
+    """A NaN weight makes the gradient NaN exactly on the voxels its ray
+    crosses, as float sums do, and leaves every other voxel as it was (the
+    workgroups that see it sum in float, the others in fixed point)."""
+    import torch
+    ops = _ops()
+    plan = ops.plan_for(_geom(128, 0.35, 160))
+    M = 3
+    g = torch.Generator(device=gpu).manual_seed(8)
+    w = torch.zeros((160, 609, 8), device=gpu)
+    w[..., :M] = torch.rand((160, 609, M), generator=g, device=gpu)
+    view, ch = 37, 300
+    one = torch.zeros_like(w)
+    one[view, ch, 0] = 1.0
+    crossed = ops.trace_adjoint(plan, one, M)[..., 0].cpu().numpy() > 0
+    assert crossed.sum() > 100
+    clean = ops.trace_adjoint(plan, w, M).cpu().numpy()
+    w[view, ch, 0] = float("nan")
+    dirty = ops.trace_adjoint(plan, w, M).cpu().numpy()
+    assert np.array_equal(np.isnan(dirty[..., 0]), crossed)
+    assert not np.isnan(dirty[..., 1:]).any()
+    keep = ~crossed
+    assert np.abs(dirty[..., 0][keep] - clean[..., 0][keep]).max() <= 1e-5 * np.abs(clean[..., 0]).max()
+    assert np.abs(dirty[..., 1:] - clean[..., 1:]).max() <= 1e-5 * np.abs(clean[..., 1:]).max()

@@ -156,9 +156,6 @@ struct RayConst {
 // The march from the entry point (M/reprojection_TOMO.cpp:113-283, either
 // branch) for one ray.  FAST selects reciprocal products with guarded
 // decisions (see the file comment) over exact divisions.
-// SYNC: the loop is wave-uniform and the segments of a committed fast step
-// go to Sink::add2_sync, which every lane calls once per step (see
-// march_ray_sync); alive_in = false makes a lane march nothing.
 // Window: only steps k_begin..k_end (the loop counter ik of :115) run.  Step
 // k's segments lie between major planes k - 1 and k (step 0: from the entry
 // to plane 0), and the state the march carries into step k is plane point
@@ -167,9 +164,9 @@ struct RayConst {
 // whole march (lengths equal, or within an ulp where a plane point was made
 // exact for a decision of the step before), and the returned counter is
 // meaningful for the whole march only.
-template <bool FAST, bool SYNC = false, class Sink>
+template <bool FAST, class Sink>
 __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const RayConst& r, Sink& sink, bool& ovf,
-                                          bool alive_in = true, int k_begin = 0, int k_end = INT_MAX)
+                                          int k_begin = 0, int k_end = INT_MAX)
 {
     constexpr bool kRot = sink_rot<Sink>::value;
     constexpr bool kTakesAB = sink_ab<Sink>::value;
@@ -332,25 +329,12 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
         ik_first = k_begin;
     }
 
-    bool alive = alive_in;
     double pik = pha * (double)ik_first;  // pha * ik, exact
     for (int ik = ik_first;; ik++, pik += pha) {
-        if constexpr (SYNC) {
-            // wave-uniform loop: finished lanes idle until every lane is done
-            if (alive && !(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) alive = false;
-            if (alive && ik >= a.max_steps) {
-                ovf = true;
-                alive = false;
-            }
-            if (!__any(alive)) break;
-        } else {
-            if (!(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) break;
-            if (ik >= a.max_steps) { ovf = true; break; }
-            if (ik > k_end) break;
-        }
-        int ev1 = -1, ev2 = -1;  // SYNC: this step's committed segments (-1: none)
-        double er1 = 0.0, er2 = 0.0;
-        if (alive) {
+        if (!(pc.a <= Lb && pc.a >= -Lb && pc.b <= Lb && pc.b >= -Lb)) break;
+        if (ik >= a.max_steps) { ovf = true; break; }
+        if (ik > k_end) break;
+        {
             const int bmb = bma;
             const double bj = pc.b;
             bool committed = false;
@@ -386,12 +370,7 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
                     const bool v1 = cross && in_image(ia1, ib1);
                     const bool v2 = in_image(ia2, ib2);
                     iks += cross ? 2 : 1;
-                    if constexpr (SYNC) {
-                        ev1 = v1 ? ia1 * stA + ib1 * stB : -1;
-                        er1 = fabs(xa - p0.a) * sec;
-                        ev2 = v2 ? ia2 * stA + ib2 * stB : -1;
-                        er2 = fabs(na - s_a) * sec;
-                    } else if constexpr (kTakesAB) {
+                    if constexpr (kTakesAB) {
                         if (v1) sink.add_ab(ia1, ib1, fabs(xa - p0.a) * sec);
                         if (v2) sink.add_ab(ia2, ib2, fabs(na - s_a) * sec);
                     } else if constexpr (kRot) {
@@ -455,7 +434,6 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
                 p0 = pc;
             }
         }
-        if constexpr (SYNC) sink.add2_sync(ev1, er1, ev2, er2);  // every lane of the wave
     }
     return iks;
 }
@@ -661,23 +639,6 @@ __host__ __device__ __forceinline__ int march_ray(const TraceArgs& a, int view, 
     // the fast path needs finite reciprocals and no r >= rmax test
     if (FAST_IN && r.dB != 0.0 && !a.rcheck && !a.exact) return march_loop<true>(a, r, sink, ovf);
     return march_loop<false>(a, r, sink, ovf);
-}
-
-// The same march for a sink that combines the segments of the lanes of a
-// wave before writing (Sink::add2_sync uses cross-lane shuffles): every lane
-// of the wave must call it.  Lanes with active == false, rays that miss the
-// image and rays that finish early stay in the wave-uniform fast loop as
-// lanes without segments; rays that need the exact march run it first,
-// adding through Sink::add, and then idle in the same way.
-template <class Sink>
-__device__ __forceinline__ void march_ray_sync(const TraceArgs& a, bool active, int view, int ch, Sink& sink,
-                                               bool& ovf)
-{
-    RayConst r{};
-    const bool hit = active && setup_ray(a, view, ch, r);
-    const bool fast = hit && r.dB != 0.0 && !a.rcheck && !a.exact;
-    if (hit && !fast) march_loop<false>(a, r, sink, ovf);
-    march_loop<true, true>(a, r, sink, ovf, fast);
 }
 
 }  // namespace vct
