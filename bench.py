@@ -350,6 +350,7 @@ def main():
     d_pdf = torch.from_numpy(pdf).to(dev)
     sigma = 5.37e-5
     pk, dt = ops.pack_densities(plan, d_ph)
+    tuned = plan.tune(M)  # launch shapes for this geometry (untimed; cached per shape)
     copy_probe(args.copy_probe)
     torch.cuda.synchronize()
 
@@ -425,6 +426,7 @@ def main():
             "parallelism": "single GPU",
         },
         "stage_ms": st_ms,
+        "tuned": tuned,
         "mbfpa_ray_energy_bins_per_s": rays_bins / (mbfpa_ms * 1e-3),
         "fbp_voxel_views_per_s": p_mask * V / (fbp_ms * 1e-3),
         "roofline": {

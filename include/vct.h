@@ -137,12 +137,24 @@ vct_status vct_plan_destroy(vct_plan* plan);
  * march must clear the guarded march's margin for the turned rays too, or
  * those rays are marched by themselves; lengths may differ in the last
  * bits.  The exact option and geometries where the reference's r >= rmax
- * test can fire use one ray per march. */
+ * test can fire use one ray per march.
+ * Launch shapes (results do not depend on them, beyond the order of float
+ * adds where noted; an autotuner times them per shape):
+ * VCT_OPT_TRACE_BLOCK = 6: lanes per workgroup of the tracer, 64, 128 or 256
+ * (0: 256).
+ * VCT_OPT_BP_SINOS_PER_LANE = 7: most sinograms one lane backprojects in a
+ * multi-sinogram call, 2..16 (0: 16); images are bit-identical for every value.
+ * VCT_OPT_ADJOINT_VIEWS = 8: views per workgroup of the adjoint, 1..256 (0: a
+ * chunk that gives about 2048 workgroups); workgroups of other chunks add
+ * their tiles in another order, which changes the float rounding of grad. */
 enum {
     VCT_OPT_EXACT_TRACE = 1,
     VCT_OPT_MAX_STEPS = 2,
     VCT_OPT_EXACT_BACKPROJECT = 4,
-    VCT_OPT_TRACE_ORBIT = 5
+    VCT_OPT_TRACE_ORBIT = 5,
+    VCT_OPT_TRACE_BLOCK = 6,
+    VCT_OPT_BP_SINOS_PER_LANE = 7,
+    VCT_OPT_ADJOINT_VIEWS = 8
 };
 vct_status vct_plan_set_option(vct_plan* plan, int option, int value);
 

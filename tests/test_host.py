@@ -160,3 +160,21 @@ def test_batched_row_layout():
     rows = batch.interleave(dens)
     assert torch.all(batch.from_planes(rows)[..., 6:] == 0)
     assert torch.equal(batch.deinterleave(rows, 6, 4), dens)
+
+
+def test_tuner_keeps_the_fastest_and_caches_it(tmp_path):
+    """The tuner times every candidate once, keeps the fastest, writes it to
+    its cache file and answers later asks (and new processes) from there."""
+    from virtualct_amd.autotune import Tuner
+    path = str(tmp_path / "tune.json")
+    t = Tuner(path)
+    calls = []
+
+    def time_fn(cfg):
+        calls.append(cfg["x"])
+        return {1: 3.0, 2: 1.0, 3: 2.0}[cfg["x"]]
+    best = t.best(("k", 1), lambda: [{"x": 1}, {"x": 2}, {"x": 3}], time_fn)
+    assert best == {"x": 2} and calls == [1, 2, 3]
+    assert t.best(("k", 1), lambda: [{"x": 1}], time_fn) == {"x": 2} and calls == [1, 2, 3]
+    assert Tuner(path).best(("k", 1), lambda: [{"x": 1}], time_fn) == {"x": 2}
+    assert t.best(("k", 2), lambda: [{"x": 3}, {"x": 1}], time_fn) == {"x": 3}

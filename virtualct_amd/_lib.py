@@ -22,6 +22,10 @@ OPT_EXACT_TRACE = 1
 OPT_MAX_STEPS = 2
 OPT_EXACT_BACKPROJECT = 4
 OPT_TRACE_ORBIT = 5
+OPT_TRACE_BLOCK = 6
+OPT_BP_SINOS_PER_LANE = 7
+OPT_ADJOINT_VIEWS = 8
+ERR_ARG = 1
 ERR_INTERNAL = 6
 SYSMAT_TRANSPOSE = 1
 SYSMAT_BLOCKED = 2

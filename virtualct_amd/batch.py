@@ -361,8 +361,17 @@ class BatchForwardFBP:
         W = {g.plan.W for g in self.groups}
         if len(V) != 1 or len(W) != 1:
             raise ValueError("all groups must share the detector geometry (views x channels)")
-        self.lanes = lanes or lanes_for(max(g.B for g in self.groups))
-        self.blocks = blocks
+        # the gather's launch shape and each group's backprojection split come
+        # from the autotuner unless given (tuned here, never under capture)
+        if lanes is None:
+            key = ("forward gather", tuple((g.plan.N, g.B) for g in self.groups), self.M,
+                   "blocked" if blocked else "direct")
+            cfg = tuner().best(key, self._candidates, self._time_config)
+            self.lanes, self.blocks = cfg["lanes"], cfg["blocks"]
+        else:
+            self.lanes, self.blocks = lanes, blocks
+        for g in self.groups:
+            g.plan.tune(self.M, g.B * self.S, what=("backproject",))
         # the filter's packed operand lives in a buffer of this run, not in the
         # context scratch, so a replay of the captured graph shares nothing
         # with later eager calls
@@ -370,6 +379,20 @@ class BatchForwardFBP:
         self.filter_work = ops.filter_workspace(self.groups[0].plan, self.n_sino)
         self.graph = None
         self.out = None
+
+    def _candidates(self):
+        base = lanes_for(max(g.B for g in self.groups))
+        cus = torch.cuda.get_device_properties(self.groups[0].dens.device).multi_processor_count
+        out = [{"lanes": base, "blocks": 0}]
+        for lanes in sorted({base, 8 if base == 1 else min(64, base * 2)}):
+            for per_cu in (2, 4, 8):
+                out.append({"lanes": lanes, "blocks": cus * per_cu})
+        return out
+
+    def _time_config(self, cfg):
+        from .autotune import events_ms
+        items = [(g.sysmat, g.dens, g.L, g.B) for g in self.groups]
+        return events_ms(lambda: batch_paths(items, self.M, cfg["lanes"], cfg["blocks"]))
 
     def run(self):
         gs = self.groups

@@ -104,6 +104,8 @@ TraceArgs make_trace_args(const vct_plan* p)
     a.ray_offset = 0;
     a.n_rays = g.n_views * g.n_channels;
     a.exact = p->exact_trace;
+    a.block = p->trace_block > 0 ? p->trace_block : 256;
+    a.adj_views = p->adj_views;
     // orbits of the half or quarter turn: every view traced (V even), a grid
     // centred on a voxel corner (N even), the guarded march (no exact option,
     // no r >= rmax test)
@@ -403,6 +405,20 @@ vct_status vct_plan_set_option(vct_plan* p, int option, int value)
     case VCT_OPT_MAX_STEPS:
         if (value < 0) return set_error(VCT_ERR_ARG, "step bound %d must be >= 0", value);
         p->max_steps = value;
+        return VCT_OK;
+    case VCT_OPT_TRACE_BLOCK:
+        if (value != 0 && value != 64 && value != 128 && value != 256)
+            return set_error(VCT_ERR_ARG, "tracer workgroup %d is not 64, 128 or 256", value);
+        p->trace_block = value;
+        return VCT_OK;
+    case VCT_OPT_BP_SINOS_PER_LANE:
+        if (value != 0 && (value < 2 || value > 16))
+            return set_error(VCT_ERR_ARG, "sinograms per lane %d outside 2..16", value);
+        p->bp_max_k = value;
+        return VCT_OK;
+    case VCT_OPT_ADJOINT_VIEWS:
+        if (value < 0 || value > 256) return set_error(VCT_ERR_ARG, "adjoint views per workgroup %d outside 0..256", value);
+        p->adj_views = value;
         return VCT_OK;
     default: return set_error(VCT_ERR_ARG, "unknown option %d", option);
     }
@@ -752,6 +768,7 @@ vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* 
     a.mask_r = (double)(g.recon_size - 1) / 2.0;
     a.mask_r2 = a.mask_r * a.mask_r;
     a.exact = p->exact_bp;
+    a.max_k = p->bp_max_k;
     return launch_backproject(a, n_sino, pick_stream(p, stream));
 }
 

@@ -516,7 +516,8 @@ static void launch_bp_w(const BackprojArgs& a, int S, hipStream_t s)
     // leave one wave per SIMD, and 3 slices of one sinogram run 2-3x faster
     const long long px = (long long)a.N * a.N;
     const long long floor_slices = (kBpMinLanes + px - 1) / px;
-    int slices = (S + kBpSinos - 1) / kBpSinos;
+    const int cap = a.max_k >= 2 && a.max_k < kBpSinos ? a.max_k : kBpSinos;
+    int slices = (S + cap - 1) / cap;
     if (slices < floor_slices) slices = (int)std::min<long long>(S, floor_slices);
     const int K = (S + slices - 1) / slices;
     if (S == 1 && !a.exact && a.V % 4 == 0 && a.N % 2 == 0 && a.use_views == a.V && a.N >= 2) {

@@ -54,6 +54,8 @@ struct TraceArgs {
     int rcheck;           // 1 when the reference's r >= rmax test can fire
     int exact;            // 1 forces the exact-division march
     int rot;              // forward: rays per orbit of k_trace_fwd_rot (1: one ray per lane)
+    int block;            // lanes per workgroup of the tracer (64, 128 or 256)
+    int adj_views;        // views per workgroup of the tile adjoint (0: automatic)
     int ray_offset;       // first ray handled by this launch
     int n_rays;           // rays handled by this launch
     double D;             // pixel pitch
@@ -109,6 +111,7 @@ struct BackprojArgs {
     float* img;           // [S][N][N]
     int N, W, V, use_views, cent, half;
     int exact;            // 1: views in the reference's order per pixel, no 4-fold sharing
+    int max_k;            // most sinograms per lane of the multi-sinogram kernel (0: kBpSinos)
     double pix, sid, dbeta_ch, inv_dbeta_ch, dbeta_view, xfactor, center, mask_r2, mask_r;
 };
 
@@ -162,6 +165,9 @@ struct vct_plan {
     int max_steps = 0;    // VCT_OPT_MAX_STEPS (0: the automatic bound)
     int exact_bp = 0;     // VCT_OPT_EXACT_BACKPROJECT
     int trace_orbit = 4;  // VCT_OPT_TRACE_ORBIT
+    int trace_block = 0;  // VCT_OPT_TRACE_BLOCK (0: 256)
+    int bp_max_k = 0;     // VCT_OPT_BP_SINOS_PER_LANE (0: the kernel's limit)
+    int adj_views = 0;    // VCT_OPT_ADJOINT_VIEWS (0: automatic)
 };
 
 namespace vct {

@@ -891,15 +891,15 @@ template <int M, int DT, int R>
 static void launch_trace_rot(const TraceArgs& a, hipStream_t s)
 {
     const long long lanes = (long long)(a.V / R) * a.W * kOrbitLanes;
-    hipLaunchKernelGGL((k_trace_fwd_rot<M, DT, R, kOrbitLanes>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
-                       s, a);
+    hipLaunchKernelGGL((k_trace_fwd_rot<M, DT, R, kOrbitLanes>), dim3((unsigned)((lanes + a.block - 1) / a.block)),
+                       dim3(a.block), 0, s, a);
 }
 
 template <int M>
 static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint, hipStream_t s)
 {
-    const dim3 block(256);
-    const dim3 grid((a.n_rays + 255) / 256);
+    const dim3 block(a.block);
+    const dim3 grid((a.n_rays + a.block - 1) / a.block);
     const bool whole = a.ray_offset == 0 && a.n_rays == a.V * a.W;
     if (!adjoint && whole && (a.rot == 4 || a.rot == 2)) {
         if (a.V % a.rot != 0 || a.N % 2 != 0 || a.VT != a.V || a.exact || a.rcheck)
@@ -918,6 +918,7 @@ static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint
             const int ntiles = ntx * ntx;
             int nchunk = std::max(1, std::min(a.VT, (2048 + ntiles - 1) / ntiles));
             int cv = std::min((a.VT + nchunk - 1) / nchunk, kAdjMaxViews);
+            if (a.adj_views > 0) cv = std::min(a.adj_views, kAdjMaxViews);
             nchunk = (a.VT + cv - 1) / cv;
             const int per_xcd = (ntiles * nchunk + 7) / 8;
             hipLaunchKernelGGL((k_trace_adj_tile<M>), dim3(per_xcd * 8), dim3(256), 0, s, a, ntx, ntiles, cv,

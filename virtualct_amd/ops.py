@@ -56,14 +56,20 @@ class Plan:
         self.N, self.W, self.V = g.recon_size, g.n_channels, g.n_views
         # "sysmat" (default): the autograd adjoint gathers over a cached
         # voxel-major system matrix, built on first use (8 bytes per segment
-        # in each direction, 4.5 GB at 512^2 / 800 x 609; ~40 ms), atomic-free
-        # and deterministic, ~50x faster per call than "atomic", which
-        # re-traces every ray and scatters with float atomics (no memory).
+        # in each direction, 4.5 GB at 512^2 / 800 x 609; ~70 ms), atomic-free
+        # and deterministic, 1.5 ms per call at 512^2; "atomic" re-traces the
+        # rays tile by tile (2.7 ms per call, no memory beyond the gradient).
         self.adjoint_mode = "sysmat"
         self._sysmat = None
 
     def set_option(self, option: int, value: int):
         call("vct_plan_set_option", self.handle, option, value)
+
+    def tune(self, M: int = 6, n_sino: int = 1, what=("trace", "adjoint", "backproject")) -> dict:
+        """Time this geometry's launch shapes and keep the fastest
+        (``autotune.tune_plan``); returns the chosen plan options."""
+        from .autotune import tune_plan
+        return tune_plan(self, M, n_sino, what)
 
     def check_errors(self):
         """``check_errors`` for this plan's device."""
