@@ -51,6 +51,17 @@ namespace vct {
 
 constexpr double kMarchMargin = 1e-9;
 
+// a * b for 0 <= a, b < 2^24 (the full-rate 24-bit multiply on the device;
+// a 32-bit multiply is a quarter-rate instruction)
+__host__ __device__ __forceinline__ int mul24(int a, int b)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return (int)__umul24((uint32_t)a, (uint32_t)b);
+#else
+    return a * b;
+#endif
+}
+
 // Sink::kRot, false when the sink does not declare it.
 template <class S, class = void>
 struct sink_rot : std::false_type {};
@@ -274,7 +285,7 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
                 const double rr = FAST ? fabs(p1.a - p0.a) * sec
                                        : sqrt((p0.a - p1.a) * (p0.a - p1.a) + (p0.b - p1.b) * (p0.b - p1.b));
                 if constexpr (kTakesAB) sink.add_ab(ia, ib, rr);
-                else sink.add(ia * stA + ib * stB, rr);
+                else sink.add(mul24(ia, stA) + mul24(ib, stB), rr);
             }
         }
     };
@@ -379,11 +390,12 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
                         sink.add2_fl(fa1, fb1, cross ? fabs(xa - p0.a) * sec : 0.0, fa2, fb2, fabs(na - s_a) * sec);
                     } else if constexpr (Sink::kAdd2) {
                         // empty segments: length 0 at voxel 0, which adds nothing
-                        sink.add2(v1 ? ia1 * stA + ib1 * stB : 0, v1 ? fabs(xa - p0.a) * sec : 0.0,
-                                  v2 ? ia2 * stA + ib2 * stB : 0, v2 ? fabs(na - s_a) * sec : 0.0);
+                        // (a voxel index is only used when its segment is in the image)
+                        sink.add2(v1 ? mul24(ia1, stA) + mul24(ib1, stB) : 0, v1 ? fabs(xa - p0.a) * sec : 0.0,
+                                  v2 ? mul24(ia2, stA) + mul24(ib2, stB) : 0, v2 ? fabs(na - s_a) * sec : 0.0);
                     } else {
-                        if (v1) sink.add(ia1 * stA + ib1 * stB, fabs(xa - p0.a) * sec);
-                        if (v2) sink.add(ia2 * stA + ib2 * stB, fabs(na - s_a) * sec);
+                        if (v1) sink.add(mul24(ia1, stA) + mul24(ib1, stB), fabs(xa - p0.a) * sec);
+                        if (v2) sink.add(mul24(ia2, stA) + mul24(ib2, stB), fabs(na - s_a) * sec);
                     }
                     bma = nbma;
                     pc = MarchPoint{na, nb, 1, false};
