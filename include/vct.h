@@ -299,6 +299,15 @@ vct_status vct_lengthproj_window(vct_plan* plan, const float* d_proj, int proj_w
  * n_sino = S is faster than S calls; the images are bit-identical either way. */
 vct_status vct_backproject(vct_plan* plan, const float* d_fltd, int n_sino, float* d_img,
                            void* stream);
+/* The backprojections of n_groups (<= 16) geometries in one launch, as one
+ * vct_backproject call per group (images bit-identical): group k takes
+ * n_sino[k] filtered sinograms at d_fltd[k] and writes d_img[k], with its
+ * plan's geometry; sinos_per_lane (2..16, 0: 16) is the number of sinograms
+ * a lane sums at once, shared by all groups.  Serves batches of phantoms of
+ * uneven sizes (the reference reconstructs one image per run,
+ * F/reconstructImageTOMO.cpp:11-105). */
+vct_status vct_backproject_groups(vct_plan* const* plans, const float* const* d_fltd, const int* n_sino,
+                                  float* const* d_img, int n_groups, int sinos_per_lane, void* stream);
 
 /* Adjoint of vct_trace_paths weighted per ray:
  *   grad[v][m] += sum_rays w[ray][m] * sum_{segments of ray in voxel v} length
@@ -386,6 +395,14 @@ vct_status vct_batch_adjoint(const vct_batch_group* groups, int n_groups, int M,
 vct_status vct_batch_spectral(vct_plan* plan, const float* d_L, int B, int M, const float* d_att,
                               const float* d_pdf, int NE, int S, float* d_raw, uint32_t* d_stats,
                               void* stream);
+/* The same for n_groups (<= 16) path-integral arrays of one detector
+ * geometry (the plan's views and channels): group g has n_phantoms[g]
+ * phantoms in rows of 8 * n_phantoms[g] floats at d_L[g]; raw and stats hold
+ * the phantoms of all groups in group order, [sum B][S][rays] and
+ * [sum B][S][2], so one detector call then covers them all. */
+vct_status vct_batch_spectral_groups(vct_plan* plan, const float* const* d_L, const int* n_phantoms, int n_groups,
+                                     int M, const float* d_att, const float* d_pdf, int NE, int S, float* d_raw,
+                                     uint32_t* d_stats, void* stream);
 /* Dual-energy decomposition loss (no counterpart in the reference): for
  * targets t [B][S][rays] and normalisation I_max,s = sum_E pdf_s,
  *   loss[b] = sum_s sum_rays (ln(I/I_max) - ln t)^2,

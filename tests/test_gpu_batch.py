@@ -507,3 +507,31 @@ def test_sysmat_rows_equal_reference_segment_lists(gpu, golden):
         total += gv.size
     assert total > 10000
     sm.close()
+
+
+def test_grouped_spectral_and_backprojection_equal_per_group_calls(gpu):
+    """One launch over several size groups (spectral sums of all groups, then
+    the backprojections of all groups) gives each group exactly what its own
+    calls give, bit for bit, for several sinograms per lane."""
+    import torch
+    ops, batch, _ = _mods()
+    from virtualct_amd import default_geometry
+    gen = torch.Generator(device=gpu).manual_seed(12)
+    sizes, Bs = (64, 96, 130), (2, 3, 1)
+    plans = [ops.plan_for(default_geometry(recon_size=N, pixel_cm=50.0 / N, n_views=160)) for N in sizes]
+    S, M, NE = 2, 6, 7
+    att = torch.rand((S, M, NE), generator=gen, device=gpu) * 0.3
+    pdf = torch.rand((S, NE), generator=gen, device=gpu)
+    Ls = [torch.rand((p.V * p.W, B, 8), generator=gen, device=gpu) * 3 for p, B in zip(plans, Bs)]
+    raw, stats = batch.batch_spectral_groups(plans[0], Ls, Bs, M, att, pdf)
+    off = 0
+    for p, L, B in zip(plans, Ls, Bs):
+        r1, s1 = batch.batch_spectral(p, L, M, att, pdf)
+        assert torch.equal(raw[off:off + B], r1) and torch.equal(stats[off:off + B], s1)
+        off += B
+    fls = [torch.randn((B * S, p.V, p.W), generator=gen, device=gpu) * 100 for p, B in zip(plans, Bs)]
+    for k in (2, 5, 16):
+        imgs = [torch.full((f.shape[0], p.N, p.N), float("nan"), device=gpu) for p, f in zip(plans, fls)]
+        batch.backproject_groups(list(zip(plans, fls, imgs)), k)
+        for p, f, im in zip(plans, fls, imgs):
+            assert torch.equal(im, ops.backproject(p, f)), (p.N, k)

@@ -90,6 +90,7 @@ _SIGS = {
     "vct_filter_workspace_bytes": (C.c_size_t, [_P, C.c_int]),
     "vct_filter_sinogram_ws": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
     "vct_backproject": (C.c_int, [_P, _P, C.c_int, _P, _P]),
+    "vct_backproject_groups": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P]),
     "vct_lengthproj": (C.c_int, [_P, _P, C.c_double, C.c_double, C.c_double, _P, _P]),
     "vct_lengthproj_window": (C.c_int, [_P, _P, C.c_int, C.c_double, C.c_double, C.c_double, _P, _P]),
     "vct_beam_hardening": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, _P, _P]),
@@ -141,6 +142,8 @@ _SIGS.update({
     "vct_batch_paths": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "vct_batch_adjoint": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "vct_batch_spectral": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P]),
+    "vct_batch_spectral_groups": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, _P, _P,
+                                            _P]),
     "vct_batch_loss": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
 })
 _OPTIONAL = {}

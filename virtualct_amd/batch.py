@@ -184,6 +184,50 @@ def batch_spectral(plan: ops.Plan, L: torch.Tensor, M: int, att: torch.Tensor, p
     return raw, stats
 
 
+def _ptr_array(values, ctype=C.c_void_p):
+    arr = (ctype * len(values))(*values)
+    return arr, C.cast(arr, C.c_void_p)
+
+
+def batch_spectral_groups(plan: ops.Plan, Ls, Bs, M: int, att: torch.Tensor, pdf: torch.Tensor):
+    """Several groups' L (rays rows, batched layout, B_g phantoms each) of the
+    plan's detector geometry in one launch -> raw [sum B, S, V, W] and stats
+    [sum B, S, 2], the phantoms in group order."""
+    if not 1 <= len(Ls) <= 16 or len(Ls) != len(Bs):
+        raise ValueError("1..16 groups, one phantom count each")
+    for L, B in zip(Ls, Bs):
+        if L.dtype != torch.float32 or L.shape != (plan.V * plan.W, B, 8) or not L.is_contiguous():
+            raise ValueError(f"path integrals must be contiguous float32 [{plan.V * plan.W},{B},8]")
+    S, _, NE = att.shape
+    nb = sum(Bs)
+    dev = Ls[0].device
+    raw = torch.empty((nb, S, plan.V, plan.W), dtype=torch.float32, device=dev)
+    stats = torch.empty((nb, S, 2), dtype=torch.int32, device=dev)
+    keep_l, ls = _ptr_array([L.data_ptr() for L in Ls])
+    keep_b, bs = _ptr_array(list(Bs), C.c_int)
+    call("vct_batch_spectral_groups", plan.handle, ls, bs, len(Ls), M, _p(att), _p(pdf), NE, S, _p(raw), _p(stats),
+         _stream(dev))
+    return raw, stats
+
+
+def backproject_groups(items, sinos_per_lane: int = 0):
+    """items: [(plan, filtered sinograms [S,V,W] float32, images [S,N,N] float32)]
+    backprojected in one launch (each image as ``ops.backproject`` gives it)."""
+    if not 1 <= len(items) <= 16:
+        raise ValueError("1..16 groups")
+    for plan, fl, img in items:
+        S = fl.shape[0]
+        if fl.dtype != torch.float32 or fl.shape != (S, plan.V, plan.W) or not fl.is_contiguous():
+            raise ValueError(f"filtered sinograms must be contiguous float32 [S,{plan.V},{plan.W}]")
+        if img.dtype != torch.float32 or img.shape != (S, plan.N, plan.N) or not img.is_contiguous():
+            raise ValueError(f"images must be contiguous float32 [{S},{plan.N},{plan.N}]")
+    keep_p, ps = _ptr_array([it[0].handle.value for it in items])
+    keep_f, fs = _ptr_array([it[1].data_ptr() for it in items])
+    keep_n, ns = _ptr_array([it[1].shape[0] for it in items], C.c_int)
+    keep_i, imgs = _ptr_array([it[2].data_ptr() for it in items])
+    call("vct_backproject_groups", ps, fs, ns, imgs, len(items), int(sinos_per_lane), _stream(items[0][1].device))
+
+
 def batch_loss(plan: ops.Plan, L, M: int, att, pdf, target, w=None, loss=None):
     """-> w (rays rows, batched layout: d loss / d L), loss [B]."""
     B = L.shape[1]
@@ -370,8 +414,12 @@ class BatchForwardFBP:
             self.lanes, self.blocks = cfg["lanes"], cfg["blocks"]
         else:
             self.lanes, self.blocks = lanes, blocks
-        for g in self.groups:
-            g.plan.tune(self.M, g.B * self.S, what=("backproject",))
+        # the grouped backprojection's sinograms per lane, timed on this run's
+        # shapes (random filtered sinograms)
+        key = ("grouped backprojection", tuple((g.plan.N, g.B * self.S) for g in self.groups),
+               self.groups[0].plan.V, self.groups[0].plan.W)
+        self.bp_k = tuner().best(key, lambda: [{"sinos_per_lane": k} for k in (4, 8, 12, 16)],
+                                 self._time_bp)["sinos_per_lane"]
         # the filter's packed operand lives in a buffer of this run, not in the
         # context scratch, so a replay of the captured graph shares nothing
         # with later eager calls
@@ -394,26 +442,38 @@ class BatchForwardFBP:
         items = [(g.sysmat, g.dens, g.L, g.B) for g in self.groups]
         return events_ms(lambda: batch_paths(items, self.M, cfg["lanes"], cfg["blocks"]))
 
+    def _time_bp(self, cfg):
+        from .autotune import events_ms
+        gen = torch.Generator(device=self.groups[0].dens.device).manual_seed(2)
+        items = []
+        for g in self.groups:
+            n = g.B * self.S
+            fl = torch.randn((n, g.plan.V, g.plan.W), generator=gen, device=g.dens.device)
+            items.append((g.plan, fl, torch.empty((n, g.plan.N, g.plan.N), device=g.dens.device)))
+        return events_ms(lambda: backproject_groups(items, cfg["sinos_per_lane"]))
+
     def run(self):
+        """One pass: the forward gather of every group (one launch), the
+        spectral sums of every group (one launch), the detector of all
+        sinograms (one launch), one stacked filter GEMM and the
+        backprojections of every group (one launch)."""
         gs = self.groups
         batch_paths([(g.sysmat, g.dens, g.L, g.B) for g in gs], self.M, self.lanes, self.blocks)
-        sinos = []
-        sid = 0
-        for g in gs:
-            raw, stats = batch_spectral(g.plan, g.L, self.M, self.att, self.pdf)
-            B, S = raw.shape[:2]
-            sino = ops.detector(g.plan, raw.view(B * S, g.plan.V, g.plan.W), stats.view(B * S, 2),
-                                self.sigma, self.seed, sid)
-            sid += B * S
-            sinos.append(sino)
-        stack = torch.cat(sinos, 0) if len(sinos) > 1 else sinos[0]
-        fltd = ops.filter_sinogram(gs[0].plan, stack, self.filt, work=self.filter_work)
-        imgs = []
+        p0 = gs[0].plan
+        raw, stats = batch_spectral_groups(p0, [g.L for g in gs], [g.B for g in gs], self.M, self.att, self.pdf)
+        n = raw.shape[0] * raw.shape[1]
+        stack = ops.detector(p0, raw.view(n, p0.V, p0.W), stats.view(n, 2), self.sigma, self.seed, 0)
+        fltd = ops.filter_sinogram(p0, stack, self.filt, work=self.filter_work)
+        sinos, imgs, items = [], [], []
         off = 0
-        for g, s in zip(gs, sinos):
-            n = s.shape[0]
-            imgs.append(ops.backproject(g.plan, fltd[off:off + n]))
-            off += n
+        for g in gs:
+            k = g.B * self.S
+            img = torch.empty((k, g.plan.N, g.plan.N), dtype=torch.float32, device=stack.device)
+            sinos.append(stack[off:off + k])
+            imgs.append(img)
+            items.append((g.plan, fltd[off:off + k], img))
+            off += k
+        backproject_groups(items, self.bp_k)
         self.out = (sinos, imgs)
         return self.out
 

@@ -741,11 +741,8 @@ vct_status vct_minmax(const float* d_x, int n, float* h_maxmin, void* stream)
     return st;
 }
 
-vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* d_img, void* stream)
+static BackprojArgs make_bp_args(const vct_plan* p, const float* d_fltd, float* d_img)
 {
-    clear_error();
-    if (!p || !d_fltd || !d_img) return set_error(VCT_ERR_ARG, "NULL argument");
-    if (n_sino < 1 || n_sino > 65535) return set_error(VCT_ERR_ARG, "n_sino %d invalid", n_sino);
     const vct_geometry& g = p->g;
     BackprojArgs a{};
     a.fltd = d_fltd;
@@ -769,7 +766,39 @@ vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* 
     a.mask_r2 = a.mask_r * a.mask_r;
     a.exact = p->exact_bp;
     a.max_k = p->bp_max_k;
-    return launch_backproject(a, n_sino, pick_stream(p, stream));
+    return a;
+}
+
+vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* d_img, void* stream)
+{
+    clear_error();
+    if (!p || !d_fltd || !d_img) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (n_sino < 1 || n_sino > 65535) return set_error(VCT_ERR_ARG, "n_sino %d invalid", n_sino);
+    return launch_backproject(make_bp_args(p, d_fltd, d_img), n_sino, pick_stream(p, stream));
+}
+
+vct_status vct_backproject_groups(vct_plan* const* plans, const float* const* d_fltd, const int* n_sino,
+                                  float* const* d_img, int n_groups, int sinos_per_lane, void* stream)
+{
+    clear_error();
+    if (!plans || !d_fltd || !n_sino || !d_img) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (n_groups < 1 || n_groups > kMaxBpGroups)
+        return set_error(VCT_ERR_ARG, "n_groups %d outside 1..%d", n_groups, kMaxBpGroups);
+    static_assert(sizeof(BpGroups) <= 4096, "kernel arguments");
+    BpGroups ga{};
+    ga.n = n_groups;
+    int smax = 0;
+    for (int k = 0; k < n_groups; k++) {
+        if (!plans[k] || !d_fltd[k] || !d_img[k]) return set_error(VCT_ERR_ARG, "group %d: NULL argument", k);
+        if (n_sino[k] < 1 || n_sino[k] > 65535) return set_error(VCT_ERR_ARG, "group %d: n_sino %d invalid", k, n_sino[k]);
+        if (plans[k]->ctx != plans[0]->ctx) return set_error(VCT_ERR_ARG, "group %d: plan of another context", k);
+        ga.a[k] = make_bp_args(plans[k], d_fltd[k], d_img[k]);
+        ga.S[k] = n_sino[k];
+        smax = std::max(smax, n_sino[k]);
+    }
+    int K = sinos_per_lane > 0 ? sinos_per_lane : 16;
+    K = std::min(K, std::max(smax, 2));
+    return launch_backproject_groups(ga, K, (hipStream_t)stream);
 }
 
 vct_status vct_lengthproj(vct_plan* p, const float* d_proj, double z_cm, double couch_speed_cm, double iso_det_cm,

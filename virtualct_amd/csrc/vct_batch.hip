@@ -560,6 +560,7 @@ __global__ __launch_bounds__(256) void k_gather_split(GatherArgs a)
 
 // ---------------------------------------------------- batched spectral sum
 
+constexpr int kBatchSpecGroups = 16;
 struct BatchSpecArgs {
     const float* L;       // [rays] rows of 8*B floats
     const float* att;     // [S][M][NE]
@@ -570,7 +571,30 @@ struct BatchSpecArgs {
     float* w;             // [rays] rows of 8*B floats (loss mode)
     float* loss;          // [B] (loss mode)
     int rays, W, VT, B, S, M, NE;
+    // forward mode over several geometries (ngroups > 0): phantom z of the
+    // launch is phantom z - pref[g] of group g, whose rows (of 8 * Bg[g]
+    // floats) start at Lg[g]; raw and stats are indexed by z
+    int ngroups;
+    const float* Lg[kBatchSpecGroups];
+    int Bg[kBatchSpecGroups];
+    int pref[kBatchSpecGroups + 1];
 };
+
+// The rows, phantom count and phantom index in its group of launch phantom z.
+__device__ __forceinline__ void spec_slot(const BatchSpecArgs& a, int z, const float*& L, int& B, int& b)
+{
+    if (a.ngroups <= 0) {
+        L = a.L;
+        B = a.B;
+        b = z;
+        return;
+    }
+    int g = 0;
+    while (g + 1 < a.ngroups && z >= a.pref[g + 1]) g++;
+    L = a.Lg[g];
+    B = a.Bg[g];
+    b = z - a.pref[g];
+}
 
 // One past the highest bin with a nonzero probability among the bins this
 // wave's lanes hold (bin e on this lane, consecutive bins on consecutive
@@ -592,7 +616,10 @@ __global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
     float* pdf = lds + a.NE * 8;
     __shared__ uint32_t red[2][4];
     __shared__ int ne_w[4];
-    const int s = blockIdx.y, b = blockIdx.z, NE = a.NE;
+    const int s = blockIdx.y, z = blockIdx.z, NE = a.NE;
+    const float* Lrows;
+    int B, b;
+    spec_slot(a, z, Lrows, B, b);
     const float kLog2e = 1.4426950408889634f;
     for (int i = threadIdx.x; i < NE * 8; i += blockDim.x) {
         const int e = i >> 3, m = i & 7;
@@ -614,7 +641,7 @@ __global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
     float I = 0.f;
     if (valid && ray / a.W < a.VT) {
         float Lm[8];
-        load_slot<M>(a.L, ray, a.B, b, Lm);
+        load_slot<M>(Lrows, ray, B, b, Lm);
         for (int e = 0; e < ne; e++) {
             const float4 m0 = reinterpret_cast<const float4*>(mu2)[2 * e];
             const float4 m1 = reinterpret_cast<const float4*>(mu2)[2 * e + 1];
@@ -625,7 +652,7 @@ __global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
             I += pdf[e] * __builtin_amdgcn_exp2f(x);  // as k_spectral
         }
     }
-    const size_t bs = (size_t)b * a.S + s;
+    const size_t bs = (size_t)z * a.S + s;
     if (valid) a.raw[bs * a.rays + ray] = I;
     uint32_t mx = valid ? __float_as_uint(I) : 0u;
     uint32_t mn = valid ? __float_as_uint(I) : 0x7f800000u;
@@ -651,7 +678,10 @@ template <int M>
 __global__ __launch_bounds__(256) void k_batch_spectral_all(BatchSpecArgs a)
 {
     extern __shared__ float lds[];
-    const int S = a.S, NE = a.NE, b = blockIdx.z;
+    const int S = a.S, NE = a.NE, z = blockIdx.z;
+    const float* Lrows;
+    int B, b;
+    spec_slot(a, z, Lrows, B, b);
     float* mu2 = lds;                                          // [S][NE][8]
     float* pdf = lds + (size_t)S * NE * 8;                     // [S][NE]
     int* ne_w = reinterpret_cast<int*>(pdf + (size_t)S * NE);  // [S][4]
@@ -675,7 +705,7 @@ __global__ __launch_bounds__(256) void k_batch_spectral_all(BatchSpecArgs a)
     const bool valid = ray < a.rays;
     const bool hit = valid && ray / a.W < a.VT;
     float Lm[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (hit) load_slot<M>(a.L, ray, a.B, b, Lm);
+    if (hit) load_slot<M>(Lrows, ray, B, b, Lm);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int s = 0; s < S; s++) {
         const int ne = max(max(ne_w[4 * s], ne_w[4 * s + 1]), max(ne_w[4 * s + 2], ne_w[4 * s + 3]));
@@ -693,7 +723,7 @@ __global__ __launch_bounds__(256) void k_batch_spectral_all(BatchSpecArgs a)
                 I += ps[e] * __builtin_amdgcn_exp2f(x);  // as k_spectral
             }
         }
-        const size_t bs = (size_t)b * S + s;
+        const size_t bs = (size_t)z * S + s;
         if (valid) a.raw[bs * a.rays + ray] = I;
         uint32_t mx = valid ? __float_as_uint(I) : 0u;
         uint32_t mn = valid ? __float_as_uint(I) : 0x7f800000u;
@@ -1694,6 +1724,55 @@ vct_status vct_batch_spectral(vct_plan* p, const float* d_L, int B, int M, const
     hipLaunchKernelGGL(k_batch_stats_init, dim3((B * S + 63) / 64), dim3(64), 0, s, d_stats, B * S);
     // all spectra of a phantom per block while their tables fit in 64 KB of
     // LDS (two blocks per CU); else one (phantom, spectrum) per block
+    const size_t lds_all = sizeof(float) * (size_t)S * NE * 9 + sizeof(int) * 4 * S;
+    if (lds_all <= 64 * 1024) {
+        const dim3 grid((a.rays + 255) / 256, 1, B);
+        switch (M) {
+#define VCT_SPEC_CASE(MM) \
+    case MM: hipLaunchKernelGGL((k_batch_spectral_all<MM>), grid, dim3(256), lds_all, s, a); break;
+            VCT_SPEC_CASE(1) VCT_SPEC_CASE(2) VCT_SPEC_CASE(3) VCT_SPEC_CASE(4)
+            VCT_SPEC_CASE(5) VCT_SPEC_CASE(6) VCT_SPEC_CASE(7) VCT_SPEC_CASE(8)
+#undef VCT_SPEC_CASE
+        }
+    } else {
+        const dim3 grid((a.rays + 255) / 256, S, B);
+        const size_t lds = sizeof(float) * (size_t)NE * 9;
+        switch (M) {
+#define VCT_SPEC_CASE(MM) \
+    case MM: hipLaunchKernelGGL((k_batch_spectral<MM>), grid, dim3(256), lds, s, a); break;
+            VCT_SPEC_CASE(1) VCT_SPEC_CASE(2) VCT_SPEC_CASE(3) VCT_SPEC_CASE(4)
+            VCT_SPEC_CASE(5) VCT_SPEC_CASE(6) VCT_SPEC_CASE(7) VCT_SPEC_CASE(8)
+#undef VCT_SPEC_CASE
+        }
+    }
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status vct_batch_spectral_groups(vct_plan* p, const float* const* d_L, const int* n_phantoms, int n_groups, int M,
+                                     const float* d_att, const float* d_pdf, int NE, int S, float* d_raw,
+                                     uint32_t* d_stats, void* stream)
+{
+    clear_error();
+    if (!p || !d_L || !n_phantoms || !d_raw || !d_stats) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (n_groups < 1 || n_groups > kBatchSpecGroups)
+        return set_error(VCT_ERR_ARG, "n_groups %d outside 1..%d", n_groups, kBatchSpecGroups);
+    BatchSpecArgs a{};
+    a.ngroups = n_groups;
+    a.pref[0] = 0;
+    for (int g = 0; g < n_groups; g++) {
+        VCT_CHECK(check_spec(p, d_L[g], n_phantoms[g], M, d_att, d_pdf, NE, S));
+        a.Lg[g] = d_L[g];
+        a.Bg[g] = n_phantoms[g];
+        a.pref[g + 1] = a.pref[g] + n_phantoms[g];
+    }
+    const int B = a.pref[n_groups];
+    if (B > 65535) return set_error(VCT_ERR_ARG, "%d phantoms in one launch", B);
+    hipStream_t s = (hipStream_t)stream;
+    a.att = d_att; a.pdf = d_pdf; a.raw = d_raw; a.stats = d_stats;
+    a.rays = p->g.n_views * p->g.n_channels; a.W = p->g.n_channels; a.VT = p->VT;
+    a.B = B; a.S = S; a.M = M; a.NE = NE;
+    hipLaunchKernelGGL(k_batch_stats_init, dim3((B * S + 63) / 64), dim3(64), 0, s, d_stats, B * S);
     const size_t lds_all = sizeof(float) * (size_t)S * NE * 9 + sizeof(int) * 4 * S;
     if (lds_all <= 64 * 1024) {
         const dim3 grid((a.rays + 255) / 256, 1, B);

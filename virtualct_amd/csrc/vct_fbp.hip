@@ -311,13 +311,14 @@ __device__ __forceinline__ BpView bp_view(const BackprojArgs& a, double xx, doub
 // with the single-sinogram expression; the images are bit-identical for
 // every K.  Workgroup z covers sinograms K*z ... K*z + K - 1; past S the
 // lane re-reads sinogram S - 1 and stores nothing.
+// One tile (bx, by) of sinograms bz*K .. bz*K + K - 1 of k_backproject_multi.
 template <bool WIDE, int K>
-__global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_multi(BackprojArgs a, int S)
+__device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int bx, int by, int bz)
 {
     const int tx = threadIdx.x & (kBpTile - 1), ty = threadIdx.x / kBpTile;
-    const int x = blockIdx.x * kBpTile + tx;
-    const int y = blockIdx.y * kBpTile + ty;
-    const int s0 = blockIdx.z * K;
+    const int x = bx * kBpTile + tx;
+    const int y = by * kBpTile + ty;
+    const int s0 = bz * K;
     const float* fl[K];
 #pragma unroll
     for (int k = 0; k < K; k++) fl[k] = a.fltd + (size_t)min(s0 + k, S - 1) * a.V * a.W;
@@ -369,6 +370,29 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_multi(Backpro
         for (int k = 0; k < K; k++)
             if (s0 + k < S) a.img[((size_t)(s0 + k) * a.N + y) * a.N + x] = acc[k];
     }
+}
+
+template <bool WIDE, int K>
+__global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_multi(BackprojArgs a, int S)
+{
+    bp_multi_tile<WIDE, K>(a, S, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Several geometries in one launch: workgroup l belongs to the group g with
+// pref[g] <= l < pref[g + 1] (a scalar search, uniform over the workgroup)
+// and runs tile l - pref[g] of that group's tiles x slices, exactly as
+// k_backproject_multi would.
+template <bool WIDE, int K>
+__global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_groups(BpGroups ga)
+{
+    const int l = blockIdx.x;
+    int g = 0;
+    while (g + 1 < ga.n && l >= ga.pref[g + 1]) g++;
+    const int local = l - ga.pref[g];
+    const int tx = ga.tiles_x[g];
+    const int per_slice = tx * tx;
+    const int bz = local / per_slice, r = local - bz * per_slice;
+    bp_multi_tile<WIDE, K>(ga.a[g], ga.S[g], r % tx, r / tx, bz);
 }
 
 // One sinogram by orbits of the quarter turn.  With V views over 360 degrees
@@ -505,6 +529,44 @@ static void launch_bp_multi(const BackprojArgs& a, int S, int K, hipStream_t s)
     }
     const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, (S + K0 - 1) / K0);
     hipLaunchKernelGGL((k_backproject_multi<WIDE, K0>), grid, dim3(kBpTile * kBpTile), 0, s, a, S);
+}
+
+template <bool WIDE, int K0>
+static void launch_bp_groups_k(const BpGroups& ga, int K, hipStream_t s)
+{
+    if constexpr (K0 < kBpSinos) {
+        if (K > K0) {
+            launch_bp_groups_k<WIDE, K0 + 1>(ga, K, s);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_backproject_groups<WIDE, K0>), dim3(ga.pref[ga.n]), dim3(kBpTile * kBpTile), 0, s, ga);
+}
+
+vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s)
+{
+    if (ga.n < 1 || ga.n > kMaxBpGroups) return set_error(VCT_ERR_ARG, "%d groups (1..%d)", ga.n, kMaxBpGroups);
+    if (K < 2) K = 2;
+    if (K > kBpSinos) K = kBpSinos;
+    bool wide = false;
+    long long total = 0;
+    ga.pref[0] = 0;
+    for (int g = 0; g < ga.n; g++) {
+        const BackprojArgs& a = ga.a[g];
+        if (ga.S[g] < 1) return set_error(VCT_ERR_ARG, "group %d has no sinograms", g);
+        const double r = a.mask_r * a.pix;
+        wide = wide || !(r < a.sid) || r / std::sqrt(a.sid * a.sid - r * r) > 0.49;
+        ga.tiles_x[g] = (a.N + kBpTile - 1) / kBpTile;
+        total += (long long)ga.tiles_x[g] * ga.tiles_x[g] * ((ga.S[g] + K - 1) / K);
+        if (total > INT_MAX) return set_error(VCT_ERR_ARG, "grouped backprojection too large");
+        ga.pref[g + 1] = (int)total;
+    }
+    if (wide)
+        launch_bp_groups_k<true, 2>(ga, K, s);
+    else
+        launch_bp_groups_k<false, 2>(ga, K, s);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
 }
 
 template <bool WIDE>

@@ -115,6 +115,17 @@ struct BackprojArgs {
     double pix, sid, dbeta_ch, inv_dbeta_ch, dbeta_view, xfactor, center, mask_r2, mask_r;
 };
 
+// Backprojections of up to kMaxBpGroups geometries in one launch
+// (launch_backproject_groups fills tiles_x and pref).
+constexpr int kMaxBpGroups = 16;
+struct BpGroups {
+    BackprojArgs a[kMaxBpGroups];
+    int S[kMaxBpGroups];        // sinograms of each group
+    int tiles_x[kMaxBpGroups];  // tiles per image row
+    int pref[kMaxBpGroups + 1]; // first workgroup of each group
+    int n;
+};
+
 struct LengthprojArgs {
     const float* proj;    // [count][W] sinogram window
     const double* cosT;   // [V]
@@ -181,6 +192,7 @@ vct_status launch_pack_i16(const int16_t* src, int M, int N, uint16_t* dst, hipS
 vct_status launch_pack_f32(const float* src, int M, int N, float* dst, hipStream_t s);
 vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipStream_t s);
 vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s);
+vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s);
 vct_status launch_beam_hardening(const float* in, float* out, int n, double alpha, double beta, hipStream_t s);
 vct_status launch_lengthproj(const LengthprojArgs& a, hipStream_t s);
 vct_status plan_filter_matrix(vct_plan* plan, int filter, float** d_bpk, float** d_toep);
