@@ -310,8 +310,8 @@ __device__ __forceinline__ BpView bp_view(const BackprojArgs& a, double xx, doub
 // sinogram, so a lane computes it once and applies it to K sinograms, each
 // with the single-sinogram expression; the images are bit-identical for
 // every K.  Workgroup z covers sinograms K*z ... K*z + K - 1; past S the
-// lane re-reads sinogram S - 1 and stores nothing.
-// One tile (bx, by) of sinograms bz*K .. bz*K + K - 1 of k_backproject_multi.
+// lane re-reads sinogram S - 1 and stores nothing.  bp_multi_tile is one
+// tile (bx, by) of workgroup z = bz.
 template <bool WIDE, int K>
 __device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int bx, int by, int bz)
 {
@@ -404,8 +404,10 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_groups(BpGrou
 // A workgroup holds 64 base pixels (an 8 x 8 tile of the quadrant); wave w
 // runs views v in [wQ, (w+1)Q) with the software pipeline of
 // k_backproject_multi, so each pixel's 4 partial sums (wave 0 .. 3) are added
-// in that order through LDS.  Each term is the single-sinogram expression;
-// the order of the float sum differs from the reference's ascending views.
+// in that order through LDS.  The geometry follows the single-sinogram
+// expression in FP64; the interpolation and distance weights of a term are
+// applied in float, and the order of the float sum differs from the
+// reference's ascending views (images within 1e-5 of the reference's).
 template <bool WIDE>
 __global__ __launch_bounds__(256) void k_backproject_sym(BackprojArgs a)
 {
@@ -420,6 +422,7 @@ __global__ __launch_bounds__(256) void k_backproject_sym(BackprojArgs a)
     const double xx = -(x - a.center) * a.pix;  // :52-53
     const double yy = -(y - a.center) * a.pix;
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    const double dbx = a.dbeta_view * a.xfactor;
     const int vb = wave * Q, ve = vb + Q;
     if (masked && Q > 0) {
         // rows of view v + kQ (mod V); vb + kQ < V + 3Q, one wrap at most
@@ -446,10 +449,14 @@ __global__ __launch_bounds__(256) void k_backproject_sym(BackprojArgs a)
                 n0[k] = row[0];
                 n1[k] = row[1];
             }
+            // the interpolation weights and the distance weight times the
+            // view step, once for the four pixels, in float: the sums already
+            // run in another order than the reference's, and these roundings
+            // (~6e-8 relative per term) stay far below that difference
+            const float w0f = (float)g.w0, wf = (float)g.w, sf = (float)(g.rL2 * dbx);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const double sum = g.w0 * r0[k] + g.w * r1[k];  // + 0.0 left out as in k_backproject
-                const float term = (float)(g.rL2 * sum * a.dbeta_view * a.xfactor);
+                const float term = sf * fmaf(wf, r1[k], w0f * r0[k]);
                 acc[k] = g.ok ? acc[k] + term : acc[k];
             }
             g = gn;
