@@ -123,7 +123,8 @@ vct_status vct_plan_destroy(vct_plan* plan);
  * than any ray of the geometry needs).
  * VCT_OPT_EXACT_BACKPROJECT = 4: 1 keeps, for a single sinogram, the
  * backprojection that adds each pixel's views in the reference's order
- * (>= 99% of pixels bit-identical to it for the same filtered input).  By
+ * (>= 99% of pixels bit-identical to it for the same filtered input), and
+ * the reference's FP64 terms for several sinograms per call.  By
  * default, when the view count is a multiple of 4 and N is even, one
  * sinogram is backprojected by orbits of the 90-degree rotation (pixel p
  * at view v + V/4 sees the geometry of pixel (y, N-1-x) at view v), which
@@ -296,11 +297,16 @@ vct_status vct_lengthproj_window(vct_plan* plan, const float* d_proj, int proj_w
 /* Pixel-driven fan backprojection, replacing reconstructImage
  * (F/reconstructImageTOMO.cpp:11-105), n_sino images.  Several sinograms in
  * one call share the per-pixel geometry (up to 16 per pass), so one call with
- * n_sino = S is faster than S calls; the images are bit-identical either way. */
+ * n_sino = S is faster than S calls.  By default their terms apply the
+ * interpolation and distance weights in float (within 1e-6 relative of the
+ * reference's FP64 terms); with VCT_OPT_EXACT_BACKPROJECT each image equals a
+ * single-sinogram call bit for bit. */
 vct_status vct_backproject(vct_plan* plan, const float* d_fltd, int n_sino, float* d_img,
                            void* stream);
 /* The backprojections of n_groups (<= 16) geometries in one launch, as one
- * vct_backproject call per group (images bit-identical): group k takes
+ * vct_backproject call per group (images bit-identical when the plans share
+ * VCT_OPT_EXACT_BACKPROJECT; if any asks for it, all groups get exact
+ * terms): group k takes
  * n_sino[k] filtered sinograms at d_fltd[k] and writes d_img[k], with its
  * plan's geometry; sinos_per_lane (2..16, 0: 16) is the number of sinograms
  * a lane sums at once, shared by all groups.  Serves batches of phantoms of

@@ -300,3 +300,29 @@ def test_backprojection_more_views_than_one_trig_block(gpu, oracle):
     ref = oracle.backproject_ref(g, fl)
     assert rel_max(img, ref) < 1e-5
     assert np.count_nonzero(img != ref) < 0.05 * img.size
+
+
+@pytest.mark.parametrize("N,pix,V,S", [(512, 0.1, 800, 16), (512, 0.1, 800, 5), (128, 0.6, 160, 6),
+                                        (64, 0.7, 90, 3), (96, 0.5, 801, 2)])
+def test_multi_sinogram_float_weights_match_exact_terms(gpu, oracle, N, pix, V, S):
+    """By default several sinograms per call apply the interpolation and
+    distance weights in float: within 1e-6 of the exact FP64 terms (same view
+    order) and 1e-5 of the reference, and each image does not depend on how
+    the call is split (one call of S, or S/2 + S/2)."""
+    import torch
+    ops = _ops()
+    rng = np.random.default_rng(70 + S)
+    fl = _t(rng.normal(size=(S, V, 609)).astype(np.float32) * 1e3, gpu)
+    plan = ops.Plan(_geom(N, pix, V))
+    fast = ops.backproject(plan, fl)
+    exact_plan = _exact_plan(_geom(N, pix, V))
+    exact = ops.backproject(exact_plan, fl)
+    assert rel_max(fast.cpu().numpy(), exact.cpu().numpy()) < 1e-6
+    ref = oracle.backproject_ref(Geom(N=N, pix=pix, V=V), fl[S - 1].cpu().numpy())
+    assert rel_max(fast[S - 1].cpu().numpy(), ref) < 1e-5
+    h = S // 2
+    if h >= 2:
+        halves = torch.cat([ops.backproject(plan, fl[:h].contiguous()), ops.backproject(plan, fl[h:].contiguous())])
+        assert torch.equal(halves, fast)
+    plan.close()
+    exact_plan.close()

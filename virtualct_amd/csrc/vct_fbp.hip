@@ -311,10 +311,16 @@ __device__ __forceinline__ BpView bp_view(const BackprojArgs& a, double xx, doub
 // with the single-sinogram expression; the images are bit-identical for
 // every K.  Workgroup z covers sinograms K*z ... K*z + K - 1; past S the
 // lane re-reads sinogram S - 1 and stores nothing.  bp_multi_tile is one
-// tile (bx, by) of workgroup z = bz.
-template <bool WIDE, int K>
+// tile (bx, by) of workgroup z = bz.  FW (float weights, the default unless
+// the plan asks for the exact backprojection): the interpolation and
+// distance weights of a term are rounded to float once per pixel-view and
+// applied in float, as in k_backproject_sym (within ~1e-7 of the exact
+// terms); otherwise every term is the single-sinogram FP64 expression and
+// the images equal k_backproject's bit for bit.
+template <bool WIDE, int K, bool FW>
 __device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int bx, int by, int bz)
 {
+    const double dbx = a.dbeta_view * a.xfactor;
     const int tx = threadIdx.x & (kBpTile - 1), ty = threadIdx.x / kBpTile;
     const int x = bx * kBpTile + tx;
     const int y = by * kBpTile + ty;
@@ -353,8 +359,13 @@ __device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int 
             }
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                const double sum = g.w0 * r0[k] + g.w * r1[k];  // + 0.0 left out as above
-                const float term = (float)(g.rL2 * sum * a.dbeta_view * a.xfactor);
+                float term;
+                if constexpr (FW) {
+                    term = (float)(g.rL2 * dbx) * fmaf((float)g.w, r1[k], (float)g.w0 * r0[k]);
+                } else {
+                    const double sum = g.w0 * r0[k] + g.w * r1[k];  // + 0.0 left out as above
+                    term = (float)(g.rL2 * sum * a.dbeta_view * a.xfactor);
+                }
                 acc[k] = g.ok ? acc[k] + term : acc[k];
             }
             g = gn;
@@ -372,17 +383,17 @@ __device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int 
     }
 }
 
-template <bool WIDE, int K>
+template <bool WIDE, int K, bool FW>
 __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_multi(BackprojArgs a, int S)
 {
-    bp_multi_tile<WIDE, K>(a, S, blockIdx.x, blockIdx.y, blockIdx.z);
+    bp_multi_tile<WIDE, K, FW>(a, S, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Several geometries in one launch: workgroup l belongs to the group g with
 // pref[g] <= l < pref[g + 1] (a scalar search, uniform over the workgroup)
 // and runs tile l - pref[g] of that group's tiles x slices, exactly as
 // k_backproject_multi would.
-template <bool WIDE, int K>
+template <bool WIDE, int K, bool FW>
 __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_groups(BpGroups ga)
 {
     const int l = blockIdx.x;
@@ -392,7 +403,7 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_groups(BpGrou
     const int tx = ga.tiles_x[g];
     const int per_slice = tx * tx;
     const int bz = local / per_slice, r = local - bz * per_slice;
-    bp_multi_tile<WIDE, K>(ga.a[g], ga.S[g], r % tx, r / tx, bz);
+    bp_multi_tile<WIDE, K, FW>(ga.a[g], ga.S[g], r % tx, r / tx, bz);
 }
 
 // One sinogram by orbits of the quarter turn.  With V views over 360 degrees
@@ -535,19 +546,22 @@ static void launch_bp_multi(const BackprojArgs& a, int S, int K, hipStream_t s)
         }
     }
     const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, (S + K0 - 1) / K0);
-    hipLaunchKernelGGL((k_backproject_multi<WIDE, K0>), grid, dim3(kBpTile * kBpTile), 0, s, a, S);
+    if (a.exact)
+        hipLaunchKernelGGL((k_backproject_multi<WIDE, K0, false>), grid, dim3(kBpTile * kBpTile), 0, s, a, S);
+    else
+        hipLaunchKernelGGL((k_backproject_multi<WIDE, K0, true>), grid, dim3(kBpTile * kBpTile), 0, s, a, S);
 }
 
-template <bool WIDE, int K0>
+template <bool WIDE, bool FW, int K0>
 static void launch_bp_groups_k(const BpGroups& ga, int K, hipStream_t s)
 {
     if constexpr (K0 < kBpSinos) {
         if (K > K0) {
-            launch_bp_groups_k<WIDE, K0 + 1>(ga, K, s);
+            launch_bp_groups_k<WIDE, FW, K0 + 1>(ga, K, s);
             return;
         }
     }
-    hipLaunchKernelGGL((k_backproject_groups<WIDE, K0>), dim3(ga.pref[ga.n]), dim3(kBpTile * kBpTile), 0, s, ga);
+    hipLaunchKernelGGL((k_backproject_groups<WIDE, K0, FW>), dim3(ga.pref[ga.n]), dim3(kBpTile * kBpTile), 0, s, ga);
 }
 
 vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s)
@@ -555,11 +569,12 @@ vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s)
     if (ga.n < 1 || ga.n > kMaxBpGroups) return set_error(VCT_ERR_ARG, "%d groups (1..%d)", ga.n, kMaxBpGroups);
     if (K < 2) K = 2;
     if (K > kBpSinos) K = kBpSinos;
-    bool wide = false;
+    bool wide = false, exact = false;
     long long total = 0;
     ga.pref[0] = 0;
     for (int g = 0; g < ga.n; g++) {
         const BackprojArgs& a = ga.a[g];
+        exact = exact || a.exact;
         if (ga.S[g] < 1) return set_error(VCT_ERR_ARG, "group %d has no sinograms", g);
         const double r = a.mask_r * a.pix;
         wide = wide || !(r < a.sid) || r / std::sqrt(a.sid * a.sid - r * r) > 0.49;
@@ -568,10 +583,10 @@ vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s)
         if (total > INT_MAX) return set_error(VCT_ERR_ARG, "grouped backprojection too large");
         ga.pref[g + 1] = (int)total;
     }
-    if (wide)
-        launch_bp_groups_k<true, 2>(ga, K, s);
-    else
-        launch_bp_groups_k<false, 2>(ga, K, s);
+    if (wide && exact) launch_bp_groups_k<true, false, 2>(ga, K, s);
+    else if (wide) launch_bp_groups_k<true, true, 2>(ga, K, s);
+    else if (exact) launch_bp_groups_k<false, false, 2>(ga, K, s);
+    else launch_bp_groups_k<false, true, 2>(ga, K, s);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
@@ -595,12 +610,14 @@ static void launch_bp_w(const BackprojArgs& a, int S, hipStream_t s)
         hipLaunchKernelGGL(k_backproject_sym<WIDE>, grid, dim3(256), 0, s, a);
         return;
     }
-    if (K == 1) {
+    if (K == 1 && (a.exact || S == 1)) {
         const dim3 grid((a.N + kBpTile - 1) / kBpTile, (a.N + kBpTile - 1) / kBpTile, S);
         hipLaunchKernelGGL(k_backproject<WIDE>, grid, dim3(kBpTile * kBpTile), 0, s, a);
         return;
     }
-    launch_bp_multi<WIDE, 2>(a, S, K, s);
+    // (several sinograms one per lane take the float-weight template too, so
+    // an image does not depend on how the call was split)
+    launch_bp_multi<WIDE, 1>(a, S, K, s);
 }
 
 vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s)
