@@ -284,3 +284,19 @@ def test_tile_adjoint_nonfinite_weight_reaches_its_voxels(gpu):
     keep = ~crossed
     assert np.abs(dirty[..., 0][keep] - clean[..., 0][keep]).max() <= 1e-5 * np.abs(clean[..., 0]).max()
     assert np.abs(dirty[..., 1:] - clean[..., 1:]).max() <= 1e-5 * np.abs(clean[..., 1:]).max()
+
+
+def test_tile_adjoint_scales_with_tiny_and_huge_weights(gpu):
+    """The fixed-point tile sums take their scale from the workgroup's weights,
+    and weights too small for that scale sum in float: the gradient of
+    weights times 1e-30 or 1e30 is the gradient times 1e-30 or 1e30."""
+    import torch
+    ops = _ops()
+    plan = ops.plan_for(_geom(128, 0.35, 160))
+    g = torch.Generator(device=gpu).manual_seed(13)
+    w = torch.zeros((160, 609, 8), device=gpu)
+    w[..., :4] = torch.rand((160, 609, 4), generator=g, device=gpu) + 0.1
+    base = ops.trace_adjoint(plan, w, 4).cpu().numpy().astype(np.float64)
+    for f in (1e-30, 1e-20, 1e20, 1e30):
+        got = ops.trace_adjoint(plan, w * f, 4).cpu().numpy().astype(np.float64) / f
+        assert rel_max(got, base) < 1e-6, f

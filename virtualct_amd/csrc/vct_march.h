@@ -386,13 +386,13 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
                         if (v2) sink.add_ab(ia2, ib2, fabs(na - s_a) * sec);
                     } else if constexpr (kRot) {
                         // every candidate segment with the floors of its index values; the
-                        // sink applies each ray's own in-image test (no crossing: length 0)
-                        sink.add2_fl(fa1, fb1, cross ? fabs(xa - p0.a) * sec : 0.0, fa2, fb2, fabs(na - s_a) * sec);
+                        // sink applies each ray's own in-image test (no crossing: the first
+                        // candidate is skipped)
+                        sink.add2_fl(cross, fa1, fb1, fabs(xa - p0.a) * sec, fa2, fb2, fabs(na - s_a) * sec);
                     } else if constexpr (Sink::kAdd2) {
-                        // empty segments: length 0 at voxel 0, which adds nothing
                         // (a voxel index is only used when its segment is in the image)
-                        sink.add2(v1 ? mul24(ia1, stA) + mul24(ib1, stB) : 0, v1 ? fabs(xa - p0.a) * sec : 0.0,
-                                  v2 ? mul24(ia2, stA) + mul24(ib2, stB) : 0, v2 ? fabs(na - s_a) * sec : 0.0);
+                        sink.add2(v1, mul24(ia1, stA) + mul24(ib1, stB), fabs(xa - p0.a) * sec,
+                                  v2, mul24(ia2, stA) + mul24(ib2, stB), fabs(na - s_a) * sec);
                     } else {
                         if (v1) sink.add(mul24(ia1, stA) + mul24(ib1, stB), fabs(xa - p0.a) * sec);
                         if (v2) sink.add(mul24(ia2, stA) + mul24(ib2, stB), fabs(na - s_a) * sec);

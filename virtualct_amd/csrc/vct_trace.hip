@@ -39,8 +39,9 @@ namespace vct {
 // sink per 32-bit word of two materials: the word as one integer and its high
 // half, which resolves the low half to ~1e-11 relative).  add2
 // takes the two candidate segments of one march step and issues both gathers
-// before using either; a segment of length 0 at voxel 0 adds exactly nothing
-// (fma(rho, +0, acc) == acc), so the march calls it without branching.
+// before using either, each only on the lanes where it is a segment in the
+// image: the gathers' texture path, not the FP64 arithmetic, is what the
+// march keeps busiest (78% / 84% TA / TD busy with unmasked gathers).
 template <int M>
 struct SinkU16 {
     static constexpr bool kAdd2 = true;
@@ -76,11 +77,15 @@ struct SinkU16 {
         for (int k = 0; k < M / 2; k++) acch[k] = __builtin_fma((double)(w[k] >> 16), rr, acch[k]);
     }
     __device__ __forceinline__ void add(int v, double rr) { accumulate(load(v), rr); }
-    __device__ __forceinline__ void add2(int v1, double r1, int v2, double r2)
+    // the two candidate segments of a committed step, each only on lanes
+    // where it is a segment (ok1, ok2): the others issue no gather
+    __device__ __forceinline__ void add2(bool ok1, int v1, double r1, bool ok2, int v2, double r2)
     {
-        const uint4 q1 = load(v1), q2 = load(v2);
-        accumulate(q1, r1);
-        accumulate(q2, r2);
+        uint4 q1, q2;
+        if (ok1) q1 = load(v1);
+        if (ok2) q2 = load(v2);
+        if (ok1) accumulate(q1, r1);
+        if (ok2) accumulate(q2, r2);
     }
     __device__ __forceinline__ void finish()
     {
@@ -123,11 +128,13 @@ struct SinkF32 {
         }
     }
     __device__ __forceinline__ void add(int v, double rr) { accumulate(load(v), rr); }
-    __device__ __forceinline__ void add2(int v1, double r1, int v2, double r2)
+    __device__ __forceinline__ void add2(bool ok1, int v1, double r1, bool ok2, int v2, double r2)
     {
-        const Vox q1 = load(v1), q2 = load(v2);
-        accumulate(q1, r1);
-        accumulate(q2, r2);
+        Vox q1, q2;
+        if (ok1) q1 = load(v1);
+        if (ok2) q2 = load(v2);
+        if (ok1) accumulate(q1, r1);
+        if (ok2) accumulate(q2, r2);
     }
 };
 
@@ -270,19 +277,28 @@ struct SinkRot {
             r[k] = in ? rr : 0.0;
         }
     }
-    __host__ __device__ __forceinline__ void add2_fl(int fa1, int fb1, double r1, int fa2, int fb2, double r2)
+    // the two candidate segments of a committed step; the first only on lanes
+    // whose step crosses a minor grid line (cross), so lanes without a
+    // crossing issue no gathers for it (the gathers' texture path is what the
+    // kernel keeps busiest); a segment of length 0 would add exactly nothing
+    __host__ __device__ __forceinline__ void add2_fl(bool cross, int fa1, int fb1, double r1, int fa2, int fb2,
+                                                     double r2)
     {
         int v1[R], v2[R];
         double q1r[R], q2r[R];
         voxels(fa1, fb1, r1, v1, q1r);
         voxels(fa2, fb2, r2, v2, q2r);
         Vox q1[R], q2[R];
-        #pragma unroll
-        for (int k = 0; k < R; k++) q1[k] = load(v1[k]);
+        if (cross) {
+            #pragma unroll
+            for (int k = 0; k < R; k++) q1[k] = load(v1[k]);
+        }
         #pragma unroll
         for (int k = 0; k < R; k++) q2[k] = load(v2[k]);
-        #pragma unroll
-        for (int k = 0; k < R; k++) accumulate(k, q1[k], q1r[k]);
+        if (cross) {
+            #pragma unroll
+            for (int k = 0; k < R; k++) accumulate(k, q1[k], q1r[k]);
+        }
         #pragma unroll
         for (int k = 0; k < R; k++) accumulate(k, q2[k], q2r[k]);
     }
@@ -621,14 +637,16 @@ __global__ __launch_bounds__(256) void k_trace_adj_tile(TraceArgs a, int ntx, in
     const float wmax = __uint_as_float(wmax_bits);
     // segment lengths are at most sqrt(2) max(1 cm, D): the first step runs from
     // the entry to the integer-cm start plane, the others one pitch
+    // (weights so small that the scale would leave float's range, wl below
+    // 2^-80, sum in float too, as non-finite ones do)
     const float wl = wmax * (float)(2.0 * fmax(1.0, D));
-    const bool fixed = nonfinite == 0u && wl <= 3.0e38f;
+    if (nonfinite == 0u && !(wmax > 0.f)) return;  // nothing to add (uniform over the workgroup)
+    const bool fixed = nonfinite == 0u && wl <= 3.0e38f && wl >= 0x1p-80f;
     int S = 0;
     if (fixed) {
-        if (!(wmax > 0.f)) return;  // nothing to add (uniform over the workgroup)
         int e;
-        frexpf(wl, &e);  // wl < 2^e
-        S = min(30 - e, 120);
+        frexpf(wl, &e);  // wl < 2^e, e >= -79
+        S = 30 - e;      // <= 109
     }
     const float scale = ldexpf(1.f, S);
     bool ovf = false;
