@@ -15,7 +15,8 @@
  *   - A stream argument is a hipStream_t passed as void* (NULL = HIP's
  *     null stream, which is ordered with all blocking streams such as
  *     PyTorch's default stream).  Calls on one stream are ordered; calls
- *     that share the context scratch buffer must not overlap across streams.
+ *     that share the context scratch buffer must not overlap across streams,
+ *     nor may traces on one plan (they share the plan's density work copy).
  *     The scratch only grows: a larger request allocates a new buffer and
  *     the old one stays allocated until vct_terminate, so work recorded in a
  *     HIP graph never writes into freed memory.
@@ -193,7 +194,10 @@ vct_status vct_pack_densities_f32(vct_plan* plan, const float* d_dens, int M,
  * ray's segments of rho_m * length, g/cm^2, divided by xfactor).  Each ray
  * is traced once, independent of the spectrum.  d_counts (optional, int32
  * [V][W]) receives the reference's per-ray segment counter
- * (reprojection_sum, :195/:280). */
+ * (reprojection_sum, :195/:280).  With orbits (VCT_OPT_TRACE_ORBIT), the
+ * call first writes d_packed transposed into a work copy that the plan
+ * allocates with itself (N*N*32 bytes when N and V are even), so that the
+ * gathers of rays along x read rows too. */
 vct_status vct_trace_paths(vct_plan* plan, const void* d_packed, int dens_type, int M,
                            float* d_L, int32_t* d_counts, void* stream);
 

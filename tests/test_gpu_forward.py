@@ -303,13 +303,16 @@ def test_step_bound_overflow_is_reported(gpu, golden):
     plan.close()
 
 
-@pytest.mark.parametrize("N,pix,V", [(512, 0.1, 800), (768, 50.0 / 768, 800), (128, 0.35, 162)])
+@pytest.mark.parametrize("N,pix,V", [(512, 0.1, 800), (768, 50.0 / 768, 800), (128, 0.35, 162),
+                                     (100, 0.5, 400)])
 def test_orbit_trace_equals_single_ray_march(gpu, golden, N, pix, V):
     """The default tracer marches the ray of view v once for the rays of views
     v + kV/4 (V a multiple of 4; 162 views take half turns): its segment
     counters equal marching every ray by itself on every ray, and the path
     integrals agree to 1e-7 (the orbit shares the base ray's lengths), for
-    integer and float densities."""
+    integer and float densities.  The orbit tracer gathers from a transposed
+    copy of the densities for rays along x (N = 100 leaves partial 32 x 32
+    tiles in the transpose)."""
     import torch
     ops = _ops()
     from virtualct_amd import _lib, phantoms

@@ -163,7 +163,8 @@ def test_batched_row_layout():
 
 
 def test_tuner_keeps_the_fastest_and_caches_it(tmp_path):
-    """The tuner times every candidate once, keeps the fastest, writes it to
+    """The tuner times every candidate in one untimed pass and three
+    interleaved timed passes, keeps the one with the best time, writes it to
     its cache file and answers later asks (and new processes) from there."""
     from virtualct_amd.autotune import Tuner
     path = str(tmp_path / "tune.json")
@@ -174,7 +175,11 @@ def test_tuner_keeps_the_fastest_and_caches_it(tmp_path):
         calls.append(cfg["x"])
         return {1: 3.0, 2: 1.0, 3: 2.0}[cfg["x"]]
     best = t.best(("k", 1), lambda: [{"x": 1}, {"x": 2}, {"x": 3}], time_fn)
-    assert best == {"x": 2} and calls == [1, 2, 3]
-    assert t.best(("k", 1), lambda: [{"x": 1}], time_fn) == {"x": 2} and calls == [1, 2, 3]
+    assert best == {"x": 2} and calls == [1, 2, 3] * 4
+    assert t.best(("k", 1), lambda: [{"x": 1}], time_fn) == {"x": 2} and calls == [1, 2, 3] * 4
+    # a candidate timed slow while clocks ramp up still wins on its best pass
+    seq = {1: [9.0, 9.0, 1.5, 1.5], 2: [2.0, 2.0, 2.0, 2.0]}
+    ramp = lambda cfg: seq[cfg["x"]].pop(0)
+    assert t.best(("ramp",), lambda: [{"x": 1}, {"x": 2}], ramp) == {"x": 1}
     assert Tuner(path).best(("k", 1), lambda: [{"x": 1}], time_fn) == {"x": 2}
     assert t.best(("k", 2), lambda: [{"x": 3}, {"x": 1}], time_fn) == {"x": 3}

@@ -106,6 +106,7 @@ TraceArgs make_trace_args(const vct_plan* p)
     a.exact = p->exact_trace;
     a.block = p->trace_block > 0 ? p->trace_block : 256;
     a.adj_views = p->adj_views;
+    a.dens_t = p->d_dens_t;
     // orbits of the half or quarter turn: every view traced (V even), a grid
     // centred on a voxel corner (N even), the guarded march (no exact option,
     // no r >= rmax test)
@@ -383,6 +384,15 @@ vct_status vct_plan_create(vct_ctx* ctx, const vct_geometry* g, vct_plan** out)
     if (st == VCT_OK) st = plan_upload(px0.data(), W * sizeof(double), (void**)&p->d_px0);
     if (st == VCT_OK) st = plan_upload(py0.data(), W * sizeof(double), (void**)&p->d_py0);
     if (st == VCT_OK) st = plan_upload(coeff.data(), W * sizeof(double), (void**)&p->d_coeff);
+    // the orbit tracer's work copy (any dens type), allocated with the plan
+    // so that no trace call allocates
+    if (st == VCT_OK && g->recon_size % 2 == 0 && V % 2 == 0) {
+        const size_t nn = (size_t)g->recon_size * g->recon_size;
+        if (hipMalloc(&p->d_dens_t, nn * 8 * sizeof(float)) != hipSuccess) {
+            (void)hipGetLastError();
+            st = set_error(VCT_ERR_NOMEM, "plan work copy of %zu bytes", nn * 8 * sizeof(float));
+        }
+    }
     if (st != VCT_OK) {
         vct_plan_destroy(p);
         return st;
@@ -436,6 +446,7 @@ vct_status vct_plan_destroy(vct_plan* p)
         if (b) (void)hipFree(b);
     for (float* b : p->d_toep)
         if (b) (void)hipFree(b);
+    if (p->d_dens_t) (void)hipFree(p->d_dens_t);
     delete p;
     return VCT_OK;
 }

@@ -151,21 +151,40 @@ struct SinkF32 {
 // unsafe: a decision of the march that a turned ray might take otherwise did
 // not clear the margin (the caller then traces rays 1..R-1 by themselves;
 // ray 0 is right in any case).
+__host__ __device__ __forceinline__ void swap_d(double& x, double& y)
+{
+    const double t = x;
+    x = y;
+    y = t;
+}
+
 template <int M, int DT, int R>
 struct SinkRot {
     static_assert(R == 1 || R == 2 || R == 4, "orbits of 1, 2 or 4 rays");
     static constexpr bool kAdd2 = true;
     static constexpr bool kRot = true;
     static constexpr int NW = (M + 1) / 2;  // u16: 32-bit words of two materials
-    const char* __restrict__ vox;
+    // Gathers read the copy of the densities whose rows run along the
+    // minor axis of the slot's ray ([y][x] voxels when y is its major axis,
+    // the transposed [x][y] copy when x is), so the lanes of a wave, which
+    // hold neighbouring rays at the same major step, meet neighbouring voxels
+    // of one row.  Slot k holds ray k (the base ray turned k quarter turns),
+    // except that rays 1 and 3 trade slots when y is the base ray's major
+    // axis; finish() puts them back.
+    const char* __restrict__ voxE;  // slots 0 and 2 (and 1 when R == 2)
+    const char* __restrict__ voxO;  // slots 1 and 3 when R == 4
     bool xmajor;
     int N;
     bool unsafe;
     double accw[R][NW], acch[R][M / 2];  // u16: word sums and high halves
     double acc[R][M];                    // f32: sums; u16: after finish()
-    __host__ __device__ void init(const void* p, bool xm, int n)
+    // p: the packed [y][x] voxels, pt: the same transposed ([x][y])
+    __host__ __device__ void init(const void* p, const void* pt, bool xm, int n)
     {
-        vox = reinterpret_cast<const char*>(p);
+        const char* P = reinterpret_cast<const char*>(p);
+        const char* T = reinterpret_cast<const char*>(pt);
+        voxE = xm ? T : P;
+        voxO = xm ? P : T;
         xmajor = xm;
         N = n;
         unsafe = false;
@@ -179,29 +198,33 @@ struct SinkRot {
             for (int m = 0; m < M; m++) acc[k][m] = 0.0;
         }
     }
-    // voxel of ray k (0 with rr = 0 when outside the image)
+    // slot k of a segment whose floors are (fa, fb) on the base ray's (major,
+    // minor) axes: the (major, minor) coordinates of the turned ray, indexed
+    // minor + N * major in its copy, with the reference's truncation of
+    // index -1 to voxel 0 (0 with rr = 0 when outside the image)
     __host__ __device__ __forceinline__ void voxels(int fa, int fb, double rr, int (&v)[R], double (&r)[R]) const
     {
-        const int fx = xmajor ? fa : fb, fy = xmajor ? fb : fa;
         const int n1 = N - 1;
+        const int ta = n1 - fa, tb = n1 - fb;
         #pragma unroll
         for (int k = 0; k < R; k++) {
-            int gx, gy;
-            if (R == 1 || k == 0) { gx = fx; gy = fy; }
-            else if (R == 2 || k == 2) { gx = n1 - fx; gy = n1 - fy; }
-            else if (k == 1) { gx = n1 - fy; gy = fx; }
-            else { gx = fy; gy = n1 - fx; }
-            const bool in = gx >= -1 && gx <= n1 && gy >= -1 && gy <= n1;
-            v[k] = in ? (gx + (gx < 0)) + N * (gy + (gy < 0)) : 0;
+            int ma, mi;
+            if (R == 1 || k == 0) { ma = fa; mi = fb; }
+            else if (R == 2 || k == 2) { ma = ta; mi = tb; }
+            else if (k == 1) { ma = fa; mi = tb; }
+            else { ma = ta; mi = fb; }
+            const bool in = (unsigned)(ma + 1) <= (unsigned)N && (unsigned)(mi + 1) <= (unsigned)N;
+            v[k] = in ? max(mi, 0) + mul24(max(ma, 0), N) : 0;
             r[k] = in ? rr : 0.0;
         }
     }
     struct Vox {
         uint4 lo, hi;  // u16: lo = the 8 materials; f32: lo = materials 0-3, hi = 4-7
     };
-    __host__ __device__ __forceinline__ Vox load(int v) const
+    __host__ __device__ __forceinline__ Vox load(int k, int v) const
     {
         Vox q;
+        const char* vox = (R == 4 && (k & 1)) ? voxO : voxE;
         if constexpr (DT == VCT_DENS_U16) {
             q.lo = *reinterpret_cast<const uint4*>(vox + (uint32_t)v * 16u);
             q.hi = make_uint4(0u, 0u, 0u, 0u);
@@ -241,7 +264,7 @@ struct SinkRot {
         voxels(fa, fb, rr, v, r);
         Vox q[R];
         #pragma unroll
-        for (int k = 0; k < R; k++) q[k] = load(v[k]);
+        for (int k = 0; k < R; k++) q[k] = load(k, v[k]);
         #pragma unroll
         for (int k = 0; k < R; k++) accumulate(k, q[k], r[k]);
     }
@@ -256,7 +279,7 @@ struct SinkRot {
         voxels_t(ta, tb, rr, v, r);
         Vox q[R];
         #pragma unroll
-        for (int k = 0; k < R; k++) q[k] = load(v[k]);
+        for (int k = 0; k < R; k++) q[k] = load(k, v[k]);
         #pragma unroll
         for (int k = 0; k < R; k++) accumulate(k, q[k], r[k]);
     }
@@ -273,8 +296,14 @@ struct SinkRot {
             else { gx = ty + c; gy = -tx + c; }
             const int ix = trunc_i32(gx), iy = trunc_i32(gy);
             const bool in = ix >= 0 && ix <= N - 1 && iy >= 0 && iy <= N - 1;
-            v[k] = in ? ix + N * iy : 0;
-            r[k] = in ? rr : 0.0;
+            // ray k goes to slot k, but rays 1 and 3 trade slots when y is the
+            // base ray's major axis; its major axis is x when xmajor differs
+            // from its own quarter-turn parity
+            const bool odd = R == 4 && (k & 1);
+            const int slot = (odd && !xmajor) ? 4 - k : k;
+            const bool majx = xmajor != odd;
+            v[slot] = in ? (majx ? iy + N * ix : ix + N * iy) : 0;
+            r[slot] = in ? rr : 0.0;
         }
     }
     // the two candidate segments of a committed step; the first only on lanes
@@ -291,10 +320,10 @@ struct SinkRot {
         Vox q1[R], q2[R];
         if (cross) {
             #pragma unroll
-            for (int k = 0; k < R; k++) q1[k] = load(v1[k]);
+            for (int k = 0; k < R; k++) q1[k] = load(k, v1[k]);
         }
         #pragma unroll
-        for (int k = 0; k < R; k++) q2[k] = load(v2[k]);
+        for (int k = 0; k < R; k++) q2[k] = load(k, v2[k]);
         if (cross) {
             #pragma unroll
             for (int k = 0; k < R; k++) accumulate(k, q1[k], q1r[k]);
@@ -304,6 +333,15 @@ struct SinkRot {
     }
     __host__ __device__ __forceinline__ void finish()
     {
+        if (R == 4 && !xmajor) {
+            // slots 1 and 3 hold rays 3 and 1
+            #pragma unroll
+            for (int w = 0; w < NW; w++) swap_d(accw[1 % R][w], accw[3 % R][w]);
+            #pragma unroll
+            for (int w = 0; w < M / 2; w++) swap_d(acch[1 % R][w], acch[3 % R][w]);
+            #pragma unroll
+            for (int m = 0; m < M; m++) swap_d(acc[1 % R][m], acc[3 % R][m]);
+        }
         if constexpr (DT == VCT_DENS_U16) {
             #pragma unroll
             for (int k = 0; k < R; k++)
@@ -484,7 +522,7 @@ __global__ __launch_bounds__(256) void k_trace_fwd_rot(TraceArgs a)
         done = true;
     } else if (safe && r.dB != 0.0 && !a.rcheck && !a.exact) {
         SinkRot<M, DT, R> sink;
-        sink.init(a.dens, r.xmajor, a.N);
+        sink.init(a.dens, a.dens_t, r.xmajor, a.N);
         bool ovf = false;
         int k0 = 0, k1 = INT_MAX;
         if (P > 1) {
@@ -899,6 +937,34 @@ __global__ __launch_bounds__(256) void k_pack_f32(const float* __restrict__ src,
     d[1] = make_float4(r[4], r[5], r[6], r[7]);
 }
 
+// out = in transposed, [N][N] voxels of CH 16-byte words each: out voxel
+// (x, y) = in voxel (y, x), by 32 x 32 voxel tiles through LDS.
+template <int CH>
+__global__ __launch_bounds__(256) void k_transpose_voxels(const uint4* __restrict__ in, uint4* __restrict__ out,
+                                                          int N)
+{
+    __shared__ uint4 tile[32][32 * CH + 1];
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int j = ty; j < 32; j += 8) {
+        const int y = y0 + j;
+#pragma unroll
+        for (int c = tx; c < 32 * CH; c += 32) {
+            const int x = x0 + c / CH;
+            if (y < N && x < N) tile[j][c] = in[((size_t)y * N + x) * CH + c % CH];
+        }
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int x = x0 + i;
+#pragma unroll
+        for (int c = tx; c < 32 * CH; c += 32) {
+            const int y = y0 + c / CH;
+            if (x < N && y < N) out[((size_t)x * N + y) * CH + c % CH] = tile[c / CH][i * CH + c % CH];
+        }
+    }
+}
+
 // --------------------------------------------------------------- launchers
 
 // Lanes per orbit: four, so that the launch holds about four times as many
@@ -920,8 +986,15 @@ static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint
     const dim3 grid((a.n_rays + a.block - 1) / a.block);
     const bool whole = a.ray_offset == 0 && a.n_rays == a.V * a.W;
     if (!adjoint && whole && (a.rot == 4 || a.rot == 2)) {
-        if (a.V % a.rot != 0 || a.N % 2 != 0 || a.VT != a.V || a.exact || a.rcheck)
+        if (a.V % a.rot != 0 || a.N % 2 != 0 || a.VT != a.V || a.exact || a.rcheck || !a.dens_t)
             return set_error(VCT_ERR_INTERNAL, "orbit trace with V %d, N %d", a.V, a.N);
+        const dim3 tgrid((unsigned)((a.N + 31) / 32), (unsigned)((a.N + 31) / 32));
+        if (dens_type == VCT_DENS_U16)
+            hipLaunchKernelGGL((k_transpose_voxels<1>), tgrid, dim3(256), 0, s,
+                               reinterpret_cast<const uint4*>(a.dens), reinterpret_cast<uint4*>(a.dens_t), a.N);
+        else
+            hipLaunchKernelGGL((k_transpose_voxels<2>), tgrid, dim3(256), 0, s,
+                               reinterpret_cast<const uint4*>(a.dens), reinterpret_cast<uint4*>(a.dens_t), a.N);
         if (dens_type == VCT_DENS_U16) {
             if (a.rot == 4) launch_trace_rot<M, VCT_DENS_U16, 4>(a, s);
             else launch_trace_rot<M, VCT_DENS_U16, 2>(a, s);
