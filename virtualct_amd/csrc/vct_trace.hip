@@ -306,13 +306,55 @@ struct SinkRot {
             r[slot] = in ? rr : 0.0;
         }
     }
+    // slot indices and in-image flags of one candidate segment (the index of
+    // a slot outside the image is not used)
+    __host__ __device__ __forceinline__ void slots(int fa, int fb, int (&v)[R], bool (&in)[R]) const
+    {
+        const int n1 = N - 1;
+        const int ta = n1 - fa, tb = n1 - fb;
+        #pragma unroll
+        for (int k = 0; k < R; k++) {
+            int ma, mi;
+            if (R == 1 || k == 0) { ma = fa; mi = fb; }
+            else if (R == 2 || k == 2) { ma = ta; mi = tb; }
+            else if (k == 1) { ma = fa; mi = tb; }
+            else { ma = ta; mi = fb; }
+            in[k] = (unsigned)(ma + 1) <= (unsigned)N && (unsigned)(mi + 1) <= (unsigned)N;
+            v[k] = max(mi, 0) + mul24(max(ma, 0), N);
+        }
+    }
     // the two candidate segments of a committed step; the first only on lanes
     // whose step crosses a minor grid line (cross), so lanes without a
     // crossing issue no gathers for it (the gathers' texture path is what the
-    // kernel keeps busiest); a segment of length 0 would add exactly nothing
+    // kernel keeps busiest); a segment of length 0 would add exactly nothing.
+    // u16: slots outside the image neither gather nor add (the two lengths
+    // stay shared by the four slots, which saves registers and selects);
+    // f32 keeps zeroed lengths, which compiles to fewer branches there.
     __host__ __device__ __forceinline__ void add2_fl(bool cross, int fa1, int fb1, double r1, int fa2, int fb2,
                                                      double r2)
     {
+        if constexpr (DT == VCT_DENS_U16) {
+            int v1[R], v2[R];
+            bool i1[R], i2[R];
+            slots(fa1, fb1, v1, i1);
+            slots(fa2, fb2, v2, i2);
+            Vox q1[R], q2[R];
+            #pragma unroll
+            for (int k = 0; k < R; k++) {
+                i1[k] = i1[k] && cross;
+                if (i1[k]) q1[k] = load(k, v1[k]);
+            }
+            #pragma unroll
+            for (int k = 0; k < R; k++)
+                if (i2[k]) q2[k] = load(k, v2[k]);
+            #pragma unroll
+            for (int k = 0; k < R; k++)
+                if (i1[k]) accumulate(k, q1[k], r1);
+            #pragma unroll
+            for (int k = 0; k < R; k++)
+                if (i2[k]) accumulate(k, q2[k], r2);
+            return;
+        }
         int v1[R], v2[R];
         double q1r[R], q2r[R];
         voxels(fa1, fb1, r1, v1, q1r);
@@ -738,13 +780,19 @@ __global__ __launch_bounds__(256) void k_trace_adj_tile(TraceArgs a, int ntx, in
 
 // Polyenergetic sum: one spectrum per blockIdx.y; tables in LDS as
 // mu2[e][8] = -mu_m(E_e) * log2(e) and pdf[e].
+// Rays per workgroup of k_spectral: each workgroup adds its max and min of
+// I to the spectrum's two statistics words with one atomic each, and those
+// atomics serialise on the one address (1904 workgroups of 256 rays took as
+// long as the sums at 800 x 609), so workgroups are large.
+constexpr int kSpecBlock = 1024;
+
 template <int M, bool DI>
-__global__ __launch_bounds__(256) void k_spectral(SpectralArgs a)
+__global__ __launch_bounds__(kSpecBlock) void k_spectral(SpectralArgs a)
 {
     extern __shared__ float lds[];
     float* mu2 = lds;
     float* pdf = lds + a.NE * 8;
-    __shared__ uint32_t red[2][4];
+    __shared__ uint32_t red[2][kSpecBlock / 64];
     const int s = blockIdx.y;
     const int NE = a.NE;
     const float kLog2e = 1.4426950408889634f;
@@ -988,6 +1036,7 @@ static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint
     if (!adjoint && whole && (a.rot == 4 || a.rot == 2)) {
         if (a.V % a.rot != 0 || a.N % 2 != 0 || a.VT != a.V || a.exact || a.rcheck || !a.dens_t)
             return set_error(VCT_ERR_INTERNAL, "orbit trace with V %d, N %d", a.V, a.N);
+        // the orbit gathers of rays along x read the transposed copy
         const dim3 tgrid((unsigned)((a.N + 31) / 32), (unsigned)((a.N + 31) / 32));
         if (dens_type == VCT_DENS_U16)
             hipLaunchKernelGGL((k_transpose_voxels<1>), tgrid, dim3(256), 0, s,
@@ -1044,12 +1093,12 @@ template <int M>
 static void launch_spectral_m(const SpectralArgs& a, hipStream_t s)
 {
     const int nr = a.V * a.W;
-    const dim3 grid((nr + 255) / 256, a.S);
+    const dim3 grid((nr + kSpecBlock - 1) / kSpecBlock, a.S);
     const size_t lds = sizeof(float) * (size_t)a.NE * 9;
     if (a.dIdL)
-        hipLaunchKernelGGL((k_spectral<M, true>), grid, dim3(256), lds, s, a);
+        hipLaunchKernelGGL((k_spectral<M, true>), grid, dim3(kSpecBlock), lds, s, a);
     else
-        hipLaunchKernelGGL((k_spectral<M, false>), grid, dim3(256), lds, s, a);
+        hipLaunchKernelGGL((k_spectral<M, false>), grid, dim3(kSpecBlock), lds, s, a);
 }
 
 vct_status launch_spectral(const SpectralArgs& a, hipStream_t s)
