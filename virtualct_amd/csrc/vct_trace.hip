@@ -164,6 +164,9 @@ struct SinkRot {
     static constexpr bool kAdd2 = true;
     static constexpr bool kRot = true;
     static constexpr int NW = (M + 1) / 2;  // u16: 32-bit words of two materials
+    // u16 with M <= 6: both copies are the plan's, of 12-byte voxels (the
+    // three words that hold the materials), so the two fit the L2 better
+    static constexpr bool k12 = DT == VCT_DENS_U16 && M <= 6;
     // Gathers read the copy of the densities whose rows run along the
     // minor axis of the slot's ray ([y][x] voxels when y is its major axis,
     // the transposed [x][y] copy when x is), so the lanes of a wave, which
@@ -178,11 +181,16 @@ struct SinkRot {
     bool unsafe;
     double accw[R][NW], acch[R][M / 2];  // u16: word sums and high halves
     double acc[R][M];                    // f32: sums; u16: after finish()
-    // p: the packed [y][x] voxels, pt: the same transposed ([x][y])
+    // p: the packed [y][x] voxels, pt: the same transposed ([x][y]); with
+    // k12, pt holds both 12-byte copies, [y][x] then [x][y]
     __host__ __device__ void init(const void* p, const void* pt, bool xm, int n)
     {
         const char* P = reinterpret_cast<const char*>(p);
         const char* T = reinterpret_cast<const char*>(pt);
+        if constexpr (k12) {
+            P = T;
+            T = T + (size_t)n * n * 12;
+        }
         voxE = xm ? T : P;
         voxO = xm ? P : T;
         xmajor = xm;
@@ -225,7 +233,11 @@ struct SinkRot {
     {
         Vox q;
         const char* vox = (R == 4 && (k & 1)) ? voxO : voxE;
-        if constexpr (DT == VCT_DENS_U16) {
+        if constexpr (k12) {
+            const uint3 t = *reinterpret_cast<const uint3*>(vox + (uint32_t)v * 12u);
+            q.lo = make_uint4(t.x, t.y, t.z, 0u);
+            q.hi = make_uint4(0u, 0u, 0u, 0u);
+        } else if constexpr (DT == VCT_DENS_U16) {
             q.lo = *reinterpret_cast<const uint4*>(vox + (uint32_t)v * 16u);
             q.hi = make_uint4(0u, 0u, 0u, 0u);
         } else {
@@ -1013,6 +1025,30 @@ __global__ __launch_bounds__(256) void k_transpose_voxels(const uint4* __restric
     }
 }
 
+// Both 12-byte work copies of u16 densities (materials 0-5): yx[y][x] and
+// xy[x][y], by 32 x 32 voxel tiles through LDS.
+__global__ __launch_bounds__(256) void k_pack_copies12(const uint4* __restrict__ in, uint3* __restrict__ yx,
+                                                       uint3* __restrict__ xy, int N)
+{
+    __shared__ uint3 tile[32][33];
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int j = ty; j < 32; j += 8) {
+        const int y = y0 + j, x = x0 + tx;
+        if (y < N && x < N) {
+            const uint4 q = in[(size_t)y * N + x];
+            const uint3 t = make_uint3(q.x, q.y, q.z);
+            tile[j][tx] = t;
+            yx[(size_t)y * N + x] = t;
+        }
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int x = x0 + i, y = y0 + tx;
+        if (x < N && y < N) xy[(size_t)x * N + y] = tile[tx][i];
+    }
+}
+
 // --------------------------------------------------------------- launchers
 
 // Lanes per orbit: four, so that the launch holds about four times as many
@@ -1036,9 +1072,15 @@ static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint
     if (!adjoint && whole && (a.rot == 4 || a.rot == 2)) {
         if (a.V % a.rot != 0 || a.N % 2 != 0 || a.VT != a.V || a.exact || a.rcheck || !a.dens_t)
             return set_error(VCT_ERR_INTERNAL, "orbit trace with V %d, N %d", a.V, a.N);
-        // the orbit gathers of rays along x read the transposed copy
+        // the orbit gathers of rays along x read the transposed copy (u16 with
+        // M <= 6: both copies are rewritten with 12-byte voxels)
         const dim3 tgrid((unsigned)((a.N + 31) / 32), (unsigned)((a.N + 31) / 32));
-        if (dens_type == VCT_DENS_U16)
+        if (dens_type == VCT_DENS_U16 && M <= 6)
+            hipLaunchKernelGGL(k_pack_copies12, tgrid, dim3(256), 0, s, reinterpret_cast<const uint4*>(a.dens),
+                               reinterpret_cast<uint3*>(a.dens_t),
+                               reinterpret_cast<uint3*>(reinterpret_cast<char*>(a.dens_t) + (size_t)a.N * a.N * 12),
+                               a.N);
+        else if (dens_type == VCT_DENS_U16)
             hipLaunchKernelGGL((k_transpose_voxels<1>), tgrid, dim3(256), 0, s,
                                reinterpret_cast<const uint4*>(a.dens), reinterpret_cast<uint4*>(a.dens_t), a.N);
         else
