@@ -304,7 +304,7 @@ def test_step_bound_overflow_is_reported(gpu, golden):
 
 
 @pytest.mark.parametrize("N,pix,V", [(512, 0.1, 800), (768, 50.0 / 768, 800), (128, 0.35, 162),
-                                     (100, 0.5, 400)])
+                                     (100, 0.5, 400), (16, 0.5, 40)])
 def test_orbit_trace_equals_single_ray_march(gpu, golden, N, pix, V):
     """The default tracer marches the ray of view v once for the rays of views
     v + kV/4 (V a multiple of 4; 162 views take half turns): its segment
@@ -312,7 +312,8 @@ def test_orbit_trace_equals_single_ray_march(gpu, golden, N, pix, V):
     integrals agree to 1e-7 (the orbit shares the base ray's lengths), for
     integer and float densities.  The orbit tracer gathers from a transposed
     copy of the densities for rays along x (N = 100 leaves partial 32 x 32
-    tiles in the transpose)."""
+    tiles in the transpose; at N = 16 the padded u16 copies are larger than
+    the 32-byte transposed one)."""
     import torch
     ops = _ops()
     from virtualct_amd import _lib, phantoms

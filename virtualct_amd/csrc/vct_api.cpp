@@ -387,10 +387,13 @@ vct_status vct_plan_create(vct_ctx* ctx, const vct_geometry* g, vct_plan** out)
     // the orbit tracer's work copy (any dens type), allocated with the plan
     // so that no trace call allocates
     if (st == VCT_OK && g->recon_size % 2 == 0 && V % 2 == 0) {
-        const size_t nn = (size_t)g->recon_size * g->recon_size;
-        if (hipMalloc(&p->d_dens_t, nn * 8 * sizeof(float)) != hipSuccess) {
+        // the transposed copy of 32-byte voxels, or the two padded copies of
+        // 12-byte voxels ((N + 4)^2 each), whichever is larger
+        const size_t n = (size_t)g->recon_size, np = n + 4;
+        const size_t bytes = std::max(n * n * 8 * sizeof(float), 2 * np * np * 12);
+        if (hipMalloc(&p->d_dens_t, bytes) != hipSuccess) {
             (void)hipGetLastError();
-            st = set_error(VCT_ERR_NOMEM, "plan work copy of %zu bytes", nn * 8 * sizeof(float));
+            st = set_error(VCT_ERR_NOMEM, "plan work copy of %zu bytes", bytes);
         }
     }
     if (st != VCT_OK) {

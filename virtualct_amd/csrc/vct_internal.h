@@ -43,7 +43,7 @@ struct TraceArgs {
     const double* py0;    // [W] detector point before rotation (y)
     const void* dens;     // packed [N][N][8] (u16 or f32)
     void* dens_t;         // orbit tracer: work copies of dens (the transposed [x][y] one; u16 with
-                          // M <= 6: [y][x] and [x][y] copies of 12-byte voxels)
+                          // M <= 6: [y][x] and [x][y] copies of 12-byte voxels padded to N + 4)
     float* L;             // [V][W][8] path integrals (forward)
     int32_t* counts;      // [V][W] optional segment counter
     const float* wts;     // [V][W][8] adjoint weights (adjoint)
@@ -174,7 +174,7 @@ struct vct_plan {
     double* d_coeff = nullptr;
     float* d_bpk[3] = {nullptr, nullptr, nullptr};   // packed filter matrix per type
     float* d_toep[3] = {nullptr, nullptr, nullptr};  // Toeplitz generator per type
-    void* d_dens_t = nullptr;  // N x N x 32 B: the orbit tracer's density work copies (N, V even)
+    void* d_dens_t = nullptr;  // the orbit tracer's density work copies (N, V even)
     int exact_trace = 0;  // VCT_OPT_EXACT_TRACE
     int max_steps = 0;    // VCT_OPT_MAX_STEPS (0: the automatic bound)
     int exact_bp = 0;     // VCT_OPT_EXACT_BACKPROJECT

@@ -165,7 +165,11 @@ struct SinkRot {
     static constexpr bool kRot = true;
     static constexpr int NW = (M + 1) / 2;  // u16: 32-bit words of two materials
     // u16 with M <= 6: both copies are the plan's, of 12-byte voxels (the
-    // three words that hold the materials), so the two fit the L2 better
+    // three words that hold the materials), so the two fit the L2 better,
+    // and padded by two voxels on every side: row and column -1 repeat row
+    // and column 0 (the reference truncates index -1 to voxel 0), rows and
+    // columns -2, N and N + 1 are zero, so the committed steps' gathers need
+    // no in-image test (a voxel outside the image reads zeros)
     static constexpr bool k12 = DT == VCT_DENS_U16 && M <= 6;
     // Gathers read the copy of the densities whose rows run along the
     // minor axis of the slot's ray ([y][x] voxels when y is its major axis,
@@ -178,6 +182,8 @@ struct SinkRot {
     const char* __restrict__ voxO;  // slots 1 and 3 when R == 4
     bool xmajor;
     int N;
+    int NP;   // row pitch of the copies in voxels: N, or N + 4 when padded
+    int ORG;  // index of voxel (0, 0): 0, or 2 NP + 2 when padded
     bool unsafe;
     double accw[R][NW], acch[R][M / 2];  // u16: word sums and high halves
     double acc[R][M];                    // f32: sums; u16: after finish()
@@ -187,9 +193,13 @@ struct SinkRot {
     {
         const char* P = reinterpret_cast<const char*>(p);
         const char* T = reinterpret_cast<const char*>(pt);
+        NP = n;
+        ORG = 0;
         if constexpr (k12) {
+            NP = n + 4;
+            ORG = 2 * NP + 2;
             P = T;
-            T = T + (size_t)n * n * 12;
+            T = T + (size_t)NP * NP * 12;
         }
         voxE = xm ? T : P;
         voxO = xm ? P : T;
@@ -222,7 +232,7 @@ struct SinkRot {
             else if (k == 1) { ma = fa; mi = tb; }
             else { ma = ta; mi = fb; }
             const bool in = (unsigned)(ma + 1) <= (unsigned)N && (unsigned)(mi + 1) <= (unsigned)N;
-            v[k] = in ? max(mi, 0) + mul24(max(ma, 0), N) : 0;
+            v[k] = in ? ORG + max(mi, 0) + mul24(max(ma, 0), NP) : 0;
             r[k] = in ? rr : 0.0;
         }
     }
@@ -314,7 +324,7 @@ struct SinkRot {
             const bool odd = R == 4 && (k & 1);
             const int slot = (odd && !xmajor) ? 4 - k : k;
             const bool majx = xmajor != odd;
-            v[slot] = in ? (majx ? iy + N * ix : ix + N * iy) : 0;
+            v[slot] = in ? ORG + (majx ? iy + NP * ix : ix + NP * iy) : 0;
             r[slot] = in ? rr : 0.0;
         }
     }
@@ -342,9 +352,53 @@ struct SinkRot {
     // u16: slots outside the image neither gather nor add (the two lengths
     // stay shared by the four slots, which saves registers and selects);
     // f32 keeps zeroed lengths, which compiles to fewer branches there.
+    // padded copies: byte offsets of the four slots of one candidate,
+    // coordinates clamped to [-2, N + 1] (values outside read zeros there too)
+    __host__ __device__ __forceinline__ void offsets12(int fa, int fb, uint32_t (&o)[R]) const
+    {
+        const int a = min(max(fa + 2, 0), N + 3), b = min(max(fb + 2, 0), N + 3);
+        const int ta = N + 3 - a, tb = N + 3 - b;  // (N - 1 - f) + 2
+        const uint32_t ra = (uint32_t)mul24(a, 12 * NP), rt = (uint32_t)mul24(ta, 12 * NP);
+        const uint32_t cb = (uint32_t)mul24(b, 12), ct = (uint32_t)mul24(tb, 12);
+        #pragma unroll
+        for (int k = 0; k < R; k++) {
+            if (R == 1 || k == 0) o[k] = ra + cb;
+            else if (R == 2 || k == 2) o[k] = rt + ct;
+            else if (k == 1) o[k] = ra + ct;
+            else o[k] = rt + cb;
+        }
+    }
+    __host__ __device__ __forceinline__ Vox load12(int k, uint32_t o) const
+    {
+        const char* vox = (R == 4 && (k & 1)) ? voxO : voxE;
+        const uint3 t = *reinterpret_cast<const uint3*>(vox + o);
+        Vox q;
+        q.lo = make_uint4(t.x, t.y, t.z, 0u);
+        q.hi = make_uint4(0u, 0u, 0u, 0u);
+        return q;
+    }
     __host__ __device__ __forceinline__ void add2_fl(bool cross, int fa1, int fb1, double r1, int fa2, int fb2,
                                                      double r2)
     {
+        if constexpr (k12) {
+            uint32_t o1[R], o2[R];
+            offsets12(fa1, fb1, o1);
+            offsets12(fa2, fb2, o2);
+            Vox q1[R], q2[R];
+            if (cross) {
+                #pragma unroll
+                for (int k = 0; k < R; k++) q1[k] = load12(k, o1[k]);
+            }
+            #pragma unroll
+            for (int k = 0; k < R; k++) q2[k] = load12(k, o2[k]);
+            if (cross) {
+                #pragma unroll
+                for (int k = 0; k < R; k++) accumulate(k, q1[k], r1);
+            }
+            #pragma unroll
+            for (int k = 0; k < R; k++) accumulate(k, q2[k], r2);
+            return;
+        }
         if constexpr (DT == VCT_DENS_U16) {
             int v1[R], v2[R];
             bool i1[R], i2[R];
@@ -1025,27 +1079,33 @@ __global__ __launch_bounds__(256) void k_transpose_voxels(const uint4* __restric
     }
 }
 
-// Both 12-byte work copies of u16 densities (materials 0-5): yx[y][x] and
-// xy[x][y], by 32 x 32 voxel tiles through LDS.
+// Both 12-byte work copies of u16 densities (materials 0-5), padded by two
+// voxels on every side (NP = N + 4 per row): yx[y + 2][x + 2] and
+// xy[x + 2][y + 2] hold voxel (x, y) for x, y in [-1, N - 1], index -1
+// reading voxel 0 as the reference's truncation does, and zeros elsewhere;
+// by 32 x 32 tiles of the padded grid through LDS.
 __global__ __launch_bounds__(256) void k_pack_copies12(const uint4* __restrict__ in, uint3* __restrict__ yx,
                                                        uint3* __restrict__ xy, int N)
 {
     __shared__ uint3 tile[32][33];
-    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const int NP = N + 4;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;  // padded coordinates
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int j = ty; j < 32; j += 8) {
-        const int y = y0 + j, x = x0 + tx;
-        if (y < N && x < N) {
-            const uint4 q = in[(size_t)y * N + x];
-            const uint3 t = make_uint3(q.x, q.y, q.z);
-            tile[j][tx] = t;
-            yx[(size_t)y * N + x] = t;
+        const int pr = r0 + j, pc = c0 + tx;
+        const int y = pr - 2, x = pc - 2;
+        uint3 t = make_uint3(0u, 0u, 0u);
+        if (y >= -1 && y < N && x >= -1 && x < N) {
+            const uint4 q = in[(size_t)max(y, 0) * N + max(x, 0)];
+            t = make_uint3(q.x, q.y, q.z);
         }
+        tile[j][tx] = t;
+        if (pr < NP && pc < NP) yx[(size_t)pr * NP + pc] = t;
     }
     __syncthreads();
     for (int i = ty; i < 32; i += 8) {
-        const int x = x0 + i, y = y0 + tx;
-        if (x < N && y < N) xy[(size_t)x * N + y] = tile[tx][i];
+        const int pr = c0 + i, pc = r0 + tx;  // row x + 2, column y + 2
+        if (pr < NP && pc < NP) xy[(size_t)pr * NP + pc] = tile[tx][i];
     }
 }
 
@@ -1075,10 +1135,12 @@ static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint
         // the orbit gathers of rays along x read the transposed copy (u16 with
         // M <= 6: both copies are rewritten with 12-byte voxels)
         const dim3 tgrid((unsigned)((a.N + 31) / 32), (unsigned)((a.N + 31) / 32));
+        const int NP = a.N + 4;
+        const dim3 pgrid((unsigned)((NP + 31) / 32), (unsigned)((NP + 31) / 32));
         if (dens_type == VCT_DENS_U16 && M <= 6)
-            hipLaunchKernelGGL(k_pack_copies12, tgrid, dim3(256), 0, s, reinterpret_cast<const uint4*>(a.dens),
+            hipLaunchKernelGGL(k_pack_copies12, pgrid, dim3(256), 0, s, reinterpret_cast<const uint4*>(a.dens),
                                reinterpret_cast<uint3*>(a.dens_t),
-                               reinterpret_cast<uint3*>(reinterpret_cast<char*>(a.dens_t) + (size_t)a.N * a.N * 12),
+                               reinterpret_cast<uint3*>(reinterpret_cast<char*>(a.dens_t) + (size_t)NP * NP * 12),
                                a.N);
         else if (dens_type == VCT_DENS_U16)
             hipLaunchKernelGGL((k_transpose_voxels<1>), tgrid, dim3(256), 0, s,
