@@ -674,8 +674,13 @@ __global__ __launch_bounds__(256) void k_batch_spectral(BatchSpecArgs a)
 // Forward mode with every spectrum of phantom blockIdx.z in one block: the
 // tables and the ray's path integrals are loaded once for all S spectra.
 // Each (phantom, spectrum, ray) is summed exactly as in k_batch_spectral.
+// Rays per workgroup of k_batch_spectral_all: each workgroup adds its max and
+// min of each (phantom, spectrum) to that pair's statistics words with one
+// atomic each, which serialise per address, so workgroups are large.
+constexpr int kSpecAllBlock = 1024, kSpecAllWaves = kSpecAllBlock / 64;
+
 template <int M>
-__global__ __launch_bounds__(256) void k_batch_spectral_all(BatchSpecArgs a)
+__global__ __launch_bounds__(kSpecAllBlock) void k_batch_spectral_all(BatchSpecArgs a)
 {
     extern __shared__ float lds[];
     const int S = a.S, NE = a.NE, z = blockIdx.z;
@@ -684,8 +689,8 @@ __global__ __launch_bounds__(256) void k_batch_spectral_all(BatchSpecArgs a)
     spec_slot(a, z, Lrows, B, b);
     float* mu2 = lds;                                          // [S][NE][8]
     float* pdf = lds + (size_t)S * NE * 8;                     // [S][NE]
-    int* ne_w = reinterpret_cast<int*>(pdf + (size_t)S * NE);  // [S][4]
-    __shared__ uint32_t red[2][4];
+    int* ne_w = reinterpret_cast<int*>(pdf + (size_t)S * NE);  // [S][kSpecAllWaves]
+    __shared__ uint32_t red[2][kSpecAllWaves];
     const float kLog2e = 1.4426950408889634f;
     for (int i = threadIdx.x; i < S * NE * 8; i += blockDim.x) {
         const int s = i / (NE * 8), r = i - s * NE * 8, e = r >> 3, m = r & 7;
@@ -698,7 +703,7 @@ __global__ __launch_bounds__(256) void k_batch_spectral_all(BatchSpecArgs a)
             pdf[s * NE + e] = p;
             last = max(last, last_nonzero_bin(e, p));
         }
-        if ((threadIdx.x & 63) == 0) ne_w[4 * s + (threadIdx.x >> 6)] = last;
+        if ((threadIdx.x & 63) == 0) ne_w[kSpecAllWaves * s + (threadIdx.x >> 6)] = last;
     }
     __syncthreads();
     const int ray = blockIdx.x * blockDim.x + threadIdx.x;
@@ -708,7 +713,8 @@ __global__ __launch_bounds__(256) void k_batch_spectral_all(BatchSpecArgs a)
     if (hit) load_slot<M>(Lrows, ray, B, b, Lm);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int s = 0; s < S; s++) {
-        const int ne = max(max(ne_w[4 * s], ne_w[4 * s + 1]), max(ne_w[4 * s + 2], ne_w[4 * s + 3]));
+        int ne = 0;
+        for (int w = 0; w < kSpecAllWaves; w++) ne = max(ne, ne_w[kSpecAllWaves * s + w]);
         const float* mus = mu2 + (size_t)s * NE * 8;
         const float* ps = pdf + (size_t)s * NE;
         float I = 0.f;
@@ -1724,12 +1730,12 @@ vct_status vct_batch_spectral(vct_plan* p, const float* d_L, int B, int M, const
     hipLaunchKernelGGL(k_batch_stats_init, dim3((B * S + 63) / 64), dim3(64), 0, s, d_stats, B * S);
     // all spectra of a phantom per block while their tables fit in 64 KB of
     // LDS (two blocks per CU); else one (phantom, spectrum) per block
-    const size_t lds_all = sizeof(float) * (size_t)S * NE * 9 + sizeof(int) * 4 * S;
+    const size_t lds_all = sizeof(float) * (size_t)S * NE * 9 + sizeof(int) * kSpecAllWaves * S;
     if (lds_all <= 64 * 1024) {
-        const dim3 grid((a.rays + 255) / 256, 1, B);
+        const dim3 grid((a.rays + kSpecAllBlock - 1) / kSpecAllBlock, 1, B);
         switch (M) {
 #define VCT_SPEC_CASE(MM) \
-    case MM: hipLaunchKernelGGL((k_batch_spectral_all<MM>), grid, dim3(256), lds_all, s, a); break;
+    case MM: hipLaunchKernelGGL((k_batch_spectral_all<MM>), grid, dim3(kSpecAllBlock), lds_all, s, a); break;
             VCT_SPEC_CASE(1) VCT_SPEC_CASE(2) VCT_SPEC_CASE(3) VCT_SPEC_CASE(4)
             VCT_SPEC_CASE(5) VCT_SPEC_CASE(6) VCT_SPEC_CASE(7) VCT_SPEC_CASE(8)
 #undef VCT_SPEC_CASE
@@ -1773,12 +1779,12 @@ vct_status vct_batch_spectral_groups(vct_plan* p, const float* const* d_L, const
     a.rays = p->g.n_views * p->g.n_channels; a.W = p->g.n_channels; a.VT = p->VT;
     a.B = B; a.S = S; a.M = M; a.NE = NE;
     hipLaunchKernelGGL(k_batch_stats_init, dim3((B * S + 63) / 64), dim3(64), 0, s, d_stats, B * S);
-    const size_t lds_all = sizeof(float) * (size_t)S * NE * 9 + sizeof(int) * 4 * S;
+    const size_t lds_all = sizeof(float) * (size_t)S * NE * 9 + sizeof(int) * kSpecAllWaves * S;
     if (lds_all <= 64 * 1024) {
-        const dim3 grid((a.rays + 255) / 256, 1, B);
+        const dim3 grid((a.rays + kSpecAllBlock - 1) / kSpecAllBlock, 1, B);
         switch (M) {
 #define VCT_SPEC_CASE(MM) \
-    case MM: hipLaunchKernelGGL((k_batch_spectral_all<MM>), grid, dim3(256), lds_all, s, a); break;
+    case MM: hipLaunchKernelGGL((k_batch_spectral_all<MM>), grid, dim3(kSpecAllBlock), lds_all, s, a); break;
             VCT_SPEC_CASE(1) VCT_SPEC_CASE(2) VCT_SPEC_CASE(3) VCT_SPEC_CASE(4)
             VCT_SPEC_CASE(5) VCT_SPEC_CASE(6) VCT_SPEC_CASE(7) VCT_SPEC_CASE(8)
 #undef VCT_SPEC_CASE

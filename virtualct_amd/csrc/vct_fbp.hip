@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject(BackprojArgs 
 // (col: its offset within the row).
 struct BpView {
     double w, w0, rL2;
-    float wf, w0f, sf;  // FW: the weights in float, sf = 1/L^2 times the view step
+    float w1s, w0s;  // FW: the two interpolation weights times 1/L^2 and the view step, in float
     size_t off;
     int col;
     bool ok;
@@ -320,9 +320,9 @@ __device__ __forceinline__ BpView bp_view(const BackprojArgs& a, double xx, doub
     g.w = fGAM - GAM;
     g.w0 = 1 - g.w;
     if constexpr (FW) {
-        g.wf = (float)g.w;
-        g.w0f = (float)g.w0;
-        g.sf = __builtin_amdgcn_rcpf((float)L2) * (float)(a.dbeta_view * a.xfactor);
+        const float sf = __builtin_amdgcn_rcpf((float)L2) * (float)(a.dbeta_view * a.xfactor);
+        g.w1s = (float)g.w * sf;
+        g.w0s = (float)g.w0 * sf;
     } else {
         g.rL2 = rcp_f64(L2);
     }
@@ -388,7 +388,7 @@ __device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int 
             for (int k = 0; k < K; k++) {
                 float term;
                 if constexpr (FW) {
-                    term = g.sf * fmaf(g.wf, r1[k], g.w0f * r0[k]);
+                    term = fmaf(g.w1s, r1[k], g.w0s * r0[k]);
                 } else {
                     const double sum = g.w0 * r0[k] + g.w * r1[k];  // + 0.0 left out as above
                     term = (float)(g.rL2 * sum * a.dbeta_view * a.xfactor);
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(256) void k_backproject_sym(BackprojArgs a)
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const float term = g.sf * fmaf(g.wf, r1[k], g.w0f * r0[k]);
+                const float term = fmaf(g.w1s, r1[k], g.w0s * r0[k]);
                 acc[k] = g.ok ? acc[k] + term : acc[k];
             }
             g = gn;
