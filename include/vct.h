@@ -143,7 +143,7 @@ vct_status vct_plan_destroy(vct_plan* plan);
  * Launch shapes (results do not depend on them, beyond the order of float
  * adds where noted; an autotuner times them per shape):
  * VCT_OPT_TRACE_BLOCK = 6: lanes per workgroup of the tracer, 64, 128 or 256
- * (0: 256).
+ * (0: 64).
  * VCT_OPT_BP_SINOS_PER_LANE = 7: most sinograms one lane backprojects in a
  * multi-sinogram call, 2..16 (0: 16); images are bit-identical for every value.
  * VCT_OPT_ADJOINT_VIEWS = 8: views per workgroup of the adjoint, 1..256 (0: a

@@ -104,7 +104,7 @@ TraceArgs make_trace_args(const vct_plan* p)
     a.ray_offset = 0;
     a.n_rays = g.n_views * g.n_channels;
     a.exact = p->exact_trace;
-    a.block = p->trace_block > 0 ? p->trace_block : 256;
+    a.block = p->trace_block > 0 ? p->trace_block : 64;
     a.adj_views = p->adj_views;
     a.dens_t = p->d_dens_t;
     // orbits of the half or quarter turn: every view traced (V even), a grid

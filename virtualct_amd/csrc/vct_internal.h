@@ -179,7 +179,7 @@ struct vct_plan {
     int max_steps = 0;    // VCT_OPT_MAX_STEPS (0: the automatic bound)
     int exact_bp = 0;     // VCT_OPT_EXACT_BACKPROJECT
     int trace_orbit = 4;  // VCT_OPT_TRACE_ORBIT
-    int trace_block = 0;  // VCT_OPT_TRACE_BLOCK (0: 256)
+    int trace_block = 0;  // VCT_OPT_TRACE_BLOCK (0: 64)
     int bp_max_k = 0;     // VCT_OPT_BP_SINOS_PER_LANE (0: the kernel's limit)
     int adj_views = 0;    // VCT_OPT_ADJOINT_VIEWS (0: automatic)
 };
