@@ -309,28 +309,34 @@ def test_orbit_trace_equals_single_ray_march(gpu, golden, N, pix, V):
     """The default tracer marches the ray of view v once for the rays of views
     v + kV/4 (V a multiple of 4; 162 views take half turns): its segment
     counters equal marching every ray by itself on every ray, and the path
-    integrals agree to 1e-7 (the orbit shares the base ray's lengths), for
+    integrals agree to a float ulp (the orbit shares the base ray's lengths), for
     integer and float densities.  The orbit tracer gathers from a transposed
     copy of the densities for rays along x (N = 100 leaves partial 32 x 32
     tiles in the transpose; at N = 16 the padded u16 copies are larger than
-    the 32-byte transposed one)."""
+    the 32-byte transposed one), and on random densities in every voxel."""
     import torch
     ops = _ops()
     from virtualct_amd import _lib, phantoms
     ph = phantoms.body_phantom(17, N=N, pix_cm=pix)
+    # every voxel nonzero, the image border included: a turned ray's index -1
+    # (truncated to voxel 0) and N - 1 read real densities
+    rnd = np.random.default_rng(N).integers(1, 2000, size=ph.shape, dtype=np.int16)
+    cases = (("u16", _t(ph, gpu)), ("f32", _t(phantoms.densities_f32(ph), gpu)), ("u16 random", _t(rnd, gpu)))
     out = {}
     for orbit in (4, 2, 1):
         plan = ops.Plan(_geom(N, pix, V))
         plan.set_option(_lib.OPT_TRACE_ORBIT, orbit)
-        for name, d in (("u16", _t(ph, gpu)), ("f32", _t(phantoms.densities_f32(ph), gpu))):
+        for name, d in cases:
             pk, dt = ops.pack_densities(plan, d)
             L, cnt = ops.trace_paths(plan, pk, dt, 6, counts=True)
             out[(orbit, name)] = (L.cpu().numpy(), cnt.cpu().numpy())
         plan.close()
-    for name in ("u16", "f32"):
+    for name, _ in cases:
         L1, c1 = out[(1, name)]
         assert c1.sum() > 0
         for orbit in (4, 2):
             L, c = out[(orbit, name)]
             np.testing.assert_array_equal(c, c1, err_msg=f"orbit {orbit} {name}")
-            assert rel_max(L, L1) < 1e-7, (orbit, name)
+            # float32 outputs of FP64 sums that differ in the last bits: at
+            # most one float ulp apart (1.2e-7 of the largest value)
+            assert rel_max(L, L1) < 2.5e-7, (orbit, name)

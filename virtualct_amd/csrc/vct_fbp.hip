@@ -311,7 +311,17 @@ __device__ __forceinline__ BpView bp_view(const BackprojArgs& a, double xx, doub
     const double Y = -xx * sn + yy * c;
     const double den = a.sid - Y;
     const double L2 = den * den + X * X;
-    const double tg = X * rcp_f64(den);
+    // FW: one Newton step on the reciprocal estimate (within ~11 ulp of the
+    // quotient, a 1e-12 channel shift); the exact form takes two, which
+    // reproduce the IEEE quotient
+    double rden;
+    if constexpr (FW) {
+        const double r = __builtin_amdgcn_rcp(den);
+        rden = __builtin_fma(r, __builtin_fma(-den, r, 1.0), r);
+    } else {
+        rden = rcp_f64(den);
+    }
+    const double tg = X * rden;
     double gam = FW ? atan_sym(tg) : atan_small(tg);
     if (WIDE && fabs(tg) > 0.5) gam = atan(X / den);
     const double fGAM = gam * a.inv_dbeta_ch + a.cent;
