@@ -35,13 +35,13 @@ MFMA_F32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 SEGMENTS_512 = 282_824_404  # sum of per-ray segments at 512^2, 0.1 cm, 800x609
 # HBM bytes per tracer launch on this workload, from rocprofv3 --pmc passes
 # over `bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-batched
-# --no-batch-fbp --copy-probe 256`: FETCH_SIZE 549658.3 KiB x 2.00 + WRITE_SIZE 15227.0 KiB
+# --no-batch-fbp --copy-probe 256`: FETCH_SIZE 724934.3 KiB x 2.00 + WRITE_SIZE 15226.8 KiB
 # x 1.00, the factors being what the same counters read for the 256 MiB
-# device copy of the same run (131086.6 KiB fetched, 262144 KiB written).
+# device copy of the same run (131087.4 KiB fetched, 262144 KiB written).
 # The gathers read two 3.2 MB copies of the densities ([y][x] and [x][y]),
 # which overflow each XCD's 4 MB L2; the write is the 15.6 MB of path
 # integrals.
-TRACE_TRAFFIC_BYTES = int((549658.3 * 2.0 + 15227.0 * 1.0) * 1024)
+TRACE_TRAFFIC_BYTES = int((724934.3 * 2.0 + 15226.8 * 1.0) * 1024)
 METRIC = "MBFPA ray·energy-bins/s, FBP voxel·views/s; batched grad steps/s vs plain"
 
 
@@ -441,8 +441,8 @@ def main():
             "algorithmic_bytes_per_launch": trace_bytes,
             "note": "algorithmic bytes = segments x 6 materials x 2 B (u16 densities) + V x W x 4 B; the "
                     "traffic is what the L2s fetch and write (the two 3.2 MB density copies together exceed each "
-                    "XCD's 4 MB L2); the kernel is bound by its density gathers' texture path (TA / TD 78% / 86% "
-                    "busy) and FP64 VALU issue (48 VALU instructions per segment, 71% VALU busy)",
+                    "XCD's 4 MB L2); the kernel is bound by its density gathers' texture path (TA / TD 84% / 90% "
+                    "busy) and FP64 VALU issue (49 VALU instructions per segment, 75% VALU busy)",
         },
         "rooflines": [
             {"kernel": "k_trace_fwd_rot (tracer, quarter-turn orbits)", "bound": "hbm", "unit": "GB/s", "achieved": achieved,
@@ -456,8 +456,8 @@ def main():
              "per_launch": "P_mask x V x 8 B + N^2 x 4 B = %.3f GB" % (bp_bytes / 1e9),
              "note": "the taps come from the 1.95 MB filtered sinogram held in L2 (16.6 MB of traffic "
                      "past the L2s per launch), so the algorithmic rate can exceed the HBM peak; the "
-                     "kernel is bound by VALU issue (76% busy, 20 VALU instructions per masked "
-                     "pixel-view) and the texture data path (80% busy)"},
+                     "kernel is bound by VALU issue (76% busy, 19 VALU instructions per masked "
+                     "pixel-view) and the texture data path (81% busy)"},
         ],
         "cpu_baseline": None,
     }
