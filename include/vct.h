@@ -195,9 +195,10 @@ vct_status vct_pack_densities_f32(vct_plan* plan, const float* d_dens, int M,
  * is traced once, independent of the spectrum.  d_counts (optional, int32
  * [V][W]) receives the reference's per-ray segment counter
  * (reprojection_sum, :195/:280).  With orbits (VCT_OPT_TRACE_ORBIT), the
- * call first writes d_packed transposed into a work buffer that the plan
- * allocates with itself (N*N*32 bytes, or 24*(N+4)^2 if larger, when N and
- * V are even), so that the gathers of rays along x read rows too. */
+ * call first writes d_packed and its transpose, each padded by two voxels,
+ * into a work buffer that the plan allocates with itself (64*(N+4)^2 bytes
+ * when N and V are even), so that the gathers of rays along x read rows
+ * too. */
 vct_status vct_trace_paths(vct_plan* plan, const void* d_packed, int dens_type, int M,
                            float* d_L, int32_t* d_counts, void* stream);
 

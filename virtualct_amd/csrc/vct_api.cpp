@@ -384,13 +384,12 @@ vct_status vct_plan_create(vct_ctx* ctx, const vct_geometry* g, vct_plan** out)
     if (st == VCT_OK) st = plan_upload(px0.data(), W * sizeof(double), (void**)&p->d_px0);
     if (st == VCT_OK) st = plan_upload(py0.data(), W * sizeof(double), (void**)&p->d_py0);
     if (st == VCT_OK) st = plan_upload(coeff.data(), W * sizeof(double), (void**)&p->d_coeff);
-    // the orbit tracer's work copy (any dens type), allocated with the plan
-    // so that no trace call allocates
+    // the orbit tracer's work copies (any dens type), allocated with the
+    // plan so that no trace call allocates
     if (st == VCT_OK && g->recon_size % 2 == 0 && V % 2 == 0) {
-        // the transposed copy of 32-byte voxels, or the two padded copies of
-        // 12-byte voxels ((N + 4)^2 each), whichever is larger
-        const size_t n = (size_t)g->recon_size, np = n + 4;
-        const size_t bytes = std::max(n * n * 8 * sizeof(float), 2 * np * np * 12);
+        // two padded copies of (N + 4)^2 voxels of up to 32 bytes
+        const size_t np = (size_t)g->recon_size + 4;
+        const size_t bytes = 2 * np * np * 32;
         if (hipMalloc(&p->d_dens_t, bytes) != hipSuccess) {
             (void)hipGetLastError();
             st = set_error(VCT_ERR_NOMEM, "plan work copy of %zu bytes", bytes);

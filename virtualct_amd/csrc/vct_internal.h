@@ -42,8 +42,7 @@ struct TraceArgs {
     const double* px0;    // [W] detector point before rotation (x)
     const double* py0;    // [W] detector point before rotation (y)
     const void* dens;     // packed [N][N][8] (u16 or f32)
-    void* dens_t;         // orbit tracer: work copies of dens (the transposed [x][y] one; u16 with
-                          // M <= 6: [y][x] and [x][y] copies of 12-byte voxels padded to N + 4)
+    void* dens_t;         // orbit tracer: work copies of dens, [y][x] and [x][y], padded to N + 4
     float* L;             // [V][W][8] path integrals (forward)
     int32_t* counts;      // [V][W] optional segment counter
     const float* wts;     // [V][W][8] adjoint weights (adjoint)

@@ -164,43 +164,36 @@ struct SinkRot {
     static constexpr bool kAdd2 = true;
     static constexpr bool kRot = true;
     static constexpr int NW = (M + 1) / 2;  // u16: 32-bit words of two materials
-    // u16 with M <= 6: both copies are the plan's, of 12-byte voxels (the
-    // three words that hold the materials), so the two fit the L2 better,
-    // and padded by two voxels on every side: row and column -1 repeat row
-    // and column 0 (the reference truncates index -1 to voxel 0), rows and
-    // columns -2, N and N + 1 are zero, so the committed steps' gathers need
-    // no in-image test (a voxel outside the image reads zeros)
-    static constexpr bool k12 = DT == VCT_DENS_U16 && M <= 6;
+    // Bytes per voxel of the work copies: u16 with M <= 6 keeps the three
+    // words that hold the materials, other u16 all four, f32 eight floats.
+    static constexpr int VB = DT == VCT_DENS_U16 ? (M <= 6 ? 12 : 16) : 32;
     // Gathers read the copy of the densities whose rows run along the
     // minor axis of the slot's ray ([y][x] voxels when y is its major axis,
     // the transposed [x][y] copy when x is), so the lanes of a wave, which
     // hold neighbouring rays at the same major step, meet neighbouring voxels
     // of one row.  Slot k holds ray k (the base ray turned k quarter turns),
     // except that rays 1 and 3 trade slots when y is the base ray's major
-    // axis; finish() puts them back.
+    // axis; finish() puts them back.  Both copies are padded by two voxels
+    // on every side (NP = N + 4 per row): row and column -1 repeat row and
+    // column 0 (the reference truncates index -1 to voxel 0), rows and
+    // columns -2, N and N + 1 are zero, so the committed steps' gathers need
+    // no in-image test (a voxel outside the image reads zeros).
     const char* __restrict__ voxE;  // slots 0 and 2 (and 1 when R == 2)
     const char* __restrict__ voxO;  // slots 1 and 3 when R == 4
     bool xmajor;
     int N;
-    int NP;   // row pitch of the copies in voxels: N, or N + 4 when padded
-    int ORG;  // index of voxel (0, 0): 0, or 2 NP + 2 when padded
+    int NP;   // row pitch of the copies in voxels, N + 4
+    int ORG;  // index of voxel (0, 0), 2 NP + 2
     bool unsafe;
     double accw[R][NW], acch[R][M / 2];  // u16: word sums and high halves
     double acc[R][M];                    // f32: sums; u16: after finish()
-    // p: the packed [y][x] voxels, pt: the same transposed ([x][y]); with
-    // k12, pt holds both 12-byte copies, [y][x] then [x][y]
-    __host__ __device__ void init(const void* p, const void* pt, bool xm, int n)
+    // pt: the two padded copies, [y][x] then [x][y]
+    __host__ __device__ void init(const void* pt, bool xm, int n)
     {
-        const char* P = reinterpret_cast<const char*>(p);
-        const char* T = reinterpret_cast<const char*>(pt);
-        NP = n;
-        ORG = 0;
-        if constexpr (k12) {
-            NP = n + 4;
-            ORG = 2 * NP + 2;
-            P = T;
-            T = T + (size_t)NP * NP * 12;
-        }
+        NP = n + 4;
+        ORG = 2 * NP + 2;
+        const char* P = reinterpret_cast<const char*>(pt);
+        const char* T = P + (size_t)NP * NP * VB;
         voxE = xm ? T : P;
         voxO = xm ? P : T;
         xmajor = xm;
@@ -218,7 +211,7 @@ struct SinkRot {
     }
     // slot k of a segment whose floors are (fa, fb) on the base ray's (major,
     // minor) axes: the (major, minor) coordinates of the turned ray, indexed
-    // minor + N * major in its copy, with the reference's truncation of
+    // minor + NP * major in its copy, with the reference's truncation of
     // index -1 to voxel 0 (0 with rr = 0 when outside the image)
     __host__ __device__ __forceinline__ void voxels(int fa, int fb, double rr, int (&v)[R], double (&r)[R]) const
     {
@@ -239,24 +232,26 @@ struct SinkRot {
     struct Vox {
         uint4 lo, hi;  // u16: lo = the 8 materials; f32: lo = materials 0-3, hi = 4-7
     };
-    __host__ __device__ __forceinline__ Vox load(int k, int v) const
+    // the voxel at byte offset o of the copy of slot k
+    __host__ __device__ __forceinline__ Vox load_at(int k, uint32_t o) const
     {
-        Vox q;
         const char* vox = (R == 4 && (k & 1)) ? voxO : voxE;
-        if constexpr (k12) {
-            const uint3 t = *reinterpret_cast<const uint3*>(vox + (uint32_t)v * 12u);
+        Vox q;
+        if constexpr (VB == 12) {
+            const uint3 t = *reinterpret_cast<const uint3*>(vox + o);
             q.lo = make_uint4(t.x, t.y, t.z, 0u);
             q.hi = make_uint4(0u, 0u, 0u, 0u);
-        } else if constexpr (DT == VCT_DENS_U16) {
-            q.lo = *reinterpret_cast<const uint4*>(vox + (uint32_t)v * 16u);
+        } else if constexpr (VB == 16) {
+            q.lo = *reinterpret_cast<const uint4*>(vox + o);
             q.hi = make_uint4(0u, 0u, 0u, 0u);
         } else {
-            const uint4* f = reinterpret_cast<const uint4*>(vox + (size_t)(uint32_t)v * 32u);
+            const uint4* f = reinterpret_cast<const uint4*>(vox + o);
             q.lo = f[0];
             q.hi = M > 4 ? f[1] : make_uint4(0u, 0u, 0u, 0u);
         }
         return q;
     }
+    __host__ __device__ __forceinline__ Vox load(int k, int v) const { return load_at(k, (uint32_t)v * (uint32_t)VB); }
     __host__ __device__ __forceinline__ void accumulate(int k, const Vox& q, double rr)
     {
         if constexpr (DT == VCT_DENS_U16) {
@@ -328,38 +323,14 @@ struct SinkRot {
             r[slot] = in ? rr : 0.0;
         }
     }
-    // slot indices and in-image flags of one candidate segment (the index of
-    // a slot outside the image is not used)
-    __host__ __device__ __forceinline__ void slots(int fa, int fb, int (&v)[R], bool (&in)[R]) const
-    {
-        const int n1 = N - 1;
-        const int ta = n1 - fa, tb = n1 - fb;
-        #pragma unroll
-        for (int k = 0; k < R; k++) {
-            int ma, mi;
-            if (R == 1 || k == 0) { ma = fa; mi = fb; }
-            else if (R == 2 || k == 2) { ma = ta; mi = tb; }
-            else if (k == 1) { ma = fa; mi = tb; }
-            else { ma = ta; mi = fb; }
-            in[k] = (unsigned)(ma + 1) <= (unsigned)N && (unsigned)(mi + 1) <= (unsigned)N;
-            v[k] = max(mi, 0) + mul24(max(ma, 0), N);
-        }
-    }
-    // the two candidate segments of a committed step; the first only on lanes
-    // whose step crosses a minor grid line (cross), so lanes without a
-    // crossing issue no gathers for it (the gathers' texture path is what the
-    // kernel keeps busiest); a segment of length 0 would add exactly nothing.
-    // u16: slots outside the image neither gather nor add (the two lengths
-    // stay shared by the four slots, which saves registers and selects);
-    // f32 keeps zeroed lengths, which compiles to fewer branches there.
-    // padded copies: byte offsets of the four slots of one candidate,
-    // coordinates clamped to [-2, N + 1] (values outside read zeros there too)
-    __host__ __device__ __forceinline__ void offsets12(int fa, int fb, uint32_t (&o)[R]) const
+    // byte offsets of the four slots of one candidate, its coordinates
+    // clamped to [-2, N + 1] (values outside read zeros there too)
+    __host__ __device__ __forceinline__ void offsets(int fa, int fb, uint32_t (&o)[R]) const
     {
         const int a = min(max(fa + 2, 0), N + 3), b = min(max(fb + 2, 0), N + 3);
         const int ta = N + 3 - a, tb = N + 3 - b;  // (N - 1 - f) + 2
-        const uint32_t ra = (uint32_t)mul24(a, 12 * NP), rt = (uint32_t)mul24(ta, 12 * NP);
-        const uint32_t cb = (uint32_t)mul24(b, 12), ct = (uint32_t)mul24(tb, 12);
+        const uint32_t ra = (uint32_t)mul24(a, VB * NP), rt = (uint32_t)mul24(ta, VB * NP);
+        const uint32_t cb = (uint32_t)mul24(b, VB), ct = (uint32_t)mul24(tb, VB);
         #pragma unroll
         for (int k = 0; k < R; k++) {
             if (R == 1 || k == 0) o[k] = ra + cb;
@@ -368,76 +339,29 @@ struct SinkRot {
             else o[k] = rt + cb;
         }
     }
-    __host__ __device__ __forceinline__ Vox load12(int k, uint32_t o) const
-    {
-        const char* vox = (R == 4 && (k & 1)) ? voxO : voxE;
-        const uint3 t = *reinterpret_cast<const uint3*>(vox + o);
-        Vox q;
-        q.lo = make_uint4(t.x, t.y, t.z, 0u);
-        q.hi = make_uint4(0u, 0u, 0u, 0u);
-        return q;
-    }
+    // the two candidate segments of a committed step; the first only on lanes
+    // whose step crosses a minor grid line (cross), so lanes without a
+    // crossing issue no gathers for it (the gathers' texture path is what the
+    // kernel keeps busiest); a segment of length 0 would add exactly nothing
     __host__ __device__ __forceinline__ void add2_fl(bool cross, int fa1, int fb1, double r1, int fa2, int fb2,
                                                      double r2)
     {
-        if constexpr (k12) {
-            uint32_t o1[R], o2[R];
-            offsets12(fa1, fb1, o1);
-            offsets12(fa2, fb2, o2);
-            Vox q1[R], q2[R];
-            if (cross) {
-                #pragma unroll
-                for (int k = 0; k < R; k++) q1[k] = load12(k, o1[k]);
-            }
-            #pragma unroll
-            for (int k = 0; k < R; k++) q2[k] = load12(k, o2[k]);
-            if (cross) {
-                #pragma unroll
-                for (int k = 0; k < R; k++) accumulate(k, q1[k], r1);
-            }
-            #pragma unroll
-            for (int k = 0; k < R; k++) accumulate(k, q2[k], r2);
-            return;
-        }
-        if constexpr (DT == VCT_DENS_U16) {
-            int v1[R], v2[R];
-            bool i1[R], i2[R];
-            slots(fa1, fb1, v1, i1);
-            slots(fa2, fb2, v2, i2);
-            Vox q1[R], q2[R];
-            #pragma unroll
-            for (int k = 0; k < R; k++) {
-                i1[k] = i1[k] && cross;
-                if (i1[k]) q1[k] = load(k, v1[k]);
-            }
-            #pragma unroll
-            for (int k = 0; k < R; k++)
-                if (i2[k]) q2[k] = load(k, v2[k]);
-            #pragma unroll
-            for (int k = 0; k < R; k++)
-                if (i1[k]) accumulate(k, q1[k], r1);
-            #pragma unroll
-            for (int k = 0; k < R; k++)
-                if (i2[k]) accumulate(k, q2[k], r2);
-            return;
-        }
-        int v1[R], v2[R];
-        double q1r[R], q2r[R];
-        voxels(fa1, fb1, r1, v1, q1r);
-        voxels(fa2, fb2, r2, v2, q2r);
+        uint32_t o1[R], o2[R];
+        offsets(fa1, fb1, o1);
+        offsets(fa2, fb2, o2);
         Vox q1[R], q2[R];
         if (cross) {
             #pragma unroll
-            for (int k = 0; k < R; k++) q1[k] = load(k, v1[k]);
+            for (int k = 0; k < R; k++) q1[k] = load_at(k, o1[k]);
         }
         #pragma unroll
-        for (int k = 0; k < R; k++) q2[k] = load(k, v2[k]);
+        for (int k = 0; k < R; k++) q2[k] = load_at(k, o2[k]);
         if (cross) {
             #pragma unroll
-            for (int k = 0; k < R; k++) accumulate(k, q1[k], q1r[k]);
+            for (int k = 0; k < R; k++) accumulate(k, q1[k], r1);
         }
         #pragma unroll
-        for (int k = 0; k < R; k++) accumulate(k, q2[k], q2r[k]);
+        for (int k = 0; k < R; k++) accumulate(k, q2[k], r2);
     }
     __host__ __device__ __forceinline__ void finish()
     {
@@ -630,7 +554,7 @@ __global__ __launch_bounds__(256) void k_trace_fwd_rot(TraceArgs a)
         done = true;
     } else if (safe && r.dB != 0.0 && !a.rcheck && !a.exact) {
         SinkRot<M, DT, R> sink;
-        sink.init(a.dens, a.dens_t, r.xmajor, a.N);
+        sink.init(a.dens_t, r.xmajor, a.N);
         bool ovf = false;
         int k0 = 0, k1 = INT_MAX;
         if (P > 1) {
@@ -1051,54 +975,36 @@ __global__ __launch_bounds__(256) void k_pack_f32(const float* __restrict__ src,
     d[1] = make_float4(r[4], r[5], r[6], r[7]);
 }
 
-// out = in transposed, [N][N] voxels of CH 16-byte words each: out voxel
-// (x, y) = in voxel (y, x), by 32 x 32 voxel tiles through LDS.
-template <int CH>
-__global__ __launch_bounds__(256) void k_transpose_voxels(const uint4* __restrict__ in, uint4* __restrict__ out,
-                                                          int N)
-{
-    __shared__ uint4 tile[32][32 * CH + 1];
-    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int j = ty; j < 32; j += 8) {
-        const int y = y0 + j;
-#pragma unroll
-        for (int c = tx; c < 32 * CH; c += 32) {
-            const int x = x0 + c / CH;
-            if (y < N && x < N) tile[j][c] = in[((size_t)y * N + x) * CH + c % CH];
-        }
-    }
-    __syncthreads();
-    for (int i = ty; i < 32; i += 8) {
-        const int x = x0 + i;
-#pragma unroll
-        for (int c = tx; c < 32 * CH; c += 32) {
-            const int y = y0 + c / CH;
-            if (x < N && y < N) out[((size_t)x * N + y) * CH + c % CH] = tile[c / CH][i * CH + c % CH];
-        }
-    }
-}
+// One voxel of the tracer's work copies: the first VB bytes of a packed
+// voxel (VB = 12: the three words of u16 materials 0-5; 16: all four; 32:
+// eight floats).
+template <int VB>
+struct alignas(VB % 16 == 0 ? 16 : 4) WorkVox {
+    uint32_t w[VB / 4];
+};
 
-// Both 12-byte work copies of u16 densities (materials 0-5), padded by two
-// voxels on every side (NP = N + 4 per row): yx[y + 2][x + 2] and
-// xy[x + 2][y + 2] hold voxel (x, y) for x, y in [-1, N - 1], index -1
+// The tracer's two work copies of the packed densities (IB bytes per voxel),
+// padded by two voxels on every side (NP = N + 4 per row): yx[y + 2][x + 2]
+// and xy[x + 2][y + 2] hold voxel (x, y) for x, y in [-1, N - 1], index -1
 // reading voxel 0 as the reference's truncation does, and zeros elsewhere;
 // by 32 x 32 tiles of the padded grid through LDS.
-__global__ __launch_bounds__(256) void k_pack_copies12(const uint4* __restrict__ in, uint3* __restrict__ yx,
-                                                       uint3* __restrict__ xy, int N)
+template <int IB, int VB>
+__global__ __launch_bounds__(256) void k_pack_copies(const char* __restrict__ in, WorkVox<VB>* __restrict__ yx,
+                                                     WorkVox<VB>* __restrict__ xy, int N)
 {
-    __shared__ uint3 tile[32][33];
+    static_assert(VB % 4 == 0 && VB <= IB, "a work voxel is a prefix of the packed voxel");
+    __shared__ WorkVox<VB> tile[32][33];
     const int NP = N + 4;
     const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;  // padded coordinates
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int j = ty; j < 32; j += 8) {
         const int pr = r0 + j, pc = c0 + tx;
         const int y = pr - 2, x = pc - 2;
-        uint3 t = make_uint3(0u, 0u, 0u);
-        if (y >= -1 && y < N && x >= -1 && x < N) {
-            const uint4 q = in[(size_t)max(y, 0) * N + max(x, 0)];
-            t = make_uint3(q.x, q.y, q.z);
-        }
+        WorkVox<VB> t;
+#pragma unroll
+        for (int q = 0; q < VB / 4; q++) t.w[q] = 0u;
+        if (y >= -1 && y < N && x >= -1 && x < N)
+            t = *reinterpret_cast<const WorkVox<VB>*>(in + ((size_t)max(y, 0) * N + max(x, 0)) * IB);
         tile[j][tx] = t;
         if (pr < NP && pc < NP) yx[(size_t)pr * NP + pc] = t;
     }
@@ -1107,6 +1013,16 @@ __global__ __launch_bounds__(256) void k_pack_copies12(const uint4* __restrict__
         const int pr = c0 + i, pc = r0 + tx;  // row x + 2, column y + 2
         if (pr < NP && pc < NP) xy[(size_t)pr * NP + pc] = tile[tx][i];
     }
+}
+
+template <int IB, int VB>
+static void launch_pack_copies(const TraceArgs& a, hipStream_t s)
+{
+    const int NP = a.N + 4;
+    const dim3 grid((unsigned)((NP + 31) / 32), (unsigned)((NP + 31) / 32));
+    WorkVox<VB>* yx = reinterpret_cast<WorkVox<VB>*>(a.dens_t);
+    hipLaunchKernelGGL((k_pack_copies<IB, VB>), grid, dim3(256), 0, s, reinterpret_cast<const char*>(a.dens), yx,
+                       yx + (size_t)NP * NP, a.N);
 }
 
 // --------------------------------------------------------------- launchers
@@ -1133,22 +1049,13 @@ static vct_status launch_trace_m(const TraceArgs& a, int dens_type, bool adjoint
     if (!adjoint && whole && (a.rot == 4 || a.rot == 2)) {
         if (a.V % a.rot != 0 || a.N % 2 != 0 || a.VT != a.V || a.exact || a.rcheck || !a.dens_t)
             return set_error(VCT_ERR_INTERNAL, "orbit trace with V %d, N %d", a.V, a.N);
-        // the orbit gathers of rays along x read the transposed copy (u16 with
-        // M <= 6: both copies are rewritten with 12-byte voxels)
-        const dim3 tgrid((unsigned)((a.N + 31) / 32), (unsigned)((a.N + 31) / 32));
-        const int NP = a.N + 4;
-        const dim3 pgrid((unsigned)((NP + 31) / 32), (unsigned)((NP + 31) / 32));
-        if (dens_type == VCT_DENS_U16 && M <= 6)
-            hipLaunchKernelGGL(k_pack_copies12, pgrid, dim3(256), 0, s, reinterpret_cast<const uint4*>(a.dens),
-                               reinterpret_cast<uint3*>(a.dens_t),
-                               reinterpret_cast<uint3*>(reinterpret_cast<char*>(a.dens_t) + (size_t)NP * NP * 12),
-                               a.N);
-        else if (dens_type == VCT_DENS_U16)
-            hipLaunchKernelGGL((k_transpose_voxels<1>), tgrid, dim3(256), 0, s,
-                               reinterpret_cast<const uint4*>(a.dens), reinterpret_cast<uint4*>(a.dens_t), a.N);
-        else
-            hipLaunchKernelGGL((k_transpose_voxels<2>), tgrid, dim3(256), 0, s,
-                               reinterpret_cast<const uint4*>(a.dens), reinterpret_cast<uint4*>(a.dens_t), a.N);
+        // the orbit gathers read two padded work copies, [y][x] and [x][y]
+        if (dens_type == VCT_DENS_U16) {
+            if constexpr (M <= 6) launch_pack_copies<16, 12>(a, s);
+            else launch_pack_copies<16, 16>(a, s);
+        } else {
+            launch_pack_copies<32, 32>(a, s);
+        }
         if (dens_type == VCT_DENS_U16) {
             if (a.rot == 4) launch_trace_rot<M, VCT_DENS_U16, 4>(a, s);
             else launch_trace_rot<M, VCT_DENS_U16, 2>(a, s);
