@@ -386,8 +386,8 @@ __host__ __device__ __forceinline__ int march_loop(const TraceArgs& a, const Ray
                         if (v2) sink.add_ab(ia2, ib2, fabs(na - s_a) * sec);
                     } else if constexpr (kRot) {
                         // every candidate segment with the floors of its index values; the
-                        // sink applies each ray's own in-image test (no crossing: the first
-                        // candidate is skipped)
+                        // sink turns them into each ray's voxel, a voxel outside the image
+                        // reading zeros (no crossing: the first candidate is skipped)
                         sink.add2_fl(cross, fa1, fb1, fabs(xa - p0.a) * sec, fa2, fb2, fabs(na - s_a) * sec);
                     } else if constexpr (Sink::kAdd2) {
                         // (a voxel index is only used when its segment is in the image)
