@@ -35,13 +35,13 @@ MFMA_F32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 SEGMENTS_512 = 282_824_404  # sum of per-ray segments at 512^2, 0.1 cm, 800x609
 # HBM bytes per tracer launch on this workload, from rocprofv3 --pmc passes
 # over `bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-batched
-# --no-batch-fbp --copy-probe 256`: FETCH_SIZE 724934.3 KiB x 2.00 + WRITE_SIZE 15226.8 KiB
+# --no-batch-fbp --copy-probe 256`: FETCH_SIZE 722793.1 KiB x 2.00 + WRITE_SIZE 15226.6 KiB
 # x 1.00, the factors being what the same counters read for the 256 MiB
-# device copy of the same run (131087.4 KiB fetched, 262144 KiB written).
+# device copy of the same run (131087.9 KiB fetched, 262144 KiB written).
 # The gathers read two 3.2 MB copies of the densities ([y][x] and [x][y]),
 # which overflow each XCD's 4 MB L2; the write is the 15.6 MB of path
 # integrals.
-TRACE_TRAFFIC_BYTES = int((724934.3 * 2.0 + 15226.8 * 1.0) * 1024)
+TRACE_TRAFFIC_BYTES = int((722793.1 * 2.0 + 15226.6 * 1.0) * 1024)
 METRIC = "MBFPA ray·energy-bins/s, FBP voxel·views/s; batched grad steps/s vs plain"
 
 
