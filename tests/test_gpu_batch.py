@@ -485,6 +485,43 @@ def test_tile_stage1_equals_blocked_stage1(gpu, N, pix, B, M):
     smn.close()
 
 
+@pytest.mark.parametrize("N,pix,B,M,built", [(128, 0.1, 3, 6, True), (96, 0.2, 16, 5, True), (160, 0.1, 2, 8, True),
+                                           (65, 0.15, 5, 1, True), (64, 0.1, 64, 2, True), (128, 0.35, 4, 6, False)])
+def test_adjoint_lds_stage1_equals_blocked_stage1(gpu, N, pix, B, M, built):
+    """The adjoint stage 1 over its LDS-resident form (blocks of 50 views x
+    one part of a 64x64 tile, the weights of the rays that cross the part in
+    LDS, runs sorted by length in 64-run slices) gives the bits of the
+    blocked stage 1 read through L2, for 2..64 phantoms and 1..8 materials,
+    partial edge tiles at 65^2 and 160^2, tiles split in 1..16 parts (96^2 x
+    0.2 cm).  At 128^2 x 0.35 cm a 16x16 cell sees more than 3328 rays of a
+    view block, so the form is not built and the L2 path runs."""
+    import torch
+    ops, batch, _ = _mods()
+    plan = ops.plan_for(_geom(N, pix, 160))
+    smt = batch.SystemMatrix(plan, transpose=True, blocked=True)
+    smn = batch.SystemMatrix(plan, transpose=True, blocked=True, tiles=False)
+    smf = batch.SystemMatrix(plan, transpose=False, blocked=True)
+    smfn = batch.SystemMatrix(plan, transpose=False, blocked=True, tiles=False)
+    # the adjoint form adds to the bytes of the forward one when built
+    assert (smt.bytes - smn.bytes > smf.bytes - smfn.bytes) == built
+    smf.close()
+    smfn.close()
+    rays = plan.V * plan.W
+    g = torch.Generator(device=gpu).manual_seed(4100 + N)
+    w = torch.zeros((rays, B, 8), device=gpu)
+    w[..., :M] = torch.randn((rays, B, M), generator=g, device=gpu)
+    w = batch.to_planes(w)
+    gt, gn = torch.empty((N * N, B, 8), device=gpu), torch.empty((N * N, B, 8), device=gpu)
+    smt.workspace(B, 0).zero_()
+    smn.workspace(B, 0).zero_()
+    batch.batch_adjoint([(smt, w, gt, B)], M)
+    batch.batch_adjoint([(smn, w, gn, B)], M)
+    assert torch.equal(gt, gn)
+    assert torch.equal(smt.workspace(B, 0), smn.workspace(B, 0))
+    smt.close()
+    smn.close()
+
+
 def test_sysmat_rows_equal_reference_segment_lists(gpu, golden):
     """The cached rows of the 56 rays whose segment caches the reference
     dumped at the full 512^2 / 800 x 609 geometry (tests/golden/ref_segments.npz:

@@ -29,13 +29,18 @@
 // it; stage 2 sums each row's partials in block order.  Both stages are
 // ordinary k_gather launches and the result is deterministic.
 //
-// Tile-resident forward stage 1 (built with the blocked form unless
-// VCT_SYSMAT_NO_TILE).  The runs of each forward tile, sorted by length into
+// LDS-resident stage 1 (built with the blocked form unless
+// VCT_SYSMAT_NO_TILE).  The runs of each LDS block, sorted by length into
 // slices of 64 with interleaved entries, are summed by k_gather_tile, one
-// workgroup per (tile, phantom) with the tile of that phantom in LDS: the
-// vector memory path then carries only the 8-byte entries, read once from
-// HBM by the B workgroups of a tile on one XCD.  Same runs, same order of
-// terms: the partial sums are those of the k_gather stage 1, bit for bit.
+// workgroup per (block, phantom) with what the block reads of that phantom
+// in LDS: the vector memory path then carries only the 8-byte entries, read
+// once from HBM by the B workgroups of a block on one XCD.
+//   forward: a block is a 64x64 tile, its densities in LDS;
+//   adjoint: a block is one part of a 64x64 tile (1 to 16 parts, the
+//     coarsest split whose tables fit) in one 50-view block, the weights of
+//     the rays of those views that cross the part in LDS.
+// Same runs, same order of terms: the partial sums are those of the k_gather
+// stage 1, bit for bit.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -55,12 +60,18 @@ struct vct_blocked {
     uint2* ent2 = nullptr;        // [rows1] (stage-1 row, 1.0f), block order per row
 };
 
-// Tile-resident form of the forward stage 1 (built with the blocked form
-// unless VCT_SYSMAT_NO_TILE): the stage-1 runs of each forward tile sorted by
-// length (longest first) in slices of 64, whose entries are interleaved so
-// that lane l of a wave reads entry k of run l at slice_ptr + 64 k + l.  An
-// entry holds its voxel's index inside the tile (ly * 64 + lx) and the
-// length; padding entries are (kTilePadVoxel, 0.0f), which read zeroed LDS.
+// LDS-resident form of a stage 1 (built with the blocked form unless
+// VCT_SYSMAT_NO_TILE): the stage-1 runs of each LDS block sorted by length
+// (longest first) in slices of 64, whose entries are interleaved so that lane
+// l of a wave reads entry k of run l at slice_ptr + 64 k + l.
+//   forward: a block is a 64x64 image tile; an entry holds its voxel's index
+//     inside the tile (ly * 64 + lx) and the length; padding entries are
+//     (kTilePadVoxel, 0.0f), which read zeroed LDS;
+//   adjoint: a block is (view block, part of a 64x64 image tile), whose LDS
+//     table holds, view after view, the rays of the view block that cross
+//     the part (the channel range [clo, clo + n) of each view); an entry
+//     holds its ray's slot in the table; slot 0 holds zeros and pads
+//     (0, 0.0f).
 struct vct_tiled {
     int ntiles = 0, nslices = 0, kmax = 0, ntx = 0;
     uint2* ell = nullptr;            // [sum slice_len * 64]
@@ -70,6 +81,11 @@ struct vct_tiled {
     int32_t* tile_slices = nullptr;  // [ntiles + 1]
     int32_t* xcd_tiles = nullptr;    // [8 * kmax] tiles of each XCD, -1: none
     size_t n_ell = 0;
+    // adjoint blocks only (slot_off == nullptr: forward)
+    int32_t* slot_off = nullptr;     // [ntiles * (kAdjViews + 1)] first slot of each view, then the table size
+    int32_t* slot_clo = nullptr;     // [ntiles * kAdjViews] first channel of each view
+    int32_t* blk_v0 = nullptr;       // [ntiles] first view of the block's view block
+    int slots_max = 0;               // largest table, zero slot included
 };
 
 struct vct_sysmat {
@@ -83,7 +99,8 @@ struct vct_sysmat {
     uint32_t* col_ptr = nullptr;  // [NN+1]
     uint2* csc = nullptr;         // [nseg] (ray, float bits of length)
     vct_blocked fwd, adj;         // blocked forms (rows1 == 0: not built)
-    vct_tiled tile;               // tile-resident forward stage 1 (ell == nullptr: not built)
+    vct_tiled tile;               // LDS-resident forward stage 1 (ell == nullptr: not built)
+    vct_tiled atile;              // LDS-resident adjoint stage 1 (ell == nullptr: not built)
 };
 
 namespace vct {
@@ -229,7 +246,8 @@ __global__ __launch_bounds__(256) void k_stage1_fill(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ run, const uint2* __restrict__ ent,
                                                      const uint32_t* __restrict__ row_of, int64_t n,
                                                      uint2* __restrict__ ent1, uint32_t* __restrict__ ptr1,
-                                                     uint32_t* __restrict__ row1)
+                                                     uint32_t* __restrict__ row1, const uint32_t* __restrict__ key_s,
+                                                     uint32_t* __restrict__ key1)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -239,6 +257,7 @@ __global__ __launch_bounds__(256) void k_stage1_fill(const uint32_t* __restrict_
         const uint32_t p = run[i];
         ptr1[p] = (uint32_t)i;
         row1[p] = row_of[e];
+        if (key1) key1[p] = key_s[i];
     }
 }
 
@@ -958,10 +977,22 @@ constexpr int kTileUnroll = 8;      // entries in flight per lane
 // or NaN density included).
 constexpr uint32_t kTilePadVoxel = 64u * 64u;
 
-__global__ __launch_bounds__(256) void k_fill_pad(uint2* __restrict__ p, size_t n)
+// Adjoint blocks.  A block is one part of a 64x64 tile in one view block:
+// the build splits each (view block, tile) into 1, 2, 4, 8 or 16 parts on a
+// 4x4 grid of 16x16 cells, the coarsest split whose ray tables all hold at
+// most kAdjSlotCap slots (slot 0, the zero slot of the padding entries,
+// included).  A table of S slots takes S * MP floats of LDS: at M <= 6 two
+// 512-lane workgroups fit one CU, so one loads its table while the other
+// sums.
+constexpr int kAtileThreads = 512;
+constexpr int kAdjSlotCap = 3328;       // 78 KB at MP = 6
+constexpr int kAdjCells = 16;           // 16x16 cells of a 64x64 tile
+constexpr int kAdjPadSlot = 0;
+
+__global__ __launch_bounds__(256) void k_fill_pad(uint2* __restrict__ p, size_t n, uint32_t pad)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = make_uint2(kTilePadVoxel, 0u);
+    if (i < n) p[i] = make_uint2(pad, 0u);
 }
 
 // Build: per stage-1 run, sort key tile << 16 | (0xffff - length) (longest
@@ -1036,8 +1067,129 @@ __global__ __launch_bounds__(256) void k_tile_fill(const uint32_t* __restrict__ 
     }
 }
 
+// Adjoint build: the first stage-1 run of each blocked-form key (view block
+// * tiles + rank of the 64x64 tile; the runs are in key order).
+__global__ __launch_bounds__(256) void k_key_first(const uint32_t* __restrict__ key1, uint32_t rows1, uint32_t nkeys,
+                                                   uint32_t* __restrict__ first, uint32_t* __restrict__ err)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows1) return;
+    const uint32_t k = key1[r], kp = r ? key1[r - 1] : 0xffffffffu;
+    if (k >= nkeys || (r && k < kp)) {
+        atomicOr(err, 1u);
+        return;
+    }
+    if (k != kp) first[k] = r;
+}
+
+__device__ __forceinline__ uint32_t adj_cell(uint32_t v, int N)
+{
+    const uint32_t iy = v / (uint32_t)N, ix = v - iy * (uint32_t)N;
+    return ((iy & 63u) >> 4) * 4u + ((ix & 63u) >> 4);
+}
+
+// Adjoint build: first and last channel of each view in each 16x16 cell of a
+// blocked-form key, over the entries of its runs; one workgroup per key.
+__global__ __launch_bounds__(256) void k_atile_range(const uint32_t* __restrict__ ptr1, const uint2* __restrict__ ent1,
+                                                     const uint32_t* __restrict__ row1,
+                                                     const uint32_t* __restrict__ kbegin,
+                                                     const uint32_t* __restrict__ kcount, int N, int W, int ntiles64,
+                                                     int32_t* __restrict__ lo_out, int32_t* __restrict__ hi_out,
+                                                     uint32_t* __restrict__ err)
+{
+    __shared__ int32_t lo[kAdjCells * kAdjViews], hi[kAdjCells * kAdjViews];
+    const int K = blockIdx.x;
+    for (int i = threadIdx.x; i < kAdjCells * kAdjViews; i += blockDim.x) {
+        lo[i] = 0x7fffffff;
+        hi[i] = -1;
+    }
+    __syncthreads();
+    const uint32_t v0 = (uint32_t)(K / ntiles64) * kAdjViews;
+    for (uint32_t q = threadIdx.x; q < kcount[K]; q += blockDim.x) {
+        const uint32_t r = kbegin[K] + q, cell = adj_cell(row1[r], N);
+        for (uint32_t j = ptr1[r]; j < ptr1[r + 1]; j++) {
+            const uint32_t ray = ent1[j].x, v = ray / (uint32_t)W, c = ray - v * (uint32_t)W;
+            const uint32_t jv = v - v0;  // wraps above kAdjViews - 1 outside the view block
+            if (jv >= (uint32_t)kAdjViews) {
+                atomicOr(err, 4u);
+                continue;
+            }
+            atomicMin(&lo[cell * kAdjViews + jv], (int32_t)c);
+            atomicMax(&hi[cell * kAdjViews + jv], (int32_t)c);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kAdjCells * kAdjViews; i += blockDim.x) {
+        lo_out[(size_t)K * kAdjCells * kAdjViews + i] = lo[i];
+        hi_out[(size_t)K * kAdjCells * kAdjViews + i] = hi[i];
+    }
+}
+
+// Adjoint build: the LDS block of each stage-1 run (cell_block[key * 16 +
+// cell of the run's voxel]), its sort key block << 16 | (0xffff - length),
+// and runs per block.
+__global__ __launch_bounds__(256) void k_atile_run_keys(const uint32_t* __restrict__ ptr1,
+                                                        const uint32_t* __restrict__ key1,
+                                                        const uint32_t* __restrict__ row1, uint32_t rows1, int N,
+                                                        const uint32_t* __restrict__ cell_block, uint32_t nblocks,
+                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ val,
+                                                        uint32_t* __restrict__ cnt, uint32_t* __restrict__ err)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows1) return;
+    const uint32_t len = ptr1[r + 1] - ptr1[r];
+    const uint32_t bt = cell_block[(size_t)key1[r] * kAdjCells + adj_cell(row1[r], N)];
+    if (len == 0u || len >= 0xffffu || bt >= nblocks) {
+        atomicOr(err, 1u);
+        key[r] = 0u;
+    } else {
+        key[r] = (bt << 16) | (0xffffu - len);
+        atomicAdd(cnt + bt, 1u);
+    }
+    val[r] = r;
+}
+
+// Adjoint build: the interleaved entries with table slots, one thread per run
+// (in sorted order).
+__global__ __launch_bounds__(256) void k_atile_fill(const uint32_t* __restrict__ ptr1, const uint2* __restrict__ ent1,
+                                                    const uint32_t* __restrict__ key_s,
+                                                    const uint32_t* __restrict__ order, uint32_t rows1,
+                                                    const uint32_t* __restrict__ begin,
+                                                    const int32_t* __restrict__ tile_slices,
+                                                    const uint32_t* __restrict__ slice_ptr,
+                                                    const int32_t* __restrict__ slot_off,
+                                                    const int32_t* __restrict__ slot_clo,
+                                                    const int32_t* __restrict__ blk_v0, int W,
+                                                    uint2* __restrict__ ell, int32_t* __restrict__ slice_runs,
+                                                    uint32_t* __restrict__ err)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows1) return;
+    const uint32_t r = order[i], K = key_s[i] >> 16;
+    const uint32_t p = i - begin[K];
+    const uint32_t sl = (uint32_t)tile_slices[K] + p / 64u, lane = p % 64u;
+    slice_runs[sl * 64u + lane] = (int32_t)r;
+    const uint32_t v0 = (uint32_t)blk_v0[K];
+    const int32_t* off = slot_off + (size_t)K * (kAdjViews + 1);
+    const int32_t* clo = slot_clo + (size_t)K * kAdjViews;
+    uint2* dst = ell + slice_ptr[sl] + lane;
+    for (uint32_t j = ptr1[r], k = 0; j < ptr1[r + 1]; j++, k++) {
+        const uint2 e = ent1[j];
+        const uint32_t v = e.x / (uint32_t)W, c = e.x - v * (uint32_t)W, jv = v - v0;
+        uint32_t slot = (uint32_t)kAdjPadSlot;
+        if (jv < (uint32_t)kAdjViews) {
+            const int32_t d = (int32_t)c - clo[jv];
+            if (d >= 0 && d < off[jv + 1] - off[jv]) slot = (uint32_t)(off[jv] + d);
+            else atomicOr(err, 8u);
+        } else {
+            atomicOr(err, 4u);
+        }
+        dst[(size_t)k * 64u] = make_uint2(slot, e.y);
+    }
+}
+
 struct TileArgs {
-    const float* in;             // densities: rows of 8*B floats
+    const float* in;             // densities (forward) or ray weights (adjoint): rows of 8*B floats
     float* out;                  // stage-1 partial sums: rows of 8*B floats
     const uint2* ell;
     const uint32_t* slice_ptr;
@@ -1046,37 +1198,87 @@ struct TileArgs {
     const int32_t* tile_slices;
     const int32_t* xcd_tiles;
     int kmax, ntx, N, B;
+    const int32_t* slot_off;     // adjoint: ray tables of the blocks
+    const int32_t* slot_clo;
+    const int32_t* blk_v0;
+    int W;
 };
 
-// Forward stage 1 with the tile in LDS.  Workgroup g runs on XCD g % 8
-// (round-robin dispatch) and takes phantom b of tile xcd_tiles[xcd][k], so
-// the B workgroups of a tile share one L2 for its entries.  The tile's M
-// materials of phantom b go to LDS (64 rows of 64 voxels x MP floats, rows
-// padded by 2 floats so that voxels one row apart fall 2 banks apart); each
-// lane then sums one run of a slice in FP64, 8 entries in flight, and writes
-// its partial sums as stage-1 row run, phantom b.  The sums are those of
-// k_gather over the same runs, term for term in the same order.
-template <int M>
-__global__ __launch_bounds__(kTileThreads) void k_gather_tile(TileArgs a)
+// Stage 1 with the block's data in LDS.  Workgroup g runs on XCD g % 8
+// (round-robin dispatch) and takes phantom b of block xcd_tiles[xcd][k], so
+// the B workgroups of a block share one L2 for its entries.
+//   forward: the tile's M materials of phantom b go to LDS (64 rows of 64
+//     voxels x MP floats, rows padded by 2 floats so that voxels one row
+//     apart fall 2 banks apart);
+//   adjoint: the M weights of phantom b of the block's rays go to LDS, slot
+//     after slot (MP floats each), slot 0 zero.
+// Each lane then sums one run of a slice in FP64, 8 entries in flight, and
+// writes its partial sums as stage-1 row run, phantom b.  The sums are those
+// of k_gather over the same runs, term for term in the same order.
+template <int M, bool ADJ>
+__global__ __launch_bounds__(ADJ ? kAtileThreads : kTileThreads) void k_gather_tile(TileArgs a)
 {
-    constexpr int MP = (M + 1) & ~1;  // floats per voxel in LDS
-    constexpr int ROW = 64 * MP + 2;  // floats per tile row
+    constexpr int MP = (M + 1) & ~1;  // floats per voxel or ray in LDS
+    constexpr int ROW = 64 * MP + 2;  // floats per tile row (forward)
     constexpr int U = kTileUnroll;
-    __shared__ float lds[64 * ROW + 8];  // the tile, then 8 zeros for padding entries
+    constexpr int LDSF = ADJ ? kAdjSlotCap * MP : 64 * ROW + 8;
+    constexpr uint32_t PAD = ADJ ? (uint32_t)kAdjPadSlot : kTilePadVoxel;
+    __shared__ float lds[LDSF];
+    __shared__ int32_t tab[2 * kAdjViews + 2];  // adjoint: slot_off, slot_clo, first view of the block
     const int g = blockIdx.x, xcd = g % kXcds, slot = g / kXcds;
     const int k = slot / a.B, b = slot % a.B;
     if (k >= a.kmax) return;  // the whole workgroup
     const int t = a.xcd_tiles[xcd * a.kmax + k];
     if (t < 0) return;
-    const int tx0 = (t % a.ntx) * 64, ty0 = (t / a.ntx) * 64;
-    if (threadIdx.x < 8) lds[64 * ROW + threadIdx.x] = 0.f;
-    for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
-        const int ly = i >> 6, lx = i & 63, gy = ty0 + ly, gx = tx0 + lx;
-        float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (gy < a.N && gx < a.N) load_slot<M>(a.in, (size_t)gy * a.N + gx, a.B, b, v);
-        float2* d = reinterpret_cast<float2*>(lds + ly * ROW + lx * MP);
+    if constexpr (ADJ) {
+        if (threadIdx.x <= kAdjViews) tab[threadIdx.x] = a.slot_off[(size_t)t * (kAdjViews + 1) + threadIdx.x];
+        else if (threadIdx.x < 2 * kAdjViews + 1)
+            tab[threadIdx.x] = a.slot_clo[(size_t)t * kAdjViews + threadIdx.x - (kAdjViews + 1)];
+        else if (threadIdx.x == 2 * kAdjViews + 1)
+            tab[threadIdx.x] = a.blk_v0[t];
+        if (threadIdx.x < MP) lds[kAdjPadSlot * MP + threadIdx.x] = 0.f;
+        __syncthreads();
+        // slot s of view j (tab[j] <= s < tab[j + 1]) holds ray
+        // (view block * kAdjViews + j) * W + clo_j + s - tab[j]; four slots
+        // per lane in flight
+        const int S = tab[kAdjViews], v0 = tab[2 * kAdjViews + 1];
+        for (int s0 = 1 + (int)threadIdx.x; s0 < S; s0 += 4 * kAtileThreads) {
+            float v[4][8];
 #pragma unroll
-        for (int q = 0; q < MP / 2; q++) d[q] = make_float2(v[2 * q], v[2 * q + 1]);
+            for (int u = 0; u < 4; u++) {
+                const int s = s0 + u * kAtileThreads;
+                if (s < S) {
+                    int lo = 0, hi = kAdjViews;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (tab[mid] <= s) lo = mid;
+                        else hi = mid;
+                    }
+                    const size_t ray = (size_t)(v0 + lo) * a.W + tab[kAdjViews + 1 + lo] + (s - tab[lo]);
+                    load_slot<M>(a.in, ray, a.B, b, v[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int s = s0 + u * kAtileThreads;
+                if (s < S) {
+                    float2* d = reinterpret_cast<float2*>(lds + s * MP);
+#pragma unroll
+                    for (int q = 0; q < MP / 2; q++) d[q] = make_float2(v[u][2 * q], v[u][2 * q + 1]);
+                }
+            }
+        }
+    } else {
+        const int tx0 = (t % a.ntx) * 64, ty0 = (t / a.ntx) * 64;
+        if (threadIdx.x < 8) lds[64 * ROW + threadIdx.x] = 0.f;
+        for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
+            const int ly = i >> 6, lx = i & 63, gy = ty0 + ly, gx = tx0 + lx;
+            float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (gy < a.N && gx < a.N) load_slot<M>(a.in, (size_t)gy * a.N + gx, a.B, b, v);
+            float2* d = reinterpret_cast<float2*>(lds + ly * ROW + lx * MP);
+#pragma unroll
+            for (int q = 0; q < MP / 2; q++) d[q] = make_float2(v[2 * q], v[2 * q + 1]);
+        }
     }
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
@@ -1088,17 +1290,18 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_tile(TileArgs a)
         for (int m = 0; m < MP; m++) acc[m] = 0.0;
         uint2 q[U], nq[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) q[u] = u < len ? e[u * 64] : make_uint2(kTilePadVoxel, 0u);
+        for (int u = 0; u < U; u++) q[u] = u < len ? e[u * 64] : make_uint2(PAD, 0u);
         for (int j = 0; j < len; j += U) {
             // padding entries (runs shorter than the slice, and past len) are
-            // (kTilePadVoxel, 0.0f): zero density times zero length
+            // (PAD, 0.0f): zero LDS data times zero length
 #pragma unroll
             for (int u = 0; u < U; u++)
-                nq[u] = j + U + u < len ? e[(j + U + u) * 64] : make_uint2(kTilePadVoxel, 0u);
+                nq[u] = j + U + u < len ? e[(j + U + u) * 64] : make_uint2(PAD, 0u);
             float2 f[U][MP / 2];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const float2* p = reinterpret_cast<const float2*>(lds + (q[u].x >> 6) * ROW + (q[u].x & 63) * MP);
+                const float* base = ADJ ? lds + q[u].x * MP : lds + (q[u].x >> 6) * ROW + (q[u].x & 63) * MP;
+                const float2* p = reinterpret_cast<const float2*>(base);
 #pragma unroll
                 for (int h = 0; h < MP / 2; h++) f[u][h] = p[h];
             }
@@ -1148,9 +1351,16 @@ static void free_blocked(vct_blocked& f)
 // Regroup the n entries ent (row of entry i: row_of[i], entries in row order)
 // by key[i] < n_keys into the two-stage form f over `rows` rows.  key is
 // overwritten.  Stage-1 rows are the (key, row) runs in key-major order;
-// stage 2 lists each row's runs in key order.
+// stage 2 lists each row's runs in key order.  With run_info, the key and the
+// row of each stage-1 run are handed out there ([rows1] each, the caller
+// frees them).
+struct RunInfo {
+    uint32_t* key = nullptr;
+    uint32_t* row = nullptr;
+};
+
 static vct_status build_blocked(const uint2* ent, const uint32_t* row_of, uint32_t* key, int64_t n, uint32_t n_keys,
-                                int rows, vct_blocked& f, hipStream_t s)
+                                int rows, vct_blocked& f, hipStream_t s, RunInfo* run_info = nullptr)
 {
     vct_status st = VCT_OK;
     auto hip = [&](hipError_t e, const char* what) {
@@ -1190,8 +1400,11 @@ static vct_status build_blocked(const uint2* ent, const uint32_t* row_of, uint32
     hip(hipMalloc(&f.ent1, 8 * nz + 16), "hipMalloc ent1");
     hip(hipMalloc(&f.ptr1, 4 * ((size_t)rows1 + 1)), "hipMalloc ptr1");
     hip(hipMalloc(&row1, 4 * ((size_t)rows1 + 1)), "hipMalloc row1");
+    uint32_t* key1 = nullptr;
+    if (run_info) hip(hipMalloc(&key1, 4 * ((size_t)rows1 + 1)), "hipMalloc key1");
     if (st == VCT_OK) {
-        hipLaunchKernelGGL(k_stage1_fill, g1, blk, 0, s, order, head, run, ent, row_of, n, f.ent1, f.ptr1, row1);
+        hipLaunchKernelGGL(k_stage1_fill, g1, blk, 0, s, order, head, run, ent, row_of, n, f.ent1, f.ptr1, row1, key_s,
+                           key1);
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, f.ptr1 + rows1, (uint32_t)n);
         hip(hipGetLastError(), "k_stage1_fill");
     }
@@ -1230,7 +1443,12 @@ static vct_status build_blocked(const uint2* ent, const uint32_t* row_of, uint32
         hip(hipGetLastError(), "k_stage2_fill");
     }
     if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
-    void* done2[] = {tmp, row1, cnt, iota2, row1_s, order2};
+    if (run_info && st == VCT_OK) {
+        run_info->key = key1;
+        run_info->row = row1;
+        key1 = row1 = nullptr;
+    }
+    void* done2[] = {tmp, row1, key1, cnt, iota2, row1_s, order2};
     for (void* b : done2)
         if (b) (void)hipFree(b);
     f.rows1 = rows1;
@@ -1241,17 +1459,119 @@ static vct_status build_blocked(const uint2* ent, const uint32_t* row_of, uint32
 
 static void free_tiled(vct_tiled& t)
 {
-    void* bufs[] = {t.ell, t.slice_ptr, t.slice_len, t.slice_runs, t.tile_slices, t.xcd_tiles};
+    void* bufs[] = {t.ell,         t.slice_ptr, t.slice_len, t.slice_runs, t.tile_slices,
+                    t.xcd_tiles,   t.slot_off,  t.slot_clo,  t.blk_v0};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     t = vct_tiled{};
 }
 
-// Tile-resident form of the forward stage 1 from its blocked form f (block =
-// forward tile of tg).  Leaves t empty, and returns VCT_OK, when a run does
-// not fit the form (a run longer than 0xfffe entries, more than 65535 tiles
-// or 2^32 interleaved entries); HIP failures are errors.
-static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tiled& t, hipStream_t s)
+// What the adjoint LDS blocks are built from: the key and the row (voxel) of
+// each stage-1 run of the adjoint blocked form, over 64x64 tiles of an N x N
+// image and views of W channels.
+struct AdjBlocks {
+    const RunInfo* runs;
+    int N, W, nvb, ntiles64;
+};
+
+// The adjoint LDS blocks on the host: per blocked-form key (view block, tile)
+// the coarsest split of the tile's 4x4 cells (1, 2, 4, 8 or 16 parts, halves
+// and eighths cut along either axis, the one with the smaller largest table)
+// whose ray tables hold at most kAdjSlotCap slots.  lo/hi: first and last
+// channel of each (key, cell, view), hi < 0 where the cell has no ray of the
+// view.  Fills the block of each (key, cell), and per block the slot offsets,
+// first channels and first view.  Returns false if a cell alone does not fit.
+struct AdjTables {
+    std::vector<uint32_t> cell_block;
+    std::vector<int32_t> off, clo, v0;
+    int nblocks = 0, slots_max = 0;
+};
+
+static bool adj_tables(const std::vector<int32_t>& lo, const std::vector<int32_t>& hi, int nkeys, int ntiles64,
+                       AdjTables& T)
+{
+    // part of cell (cy, cx) under each split, coarsest first; entries of a
+    // level are alternatives
+    auto part = [](int split, int cy, int cx) {
+        switch (split) {
+        case 0: return 0;                          // 64x64
+        case 1: return cy >> 1;                    // 64 wide x 32 tall
+        case 2: return cx >> 1;                    // 32 wide x 64 tall
+        case 3: return (cy >> 1) * 2 + (cx >> 1);  // 32x32
+        case 4: return cy * 2 + (cx >> 1);         // 32 wide x 16 tall
+        case 5: return (cy >> 1) * 4 + cx;         // 16 wide x 32 tall
+        default: return cy * 4 + cx;               // 16x16
+        }
+    };
+    const int nparts[7] = {1, 2, 2, 4, 8, 8, 16};
+    const int levels[5][2] = {{0, 0}, {1, 2}, {3, 3}, {4, 5}, {6, 6}};
+    T = AdjTables{};
+    T.cell_block.assign((size_t)nkeys * kAdjCells, 0u);
+    std::vector<int32_t> plo(kAdjCells * kAdjViews), phi(kAdjCells * kAdjViews);
+    // tables of the parts of one split of key k into plo/phi; returns the largest table
+    auto tables = [&](int k, int split) {
+        std::fill(plo.begin(), plo.end(), 0x7fffffff);
+        std::fill(phi.begin(), phi.end(), -1);
+        for (int c = 0; c < kAdjCells; c++) {
+            const int q = part(split, c >> 2, c & 3);
+            for (int j = 0; j < kAdjViews; j++) {
+                const size_t i = ((size_t)k * kAdjCells + c) * kAdjViews + j;
+                if (hi[i] < 0) continue;
+                plo[q * kAdjViews + j] = std::min(plo[q * kAdjViews + j], lo[i]);
+                phi[q * kAdjViews + j] = std::max(phi[q * kAdjViews + j], hi[i]);
+            }
+        }
+        int worst = 0;
+        for (int q = 0; q < nparts[split]; q++) {
+            int n = kAdjPadSlot + 1;
+            for (int j = 0; j < kAdjViews; j++)
+                if (phi[q * kAdjViews + j] >= 0) n += phi[q * kAdjViews + j] - plo[q * kAdjViews + j] + 1;
+            worst = std::max(worst, n);
+        }
+        return worst;
+    };
+    for (int k = 0; k < nkeys; k++) {
+        int split = -1;
+        for (int l = 0; l < 5 && split < 0; l++) {
+            int best = -1, best_worst = 0;
+            for (int a = 0; a < 2; a++) {
+                const int sp = levels[l][a], w = tables(k, sp);
+                if (w <= kAdjSlotCap && (best < 0 || w < best_worst)) {
+                    best = sp;
+                    best_worst = w;
+                }
+            }
+            split = best;
+        }
+        if (split < 0) return false;
+        tables(k, split);
+        for (int c = 0; c < kAdjCells; c++)
+            T.cell_block[(size_t)k * kAdjCells + c] = (uint32_t)(T.nblocks + part(split, c >> 2, c & 3));
+        for (int q = 0; q < nparts[split]; q++) {
+            int32_t o = kAdjPadSlot + 1;
+            for (int j = 0; j < kAdjViews; j++) {
+                const bool any = phi[q * kAdjViews + j] >= 0;
+                T.off.push_back(o);
+                T.clo.push_back(any ? plo[q * kAdjViews + j] : 0);
+                o += any ? phi[q * kAdjViews + j] - plo[q * kAdjViews + j] + 1 : 0;
+            }
+            T.off.push_back(o);
+            T.v0.push_back((k / ntiles64) * kAdjViews);
+            T.slots_max = std::max(T.slots_max, (int)o);
+        }
+        T.nblocks += nparts[split];
+    }
+    return true;
+}
+
+// LDS-resident form of a stage 1 from its blocked form f: forward (adj ==
+// nullptr, block = forward tile of tg) or adjoint (blocks from adj_tables).
+// Leaves t empty, and returns VCT_OK, when a run does not fit the form (a
+// run longer than 0xfffe entries, more than 65535 blocks, 2^32 interleaved
+// entries or, in the adjoint, a 16x16 cell whose ray table exceeds
+// kAdjSlotCap slots); HIP failures are errors.
+static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tiled& t, hipStream_t s,
+                              const AdjBlocks* adj = nullptr)
 {
     vct_status st = VCT_OK;
     auto hip = [&](hipError_t e, const char* what) {
@@ -1259,56 +1579,138 @@ static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tile
     };
     t = vct_tiled{};
     const uint32_t R = f.rows1;
-    if (R == 0 || tg.ntiles > 65535 || tg.T != 64) return VCT_OK;
+    if (R == 0 || (!adj && tg.T != 64)) return VCT_OK;
+    const dim3 gr((R + 255) / 256), blk(256);
+    uint32_t herr = 0;
+    // adjoint: channel ranges per (key, cell, view), then the blocks and
+    // their ray tables
+    AdjTables at;
+    uint32_t* cell_block = nullptr;
+    if (adj) {
+        const int nkeys = adj->nvb * adj->ntiles64;
+        const size_t nr = (size_t)nkeys * kAdjCells * kAdjViews;
+        uint32_t *first = nullptr, *kbegin = nullptr, *kcount = nullptr, *e = nullptr;
+        int32_t *lo = nullptr, *hi = nullptr;
+        hip(hipMalloc(&first, 4 * (size_t)nkeys), "hipMalloc key first");
+        hip(hipMalloc(&kbegin, 4 * (size_t)nkeys), "hipMalloc key begin");
+        hip(hipMalloc(&kcount, 4 * (size_t)nkeys), "hipMalloc key count");
+        hip(hipMalloc(&e, 16), "hipMalloc err");
+        hip(hipMalloc(&lo, 4 * nr), "hipMalloc channel range");
+        hip(hipMalloc(&hi, 4 * nr), "hipMalloc channel range");
+        std::vector<uint32_t> hfirst(nkeys), hbegin(nkeys), hcount(nkeys);
+        if (st == VCT_OK) {
+            hipLaunchKernelGGL(k_zero_f32, dim3(1), blk, 0, s, (float*)e, 1);
+            hipLaunchKernelGGL(k_set_u32_n, dim3((nkeys + 255) / 256), blk, 0, s, first, nkeys, 0xffffffffu);
+            hipLaunchKernelGGL(k_key_first, gr, blk, 0, s, adj->runs->key, R, (uint32_t)nkeys, first, e);
+            hip(hipGetLastError(), "k_key_first");
+            hip(hipMemcpyAsync(hfirst.data(), first, 4 * (size_t)nkeys, hipMemcpyDeviceToHost, s), "copy");
+            hip(hipMemcpyAsync(&herr, e, 4, hipMemcpyDeviceToHost, s), "copy");
+            hip(hipStreamSynchronize(s), "sync");
+        }
+        uint32_t next = R;
+        for (int k = nkeys - 1; k >= 0; k--) {
+            hbegin[k] = hfirst[k] == 0xffffffffu ? next : hfirst[k];
+            hcount[k] = next - hbegin[k];
+            next = hbegin[k];
+        }
+        bool fits = st == VCT_OK && herr == 0 && next == 0;
+        std::vector<int32_t> hlo(fits ? nr : 0), hhi(fits ? nr : 0);
+        if (fits) {
+            hip(hipMemcpyAsync(kbegin, hbegin.data(), 4 * (size_t)nkeys, hipMemcpyHostToDevice, s), "copy");
+            hip(hipMemcpyAsync(kcount, hcount.data(), 4 * (size_t)nkeys, hipMemcpyHostToDevice, s), "copy");
+            hipLaunchKernelGGL(k_atile_range, dim3(nkeys), blk, 0, s, f.ptr1, f.ent1, adj->runs->row, kbegin, kcount,
+                               adj->N, adj->W, adj->ntiles64, lo, hi, e);
+            hip(hipGetLastError(), "k_atile_range");
+            hip(hipMemcpyAsync(hlo.data(), lo, 4 * nr, hipMemcpyDeviceToHost, s), "copy");
+            hip(hipMemcpyAsync(hhi.data(), hi, 4 * nr, hipMemcpyDeviceToHost, s), "copy");
+            hip(hipMemcpyAsync(&herr, e, 4, hipMemcpyDeviceToHost, s), "copy");
+            hip(hipStreamSynchronize(s), "sync");
+            fits = st == VCT_OK && herr == 0 && adj_tables(hlo, hhi, nkeys, adj->ntiles64, at) && at.nblocks <= 65535;
+        }
+        if (fits) {
+            hip(hipMalloc(&cell_block, 4 * at.cell_block.size()), "hipMalloc cell blocks");
+            hip(hipMalloc(&t.slot_off, 4 * at.off.size()), "hipMalloc slot offsets");
+            hip(hipMalloc(&t.slot_clo, 4 * at.clo.size()), "hipMalloc slot channels");
+            hip(hipMalloc(&t.blk_v0, 4 * at.v0.size()), "hipMalloc block views");
+            if (st == VCT_OK) {
+                hip(hipMemcpyAsync(cell_block, at.cell_block.data(), 4 * at.cell_block.size(), hipMemcpyHostToDevice, s),
+                    "copy");
+                hip(hipMemcpyAsync(t.slot_off, at.off.data(), 4 * at.off.size(), hipMemcpyHostToDevice, s), "copy");
+                hip(hipMemcpyAsync(t.slot_clo, at.clo.data(), 4 * at.clo.size(), hipMemcpyHostToDevice, s), "copy");
+                hip(hipMemcpyAsync(t.blk_v0, at.v0.data(), 4 * at.v0.size(), hipMemcpyHostToDevice, s), "copy");
+                hip(hipStreamSynchronize(s), "sync");
+            }
+            t.slots_max = at.slots_max;
+        }
+        void* bufs[] = {first, kbegin, kcount, e, lo, hi};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        if (st != VCT_OK || !fits) {
+            if (cell_block) (void)hipFree(cell_block);
+            free_tiled(t);
+            return st;
+        }
+    }
+    const int nblocks = adj ? at.nblocks : tg.ntiles;
+    if (nblocks > 65535) {
+        if (cell_block) (void)hipFree(cell_block);
+        free_tiled(t);
+        return VCT_OK;
+    }
     uint32_t *key = nullptr, *key_s = nullptr, *val = nullptr, *order = nullptr, *cnt = nullptr, *err = nullptr;
     uint32_t* tile_begin = nullptr;
     int32_t* slice_tile = nullptr;
     void* tmp = nullptr;
     size_t tb = 0;
-    const dim3 gr((R + 255) / 256), blk(256);
     hip(hipMalloc(&key, 4 * (size_t)R), "hipMalloc tile keys");
     hip(hipMalloc(&key_s, 4 * (size_t)R), "hipMalloc tile keys");
     hip(hipMalloc(&val, 4 * (size_t)R), "hipMalloc tile order");
     hip(hipMalloc(&order, 4 * (size_t)R), "hipMalloc tile order");
-    hip(hipMalloc(&cnt, 4 * (size_t)tg.ntiles + 16), "hipMalloc tile counts");
+    hip(hipMalloc(&cnt, 4 * (size_t)nblocks + 16), "hipMalloc tile counts");
     hip(hipMalloc(&err, 16), "hipMalloc tile err");
     if (st == VCT_OK) {
-        hipLaunchKernelGGL(k_set_u32_n, dim3((tg.ntiles + 255) / 256), blk, 0, s, cnt, tg.ntiles, 0xffffffffu);
         hipLaunchKernelGGL(k_zero_f32, dim3(1), blk, 0, s, (float*)err, 1);
-        hipLaunchKernelGGL(k_tile_run_keys, gr, blk, 0, s, f.ptr1, f.ent1, R, tg, key, val, cnt, err);
-        hipLaunchKernelGGL(k_first_to_count, dim3(1), dim3(1), 0, s, cnt, tg.ntiles, R);
-        hip(hipGetLastError(), "k_tile_run_keys");
+        if (adj) {
+            hipLaunchKernelGGL(k_zero_f32, dim3((nblocks + 255) / 256), blk, 0, s, (float*)cnt, nblocks);
+            hipLaunchKernelGGL(k_atile_run_keys, gr, blk, 0, s, f.ptr1, adj->runs->key, adj->runs->row, R, adj->N,
+                               cell_block, (uint32_t)nblocks, key, val, cnt, err);
+        } else {
+            hipLaunchKernelGGL(k_set_u32_n, dim3((nblocks + 255) / 256), blk, 0, s, cnt, nblocks, 0xffffffffu);
+            hipLaunchKernelGGL(k_tile_run_keys, gr, blk, 0, s, f.ptr1, f.ent1, R, tg, key, val, cnt, err);
+            hipLaunchKernelGGL(k_first_to_count, dim3(1), dim3(1), 0, s, cnt, nblocks, R);
+        }
+        hip(hipGetLastError(), "tile run keys");
     }
     if (st == VCT_OK) hip(rocprim::radix_sort_pairs(nullptr, tb, key, key_s, val, order, (size_t)R, 0u, 32u, s), "sort size");
     hip(hipMalloc(&tmp, tb + 16), "hipMalloc sort");
     if (st == VCT_OK) hip(rocprim::radix_sort_pairs(tmp, tb, key, key_s, val, order, (size_t)R, 0u, 32u, s), "sort");
-    // host: tile begins (in sorted order), slices per tile, slice -> tile
-    std::vector<uint32_t> n_t(tg.ntiles);
-    uint32_t herr = 0;
-    if (st == VCT_OK) hip(hipMemcpyAsync(n_t.data(), cnt, 4 * (size_t)tg.ntiles, hipMemcpyDeviceToHost, s), "copy");
+    // host: block begins (in sorted order), slices per block, slice -> block
+    std::vector<uint32_t> n_t(nblocks);
+    herr = 0;
+    if (st == VCT_OK) hip(hipMemcpyAsync(n_t.data(), cnt, 4 * (size_t)nblocks, hipMemcpyDeviceToHost, s), "copy");
     if (st == VCT_OK) hip(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s), "copy");
     if (st == VCT_OK) hip(hipStreamSynchronize(s), "sync");
-    std::vector<uint32_t> begin(tg.ntiles);
-    std::vector<int32_t> tslices(tg.ntiles + 1, 0), stile;
+    std::vector<uint32_t> begin(nblocks);
+    std::vector<int32_t> tslices(nblocks + 1, 0), stile;
     uint32_t acc = 0;
-    for (int k = 0; k < tg.ntiles; k++) {
+    for (int k = 0; k < nblocks; k++) {
         begin[k] = acc;
         acc += n_t[k];
         tslices[k + 1] = tslices[k] + (int32_t)((n_t[k] + 63) / 64);
         for (uint32_t q = 0; q < (n_t[k] + 63) / 64; q++) stile.push_back(k);
     }
-    const int nslices = tslices[tg.ntiles];
+    const int nslices = tslices[nblocks];
     bool ok = st == VCT_OK && herr == 0 && acc == R && nslices > 0;
     if (ok) {
-        hip(hipMalloc(&tile_begin, 4 * (size_t)tg.ntiles), "hipMalloc tile begin");
-        hip(hipMalloc(&t.tile_slices, 4 * ((size_t)tg.ntiles + 1)), "hipMalloc tile slices");
+        hip(hipMalloc(&tile_begin, 4 * (size_t)nblocks), "hipMalloc tile begin");
+        hip(hipMalloc(&t.tile_slices, 4 * ((size_t)nblocks + 1)), "hipMalloc tile slices");
         hip(hipMalloc(&slice_tile, 4 * (size_t)nslices), "hipMalloc slice tile");
         hip(hipMalloc(&t.slice_len, 4 * (size_t)nslices), "hipMalloc slice len");
         hip(hipMalloc(&t.slice_ptr, 4 * (size_t)nslices), "hipMalloc slice ptr");
         hip(hipMalloc(&t.slice_runs, 4 * 64 * (size_t)nslices), "hipMalloc slice runs");
         if (st == VCT_OK) {
-            hip(hipMemcpyAsync(tile_begin, begin.data(), 4 * (size_t)tg.ntiles, hipMemcpyHostToDevice, s), "copy");
-            hip(hipMemcpyAsync(t.tile_slices, tslices.data(), 4 * ((size_t)tg.ntiles + 1), hipMemcpyHostToDevice, s),
+            hip(hipMemcpyAsync(tile_begin, begin.data(), 4 * (size_t)nblocks, hipMemcpyHostToDevice, s), "copy");
+            hip(hipMemcpyAsync(t.tile_slices, tslices.data(), 4 * ((size_t)nblocks + 1), hipMemcpyHostToDevice, s),
                 "copy");
             hip(hipMemcpyAsync(slice_tile, stile.data(), 4 * (size_t)nslices, hipMemcpyHostToDevice, s), "copy");
             hipLaunchKernelGGL(k_tile_slice_len, dim3((nslices + 255) / 256), blk, 0, s, f.ptr1, order, slice_tile,
@@ -1316,15 +1718,14 @@ static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tile
             hip(hipGetLastError(), "k_tile_slice_len");
         }
     }
-    // host: slice offsets, per-tile work and the XCD tile lists (largest first
-    // onto the least loaded XCD)
+    // host: slice offsets, per-block work and the XCD block lists
     std::vector<uint32_t> slen(ok ? nslices : 0), sptr(ok ? nslices : 0);
     if (ok && st == VCT_OK) {
         hip(hipMemcpyAsync(slen.data(), t.slice_len, 4 * (size_t)nslices, hipMemcpyDeviceToHost, s), "copy");
         hip(hipStreamSynchronize(s), "sync");
     }
     uint64_t total = 0;
-    std::vector<uint64_t> work(tg.ntiles, 0);
+    std::vector<uint64_t> work(nblocks, 0);
     if (ok && st == VCT_OK) {
         for (int sl = 0; sl < nslices; sl++) {
             sptr[sl] = (uint32_t)total;
@@ -1334,25 +1735,39 @@ static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tile
         ok = total < (1ull << 32);
     }
     if (ok && st == VCT_OK) {
-        std::vector<int> idx(tg.ntiles);
-        for (int k = 0; k < tg.ntiles; k++) idx[k] = k;
-        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return work[x] > work[y]; });
         std::vector<std::vector<int32_t>> lists(kXcds);
-        std::vector<uint64_t> load(kXcds, 0);
-        for (int k : idx) {
-            if (n_t[k] == 0) continue;
-            int best = 0;
-            for (int x = 1; x < kXcds; x++)
-                if (load[x] < load[best]) best = x;
-            lists[best].push_back(k);
-            load[best] += work[k];
+        if (adj) {
+            // consecutive blocks (the quarters of a tile, then the next tile
+            // along the rays) share most of their rays: each XCD takes one
+            // contiguous run of blocks of about an eighth of the work
+            uint64_t run = 0;
+            for (int k = 0; k < nblocks; k++) {
+                if (n_t[k] == 0) continue;
+                const int x = (int)std::min<uint64_t>(kXcds - 1, (run + work[k] / 2) * kXcds / total);
+                lists[x].push_back(k);
+                run += work[k];
+            }
+        } else {
+            // largest tile first onto the least loaded XCD
+            std::vector<int> idx(nblocks);
+            for (int k = 0; k < nblocks; k++) idx[k] = k;
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return work[x] > work[y]; });
+            std::vector<uint64_t> load(kXcds, 0);
+            for (int k : idx) {
+                if (n_t[k] == 0) continue;
+                int best = 0;
+                for (int x = 1; x < kXcds; x++)
+                    if (load[x] < load[best]) best = x;
+                lists[best].push_back(k);
+                load[best] += work[k];
+            }
         }
         int kmax = 0;
         for (auto& l : lists) kmax = std::max(kmax, (int)l.size());
         std::vector<int32_t> xt((size_t)kXcds * kmax, -1);
         for (int x = 0; x < kXcds; x++)
             for (size_t q = 0; q < lists[x].size(); q++) xt[(size_t)x * kmax + q] = lists[x][q];
-        t.ntiles = tg.ntiles;
+        t.ntiles = nblocks;
         t.ntx = tg.ntx;
         t.nslices = nslices;
         t.kmax = kmax;
@@ -1362,18 +1777,24 @@ static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tile
         if (st == VCT_OK) {
             hip(hipMemcpyAsync(t.xcd_tiles, xt.data(), 4 * xt.size(), hipMemcpyHostToDevice, s), "copy");
             hip(hipMemcpyAsync(t.slice_ptr, sptr.data(), 4 * (size_t)nslices, hipMemcpyHostToDevice, s), "copy");
-            hipLaunchKernelGGL(k_fill_pad, dim3((unsigned)((total + 2 + 255) / 256)), blk, 0, s, t.ell, (size_t)total + 2);
+            hipLaunchKernelGGL(k_fill_pad, dim3((unsigned)((total + 2 + 255) / 256)), blk, 0, s, t.ell, (size_t)total + 2,
+                               adj ? (uint32_t)kAdjPadSlot : kTilePadVoxel);
             hip(hipMemsetAsync(t.slice_runs, 0xff, 4 * 64 * (size_t)nslices, s), "memset");
             hipLaunchKernelGGL(k_zero_f32, dim3(1), blk, 0, s, (float*)err, 1);
-            hipLaunchKernelGGL(k_tile_fill, gr, blk, 0, s, f.ptr1, f.ent1, key_s, order, R, tile_begin, t.tile_slices,
-                               t.slice_ptr, tg, t.ell, t.slice_runs, err);
-            hip(hipGetLastError(), "k_tile_fill");
+            if (adj)
+                hipLaunchKernelGGL(k_atile_fill, gr, blk, 0, s, f.ptr1, f.ent1, key_s, order, R, tile_begin,
+                                   t.tile_slices, t.slice_ptr, t.slot_off, t.slot_clo, t.blk_v0, adj->W, t.ell,
+                                   t.slice_runs, err);
+            else
+                hipLaunchKernelGGL(k_tile_fill, gr, blk, 0, s, f.ptr1, f.ent1, key_s, order, R, tile_begin,
+                                   t.tile_slices, t.slice_ptr, tg, t.ell, t.slice_runs, err);
+            hip(hipGetLastError(), "tile fill");
             hip(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s), "copy");
             hip(hipStreamSynchronize(s), "sync");
             ok = herr == 0;
         }
     }
-    void* bufs[] = {key, key_s, val, order, cnt, err, tile_begin, slice_tile, tmp};
+    void* bufs[] = {key, key_s, val, order, cnt, err, tile_begin, slice_tile, cell_block, tmp};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (st != VCT_OK || !ok || !t.ell) free_tiled(t);
@@ -1483,8 +1904,16 @@ vct_status vct_sysmat_build(vct_plan* p, int flags, vct_sysmat** out, void* stre
             hipLaunchKernelGGL(k_keys_adj, ge, dim3(256), 0, s, m->csc, keys2, (int64_t)total, ta, g.n_channels,
                                g.n_views, a.cosT, a.sinT, bkey);
             hip(hipGetLastError(), "k_keys_adj");
+            RunInfo ri;
             if (st == VCT_OK)
-                st = build_blocked(m->csc, keys2, bkey, total, (uint32_t)(nvb * ta.ntiles), NN, m->adj, s);
+                st = build_blocked(m->csc, keys2, bkey, total, (uint32_t)(nvb * ta.ntiles), NN, m->adj, s,
+                                   tiled ? &ri : nullptr);
+            if (st == VCT_OK && tiled) {
+                const AdjBlocks ab{&ri, g.recon_size, g.n_channels, nvb, ta.ntiles};
+                st = build_tiled(m->adj, ta, m->atile, s, &ab);
+            }
+            if (ri.key) (void)hipFree(ri.key);
+            if (ri.row) (void)hipFree(ri.row);
         }
         if (!keys && bkey) (void)hipFree(bkey);
     }
@@ -1512,6 +1941,7 @@ vct_status vct_sysmat_destroy(vct_sysmat* m)
     free_blocked(m->fwd);
     free_blocked(m->adj);
     free_tiled(m->tile);
+    free_tiled(m->atile);
     delete m;
     return VCT_OK;
 }
@@ -1555,9 +1985,11 @@ size_t vct_sysmat_bytes(const vct_sysmat* m)
         if (fs[k]->rows1)
             b += sizeof(uint2) * (size_t)m->nseg + sizeof(uint32_t) * ((size_t)fs[k]->rows1 + 1) +
                  sizeof(uint32_t) * ((size_t)rows[k] + 1) + sizeof(uint2) * ((size_t)fs[k]->rows1 + 1);
-    if (m->tile.ell)
-        b += sizeof(uint2) * m->tile.n_ell + sizeof(uint32_t) * 66 * (size_t)m->tile.nslices +
-             sizeof(int32_t) * ((size_t)m->tile.ntiles + 1 + (size_t)kXcds * m->tile.kmax);
+    for (const vct_tiled* t : {&m->tile, &m->atile})
+        if (t->ell)
+            b += sizeof(uint2) * t->n_ell + sizeof(uint32_t) * 66 * (size_t)t->nslices +
+                 sizeof(int32_t) * ((size_t)t->ntiles + 1 + (size_t)kXcds * t->kmax) +
+                 (t->slot_off ? sizeof(int32_t) * (2 * kAdjViews + 1) * (size_t)t->ntiles : 0);
     return b;
 }
 
@@ -1607,16 +2039,25 @@ static vct_status run_gather(GatherArgs& a, int items, int M, int lanes, int blo
     return VCT_OK;
 }
 
-// Forward stage 1 of one group over the tile-resident form: B workgroups per
-// tile, kmax tiles per XCD.
-static vct_status run_gather_tile(const vct_batch_group& G, int M, hipStream_t s)
+// The LDS-resident stage 1 of a direction, if built.
+static const vct_tiled* lds_form(const vct_sysmat* m, bool adjoint)
 {
-    const vct_tiled& t = G.sysmat->tile;
+    const vct_tiled& t = adjoint ? m->atile : m->tile;
+    if (!t.ell || (adjoint && t.slots_max > kAdjSlotCap)) return nullptr;
+    return &t;
+}
+
+// Stage 1 of one group over the LDS-resident form: B workgroups per block,
+// kmax blocks per XCD.
+static vct_status run_gather_tile(const vct_batch_group& G, const vct_tiled& t, bool adjoint, int M, hipStream_t s)
+{
     TileArgs ta{G.d_in, G.d_work, t.ell, t.slice_ptr, t.slice_len, t.slice_runs, t.tile_slices, t.xcd_tiles,
-                t.kmax, t.ntx, G.sysmat->plan->g.recon_size, G.n_phantoms};
+                t.kmax, t.ntx, G.sysmat->plan->g.recon_size, G.n_phantoms, t.slot_off, t.slot_clo, t.blk_v0,
+                G.sysmat->plan->g.n_channels};
     const unsigned grid = (unsigned)(kXcds * t.kmax * G.n_phantoms);
-#define VCT_TILE_M(MM) \
-    if (M == MM) hipLaunchKernelGGL(k_gather_tile<MM>, dim3(grid), dim3(kTileThreads), 0, s, ta);
+#define VCT_TILE_M(MM)                                                                                      \
+    if (M == MM && !adjoint) hipLaunchKernelGGL((k_gather_tile<MM, false>), dim3(grid), dim3(kTileThreads), 0, s, ta); \
+    if (M == MM && adjoint) hipLaunchKernelGGL((k_gather_tile<MM, true>), dim3(grid), dim3(kAtileThreads), 0, s, ta);
     VCT_TILE_M(1) VCT_TILE_M(2) VCT_TILE_M(3) VCT_TILE_M(4)
     VCT_TILE_M(5) VCT_TILE_M(6) VCT_TILE_M(7) VCT_TILE_M(8)
 #undef VCT_TILE_M
@@ -1639,6 +2080,7 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
     a.rows_per_item = b.rows_per_item = 256 / (lanes == 1 ? kSplit : lanes);
     int items_a = 0, items_b = 0;
     int tiles[kMaxGroups], n_tiles = 0;
+    const vct_tiled* tforms[kMaxGroups];
     for (int k = 0; k < n_groups; k++) {
         const vct_batch_group& G = groups[k];
         if (!G.sysmat || !G.d_in || !G.d_out) return set_error(VCT_ERR_ARG, "group %d has NULL pointers", k);
@@ -1649,7 +2091,8 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
         const vct_blocked& f = adjoint ? m->adj : m->fwd;
         const int rows = adjoint ? m->NN : m->nrays;
         if (G.d_work && f.rows1) {
-            if (!adjoint && m->tile.ell) {
+            if (const vct_tiled* t = lds_form(m, adjoint)) {
+                tforms[n_tiles] = t;
                 tiles[n_tiles++] = k;  // stage 1 by k_gather_tile
             } else {
                 add_group(a, items_a, f.ptr1, f.ent1, (int)f.rows1, G.d_in, G.d_work, G.n_phantoms, 1.f, 1);
@@ -1662,7 +2105,7 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
         }
     }
     hipStream_t s = (hipStream_t)stream;
-    for (int q = 0; q < n_tiles; q++) VCT_CHECK(run_gather_tile(groups[tiles[q]], M, s));
+    for (int q = 0; q < n_tiles; q++) VCT_CHECK(run_gather_tile(groups[tiles[q]], *tforms[q], adjoint, M, s));
     VCT_CHECK(run_gather(a, items_a, M, lanes, blocks, s));
     return run_gather(b, items_b, M, lanes, blocks, s);
 }
