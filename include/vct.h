@@ -345,9 +345,13 @@ vct_status vct_trace_adjoint(vct_plan* plan, const void* d_packed, int dens_type
  *                         so that each block's reads stay in one L2; used by
  *                         the batched gathers of groups that pass a
  *                         workspace.  With it, the forward stage 1 of a
- *                         group reads the 64x64 tile of its phantom from LDS
+ *                         group reads the 64x64 tile of its phantom from LDS,
+ *                         and the adjoint stage 1 the weights of the rays
+ *                         that cross one part of a tile in a 50-view block
  *                         (runs sorted by length in interleaved 64-run
- *                         slices), unless
+ *                         slices; the adjoint form is left out when a 16x16
+ *                         image cell is crossed by more than 3328 rays of a
+ *                         view block), unless
  *   VCT_SYSMAT_NO_TILE    is also given (the same sums, read through L2).
  * No counterpart in the reference. */
 enum { VCT_SYSMAT_TRANSPOSE = 1, VCT_SYSMAT_BLOCKED = 2, VCT_SYSMAT_NO_TILE = 4 };
