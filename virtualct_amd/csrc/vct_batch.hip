@@ -74,9 +74,9 @@ struct vct_blocked {
 //     (0, 0.0f).
 struct vct_tiled {
     int ntiles = 0, nslices = 0, kmax = 0, ntx = 0;
-    uint2* ell = nullptr;            // [sum slice_len * 64]
+    uint2* ell = nullptr;            // [sum slice_len * 64 + 64 * kTileUnroll]
     uint32_t* slice_ptr = nullptr;   // [nslices] first entry of the slice
-    uint32_t* slice_len = nullptr;   // [nslices] entries per lane
+    uint32_t* slice_len = nullptr;   // [nslices] entries per lane (adjoint: a multiple of kTileUnroll)
     int32_t* slice_runs = nullptr;   // [nslices * 64] stage-1 row of each lane, -1: none
     int32_t* tile_slices = nullptr;  // [ntiles + 1]
     int32_t* xcd_tiles = nullptr;    // [8 * kmax] tiles of each XCD, -1: none
@@ -1289,14 +1289,16 @@ __global__ __launch_bounds__(ADJ ? kAtileThreads : kTileThreads) void k_gather_t
 #pragma unroll
         for (int m = 0; m < MP; m++) acc[m] = 0.0;
         uint2 q[U], nq[U];
+        // padding entries (runs shorter than the slice, and in the forward
+        // past len) are (PAD, 0.0f): zero LDS data times zero length.  An
+        // adjoint len is a multiple of U, and the last prefetch reads the
+        // next slice or the slack after the last one.
 #pragma unroll
-        for (int u = 0; u < U; u++) q[u] = u < len ? e[u * 64] : make_uint2(PAD, 0u);
+        for (int u = 0; u < U; u++) q[u] = ADJ || u < len ? e[u * 64] : make_uint2(PAD, 0u);
         for (int j = 0; j < len; j += U) {
-            // padding entries (runs shorter than the slice, and past len) are
-            // (PAD, 0.0f): zero LDS data times zero length
 #pragma unroll
             for (int u = 0; u < U; u++)
-                nq[u] = j + U + u < len ? e[(j + U + u) * 64] : make_uint2(PAD, 0u);
+                nq[u] = ADJ || j + U + u < len ? e[(j + U + u) * 64] : make_uint2(PAD, 0u);
             float2 f[U][MP / 2];
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -1728,6 +1730,10 @@ static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tile
     std::vector<uint64_t> work(nblocks, 0);
     if (ok && st == VCT_OK) {
         for (int sl = 0; sl < nslices; sl++) {
+            // adjoint slices hold a multiple of kTileUnroll entries per lane,
+            // so the kernel reads them without bounds tests (the forward's
+            // short runs near tile edges would pay more padding than it saves)
+            if (adj) slen[sl] = (slen[sl] + kTileUnroll - 1) / kTileUnroll * kTileUnroll;
             sptr[sl] = (uint32_t)total;
             total += 64ull * slen[sl];
             work[stile[sl]] += 64ull * slen[sl];
@@ -1773,11 +1779,13 @@ static vct_status build_tiled(const vct_blocked& f, const TileGrid& tg, vct_tile
         t.kmax = kmax;
         t.n_ell = (size_t)total;
         hip(hipMalloc(&t.xcd_tiles, 4 * xt.size()), "hipMalloc xcd tiles");
-        hip(hipMalloc(&t.ell, 8 * (size_t)total + 16), "hipMalloc tile entries");
+        const size_t n_fill = (size_t)total + 64 * kTileUnroll;  // slack for the adjoint's last prefetch
+        hip(hipMalloc(&t.ell, 8 * n_fill), "hipMalloc tile entries");
         if (st == VCT_OK) {
             hip(hipMemcpyAsync(t.xcd_tiles, xt.data(), 4 * xt.size(), hipMemcpyHostToDevice, s), "copy");
             hip(hipMemcpyAsync(t.slice_ptr, sptr.data(), 4 * (size_t)nslices, hipMemcpyHostToDevice, s), "copy");
-            hipLaunchKernelGGL(k_fill_pad, dim3((unsigned)((total + 2 + 255) / 256)), blk, 0, s, t.ell, (size_t)total + 2,
+            hip(hipMemcpyAsync(t.slice_len, slen.data(), 4 * (size_t)nslices, hipMemcpyHostToDevice, s), "copy");
+            hipLaunchKernelGGL(k_fill_pad, dim3((unsigned)((n_fill + 255) / 256)), blk, 0, s, t.ell, n_fill,
                                adj ? (uint32_t)kAdjPadSlot : kTilePadVoxel);
             hip(hipMemsetAsync(t.slice_runs, 0xff, 4 * 64 * (size_t)nslices, s), "memset");
             hipLaunchKernelGGL(k_zero_f32, dim3(1), blk, 0, s, (float*)err, 1);
