@@ -319,6 +319,20 @@ vct_status vct_backproject(vct_plan* plan, const float* d_fltd, int n_sino, floa
  * F/reconstructImageTOMO.cpp:11-105). */
 vct_status vct_backproject_groups(vct_plan* const* plans, const float* const* d_fltd, const int* n_sino,
                                   float* const* d_img, int n_groups, int sinos_per_lane, void* stream);
+/* The same with each group's filtered sinograms interleaved in slices of k
+ * = sinos_per_lane (4, 8, 12 or 16): d_il[g] as vct_interleave_sinograms
+ * writes it.  A lane then reads the taps of its k sinograms as contiguous
+ * 16-byte loads; the images equal vct_backproject_groups' bit for bit.
+ * vct_interleave_sinograms writes n_sino sinograms [n_sino][V][W] at d_fltd
+ * as d_il[s / k][V][W][k] (zeros past n_sino), into
+ * vct_interleaved_bytes(plan, n_sino, k) bytes (0 for an invalid k).  No
+ * counterpart in the reference (it reconstructs one image per run,
+ * F/reconstructImageTOMO.cpp:11-105); serves batched FBP. */
+size_t vct_interleaved_bytes(const vct_plan* plan, int n_sino, int k);
+vct_status vct_interleave_sinograms(const vct_plan* plan, const float* d_fltd, int n_sino, int k, float* d_il,
+                                    void* stream);
+vct_status vct_backproject_groups_il(vct_plan* const* plans, const float* const* d_il, const int* n_sino,
+                                     float* const* d_img, int n_groups, int sinos_per_lane, void* stream);
 
 /* Adjoint of vct_trace_paths weighted per ray:
  *   grad[v][m] += sum_rays w[ray][m] * sum_{segments of ray in voxel v} length

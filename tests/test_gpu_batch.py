@@ -546,6 +546,42 @@ def test_sysmat_rows_equal_reference_segment_lists(gpu, golden):
     sm.close()
 
 
+@pytest.mark.parametrize("exact", [False, True])
+def test_interleaved_grouped_backprojection_equals_plain(gpu, exact):
+    """Sinograms interleaved in slices of k (4, 8, 12, 16; the last slice
+    zero-padded) give the grouped backprojection's images bit for bit, with
+    float-weight and exact terms, at 256^2 and 768^2 over 50/N cm plus an odd
+    size; the interleaved array is the permuted stack."""
+    import torch
+    ops, batch, _ = _mods()
+    from virtualct_amd import _lib, default_geometry
+    gen = torch.Generator(device=gpu).manual_seed(13)
+    sizes, Ss = (256, 768, 97), (5, 9, 3)
+    plans = [ops.plan_for(default_geometry(recon_size=N, pixel_cm=50.0 / N, n_views=120)) for N in sizes]
+    if exact:
+        for p in plans:
+            p.set_option(_lib.OPT_EXACT_BACKPROJECT, 1)
+    try:
+        fls = [torch.randn((S, p.V, p.W), generator=gen, device=gpu) * 100 for p, S in zip(plans, Ss)]
+        for k in batch.IL_WIDTHS:
+            ils = [batch.interleave_sinograms(p, f, k) for p, f in zip(plans, fls)]
+            for p, f, il in zip(plans, fls, ils):
+                n = il.shape[0] * k
+                ref = torch.zeros((n, p.V, p.W), device=gpu)
+                ref[: f.shape[0]] = f
+                assert torch.equal(il, ref.view(il.shape[0], k, p.V, p.W).permute(0, 2, 3, 1)), (p.N, k)
+            imgs = [torch.full((f.shape[0], p.N, p.N), float("nan"), device=gpu) for p, f in zip(plans, fls)]
+            batch.backproject_groups_il([(p, il, f.shape[0], im) for p, f, il, im in zip(plans, fls, ils, imgs)], k)
+            ref = [torch.full_like(im, float("nan")) for im in imgs]
+            batch.backproject_groups(list(zip(plans, fls, ref)), k)
+            for p, im, r in zip(plans, imgs, ref):
+                assert torch.equal(im, r), (p.N, k, exact)
+    finally:
+        if exact:
+            for p in plans:
+                p.set_option(_lib.OPT_EXACT_BACKPROJECT, 0)
+
+
 def test_grouped_spectral_and_backprojection_equal_per_group_calls(gpu):
     """One launch over several size groups (spectral sums of all groups, then
     the backprojections of all groups) gives each group exactly what its own

@@ -91,6 +91,9 @@ _SIGS = {
     "vct_filter_sinogram_ws": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
     "vct_backproject": (C.c_int, [_P, _P, C.c_int, _P, _P]),
     "vct_backproject_groups": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P]),
+    "vct_interleaved_bytes": (C.c_size_t, [_P, C.c_int, C.c_int]),
+    "vct_interleave_sinograms": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P]),
+    "vct_backproject_groups_il": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P]),
     "vct_lengthproj": (C.c_int, [_P, _P, C.c_double, C.c_double, C.c_double, _P, _P]),
     "vct_lengthproj_window": (C.c_int, [_P, _P, C.c_int, C.c_double, C.c_double, C.c_double, _P, _P]),
     "vct_beam_hardening": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, _P, _P]),

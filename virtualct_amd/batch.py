@@ -228,6 +228,48 @@ def backproject_groups(items, sinos_per_lane: int = 0):
     call("vct_backproject_groups", ps, fs, ns, imgs, len(items), int(sinos_per_lane), _stream(items[0][1].device))
 
 
+IL_WIDTHS = (4, 8, 12, 16)
+
+
+def interleave_sinograms(plan: ops.Plan, fltd: torch.Tensor, k: int, out: torch.Tensor | None = None):
+    """Filtered sinograms [S,V,W] -> slices of k, [ceil(S/k),V,W,k] (zeros
+    past S), the layout ``backproject_groups_il`` reads."""
+    if k not in IL_WIDTHS:
+        raise ValueError(f"k must be one of {IL_WIDTHS}")
+    S = fltd.shape[0]
+    if fltd.dtype != torch.float32 or fltd.shape != (S, plan.V, plan.W) or not fltd.is_contiguous():
+        raise ValueError(f"filtered sinograms must be contiguous float32 [S,{plan.V},{plan.W}]")
+    shape = ((S + k - 1) // k, plan.V, plan.W, k)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=fltd.device)
+    elif out.dtype != torch.float32 or tuple(out.shape) != shape or not out.is_contiguous():
+        raise ValueError(f"out must be contiguous float32 {shape}")
+    call("vct_interleave_sinograms", plan.handle, _p(fltd), S, k, _p(out), _stream(fltd.device))
+    return out
+
+
+def backproject_groups_il(items, k: int):
+    """items: [(plan, interleaved sinograms [ceil(S/k),V,W,k], S, images
+    [S,N,N] float32)] backprojected in one launch, k sinograms per lane
+    (each image as ``backproject_groups`` gives it)."""
+    if not 1 <= len(items) <= 16:
+        raise ValueError("1..16 groups")
+    if k not in IL_WIDTHS:
+        raise ValueError(f"k must be one of {IL_WIDTHS}")
+    for plan, il, S, img in items:
+        if il.dtype != torch.float32 or tuple(il.shape) != ((S + k - 1) // k, plan.V, plan.W, k) \
+                or not il.is_contiguous():
+            raise ValueError(f"interleaved sinograms must be contiguous float32 [{(S + k - 1) // k},{plan.V},"
+                             f"{plan.W},{k}]")
+        if img.dtype != torch.float32 or img.shape != (S, plan.N, plan.N) or not img.is_contiguous():
+            raise ValueError(f"images must be contiguous float32 [{S},{plan.N},{plan.N}]")
+    keep_p, ps = _ptr_array([it[0].handle.value for it in items])
+    keep_f, fs = _ptr_array([it[1].data_ptr() for it in items])
+    keep_n, ns = _ptr_array([it[2] for it in items], C.c_int)
+    keep_i, imgs = _ptr_array([it[3].data_ptr() for it in items])
+    call("vct_backproject_groups_il", ps, fs, ns, imgs, len(items), int(k), _stream(items[0][1].device))
+
+
 def batch_loss(plan: ops.Plan, L, M: int, att, pdf, target, w=None, loss=None):
     """-> w (rays rows, batched layout: d loss / d L), loss [B]."""
     B = L.shape[1]
@@ -414,12 +456,14 @@ class BatchForwardFBP:
             self.lanes, self.blocks = cfg["lanes"], cfg["blocks"]
         else:
             self.lanes, self.blocks = lanes, blocks
-        # the grouped backprojection's sinograms per lane, timed on this run's
-        # shapes (random filtered sinograms)
-        key = ("grouped backprojection", tuple((g.plan.N, g.B * self.S) for g in self.groups),
+        # the grouped backprojection's sinograms per lane (= the interleaved
+        # slice width), timed on this run's shapes (random filtered sinograms)
+        key = ("grouped backprojection, interleaved", tuple((g.plan.N, g.B * self.S) for g in self.groups),
                self.groups[0].plan.V, self.groups[0].plan.W)
-        self.bp_k = tuner().best(key, lambda: [{"sinos_per_lane": k} for k in (4, 8, 12, 16)],
+        self.bp_k = tuner().best(key, lambda: [{"sinos_per_lane": k} for k in IL_WIDTHS],
                                  self._time_bp)["sinos_per_lane"]
+        self.il = [torch.empty(((g.B * self.S + self.bp_k - 1) // self.bp_k, g.plan.V, g.plan.W, self.bp_k),
+                               dtype=torch.float32, device=g.dens.device) for g in self.groups]
         # the filter's packed operand lives in a buffer of this run, not in the
         # context scratch, so a replay of the captured graph shares nothing
         # with later eager calls
@@ -445,18 +489,21 @@ class BatchForwardFBP:
     def _time_bp(self, cfg):
         from .autotune import events_ms
         gen = torch.Generator(device=self.groups[0].dens.device).manual_seed(2)
+        k = cfg["sinos_per_lane"]
         items = []
         for g in self.groups:
             n = g.B * self.S
             fl = torch.randn((n, g.plan.V, g.plan.W), generator=gen, device=g.dens.device)
-            items.append((g.plan, fl, torch.empty((n, g.plan.N, g.plan.N), device=g.dens.device)))
-        return events_ms(lambda: backproject_groups(items, cfg["sinos_per_lane"]))
+            items.append((g.plan, interleave_sinograms(g.plan, fl, k), n,
+                          torch.empty((n, g.plan.N, g.plan.N), device=g.dens.device)))
+        return events_ms(lambda: backproject_groups_il(items, k))
 
     def run(self):
         """One pass: the forward gather of every group (one launch), the
         spectral sums of every group (one launch), the detector of all
-        sinograms (one launch), one stacked filter GEMM and the
-        backprojections of every group (one launch)."""
+        sinograms (one launch), one stacked filter GEMM, each group's
+        filtered sinograms interleaved in slices of bp_k (one launch per
+        group) and the backprojections of every group (one launch)."""
         gs = self.groups
         batch_paths([(g.sysmat, g.dens, g.L, g.B) for g in gs], self.M, self.lanes, self.blocks)
         p0 = gs[0].plan
@@ -466,14 +513,15 @@ class BatchForwardFBP:
         fltd = ops.filter_sinogram(p0, stack, self.filt, work=self.filter_work)
         sinos, imgs, items = [], [], []
         off = 0
-        for g in gs:
+        for g, il in zip(gs, self.il):
             k = g.B * self.S
             img = torch.empty((k, g.plan.N, g.plan.N), dtype=torch.float32, device=stack.device)
             sinos.append(stack[off:off + k])
             imgs.append(img)
-            items.append((g.plan, fltd[off:off + k], img))
+            interleave_sinograms(g.plan, fltd[off:off + k], self.bp_k, il)
+            items.append((g.plan, il, k, img))
             off += k
-        backproject_groups(items, self.bp_k)
+        backproject_groups_il(items, self.bp_k)
         self.out = (sinos, imgs)
         return self.out
 

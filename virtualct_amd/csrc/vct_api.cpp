@@ -790,10 +790,9 @@ vct_status vct_backproject(vct_plan* p, const float* d_fltd, int n_sino, float* 
     return launch_backproject(make_bp_args(p, d_fltd, d_img), n_sino, pick_stream(p, stream));
 }
 
-vct_status vct_backproject_groups(vct_plan* const* plans, const float* const* d_fltd, const int* n_sino,
-                                  float* const* d_img, int n_groups, int sinos_per_lane, void* stream)
+static vct_status backproject_groups(vct_plan* const* plans, const float* const* d_fltd, const int* n_sino,
+                                     float* const* d_img, int n_groups, int sinos_per_lane, bool il, void* stream)
 {
-    clear_error();
     if (!plans || !d_fltd || !n_sino || !d_img) return set_error(VCT_ERR_ARG, "NULL argument");
     if (n_groups < 1 || n_groups > kMaxBpGroups)
         return set_error(VCT_ERR_ARG, "n_groups %d outside 1..%d", n_groups, kMaxBpGroups);
@@ -810,8 +809,42 @@ vct_status vct_backproject_groups(vct_plan* const* plans, const float* const* d_
         smax = std::max(smax, n_sino[k]);
     }
     int K = sinos_per_lane > 0 ? sinos_per_lane : 16;
-    K = std::min(K, std::max(smax, 2));
-    return launch_backproject_groups(ga, K, (hipStream_t)stream);
+    if (!il) K = std::min(K, std::max(smax, 2));
+    return launch_backproject_groups(ga, K, (hipStream_t)stream, il);
+}
+
+vct_status vct_backproject_groups(vct_plan* const* plans, const float* const* d_fltd, const int* n_sino,
+                                  float* const* d_img, int n_groups, int sinos_per_lane, void* stream)
+{
+    clear_error();
+    return backproject_groups(plans, d_fltd, n_sino, d_img, n_groups, sinos_per_lane, false, stream);
+}
+
+static bool il_width(int k) { return k == 4 || k == 8 || k == 12 || k == 16; }
+
+size_t vct_interleaved_bytes(const vct_plan* p, int n_sino, int k)
+{
+    if (!p || n_sino < 1 || !il_width(k)) return 0;
+    return (size_t)((n_sino + k - 1) / k) * k * p->g.n_views * p->g.n_channels * sizeof(float);
+}
+
+vct_status vct_interleave_sinograms(const vct_plan* p, const float* d_fltd, int n_sino, int k, float* d_il,
+                                    void* stream)
+{
+    clear_error();
+    if (!p || !d_fltd || !d_il) return set_error(VCT_ERR_ARG, "NULL argument");
+    if (n_sino < 1 || n_sino > 65535) return set_error(VCT_ERR_ARG, "n_sino %d invalid", n_sino);
+    if (!il_width(k)) return set_error(VCT_ERR_ARG, "slices of %d sinograms (4, 8, 12 or 16)", k);
+    return launch_interleave_sinos(d_fltd, n_sino, p->g.n_views * p->g.n_channels, k, d_il, (hipStream_t)stream);
+}
+
+vct_status vct_backproject_groups_il(vct_plan* const* plans, const float* const* d_il, const int* n_sino,
+                                     float* const* d_img, int n_groups, int sinos_per_lane, void* stream)
+{
+    clear_error();
+    if (!il_width(sinos_per_lane))
+        return set_error(VCT_ERR_ARG, "sinos_per_lane %d: the interleaved slices hold 4, 8, 12 or 16", sinos_per_lane);
+    return backproject_groups(plans, d_il, n_sino, d_img, n_groups, sinos_per_lane, true, stream);
 }
 
 vct_status vct_lengthproj(vct_plan* p, const float* d_proj, double z_cm, double couch_speed_cm, double iso_det_cm,

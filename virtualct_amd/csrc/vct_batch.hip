@@ -987,6 +987,7 @@ constexpr uint32_t kTilePadVoxel = 64u * 64u;
 constexpr int kAtileThreads = 512;
 constexpr int kAdjSlotCap = 3328;       // 78 KB at MP = 6
 constexpr int kAdjCells = 16;           // 16x16 cells of a 64x64 tile
+static_assert(kAdjTile == 64, "adjoint LDS blocks split 64x64 tiles into 4x4 cells of 16x16");
 constexpr int kAdjPadSlot = 0;
 
 __global__ __launch_bounds__(256) void k_fill_pad(uint2* __restrict__ p, size_t n, uint32_t pad)

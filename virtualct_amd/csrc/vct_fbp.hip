@@ -354,9 +354,14 @@ __device__ __forceinline__ BpView bp_view(const BackprojArgs& a, double xx, doub
 // bp_view and applied in float, as in k_backproject_sym (within ~1e-6 of
 // the exact terms); otherwise every term is the single-sinogram FP64
 // expression and the images equal k_backproject's bit for bit.
-template <bool WIDE, int K, bool FW>
+// IL (K a multiple of 4): a.fltd holds the sinograms interleaved in slices
+// of K, [S/K][V][W][K] (k_interleave_sinos), so the K tap pairs of a
+// pixel-view are 2K contiguous floats, read as K/2 16-byte loads instead of
+// 2K 4-byte ones; the same values, so the same images.
+template <bool WIDE, int K, bool FW, bool IL = false>
 __device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int bx, int by, int bz)
 {
+    static_assert(!IL || K % 4 == 0, "interleaved slices hold a multiple of 4 sinograms");
     const double dbx = a.dbeta_view * a.xfactor;
     const int tx = threadIdx.x & (kBpTile - 1), ty = threadIdx.x / kBpTile;
     const int x = bx * kBpTile + tx;
@@ -365,6 +370,26 @@ __device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int 
     const float* fl[K];
 #pragma unroll
     for (int k = 0; k < K; k++) fl[k] = a.fltd + (size_t)min(s0 + k, S - 1) * a.V * a.W;
+    // IL: the tap pairs of bin pair off are floats [off * K, off * K + 2K) of
+    // slice bz; off <= V * W - 2, so they stay inside the slice
+    const float* slice = a.fltd + (size_t)bz * a.V * a.W * K;
+    auto taps = [&](size_t off, float (&t0)[K], float (&t1)[K]) {
+        if constexpr (IL) {
+            const float4* q = reinterpret_cast<const float4*>(slice + off * K);
+#pragma unroll
+            for (int h = 0; h < K / 4; h++) {
+                const float4 u = q[h], w = q[K / 4 + h];
+                t0[4 * h] = u.x; t0[4 * h + 1] = u.y; t0[4 * h + 2] = u.z; t0[4 * h + 3] = u.w;
+                t1[4 * h] = w.x; t1[4 * h + 1] = w.y; t1[4 * h + 2] = w.z; t1[4 * h + 3] = w.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                t0[k] = fl[k][off];
+                t1[k] = fl[k][off + 1];
+            }
+        }
+    };
     const bool in_img = x < a.N && y < a.N;
     // prepareMaskData, F/misc.cpp:20-36
     const double dyc = (double)y - a.mask_r, dxc = (double)x - a.mask_r;
@@ -380,20 +405,12 @@ __device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int 
         // itself again), so the gathers' latency overlaps the arithmetic
         BpView g = bp_view<WIDE, FW>(a, xx, yy, 0);
         float r0[K], r1[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            r0[k] = fl[k][g.off];
-            r1[k] = fl[k][g.off + 1];
-        }
+        taps(g.off, r0, r1);
 #pragma unroll 1
         for (int v = 0; v < a.use_views; v++) {
             const BpView gn = bp_view<WIDE, FW>(a, xx, yy, min(v + 1, a.use_views - 1));
             float n0[K], n1[K];
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                n0[k] = fl[k][gn.off];
-                n1[k] = fl[k][gn.off + 1];
-            }
+            taps(gn.off, n0, n1);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 float term;
@@ -430,7 +447,7 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_multi(Backpro
 // pref[g] <= l < pref[g + 1] (a scalar search, uniform over the workgroup)
 // and runs tile l - pref[g] of that group's tiles x slices, exactly as
 // k_backproject_multi would.
-template <bool WIDE, int K, bool FW>
+template <bool WIDE, int K, bool FW, bool IL = false>
 __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_groups(BpGroups ga)
 {
     const int l = blockIdx.x;
@@ -440,7 +457,27 @@ __global__ __launch_bounds__(kBpTile * kBpTile) void k_backproject_groups(BpGrou
     const int tx = ga.tiles_x[g];
     const int per_slice = tx * tx;
     const int bz = local / per_slice, r = local - bz * per_slice;
-    bp_multi_tile<WIDE, K, FW>(ga.a[g], ga.S[g], r % tx, r / tx, bz);
+    bp_multi_tile<WIDE, K, FW, IL>(ga.a[g], ga.S[g], r % tx, r / tx, bz);
+}
+
+// Sinograms [S][VW] -> slices of K, dst[S/K][VW][K] (zeros past S): lane p
+// of slice sl gathers the K values of pixel-row entry p (each read coalesced
+// over the lanes) and stores them as K/4 16-byte stores.
+template <int K>
+__global__ __launch_bounds__(256) void k_interleave_sinos(const float* __restrict__ src, int S, int VW,
+                                                          float* __restrict__ dst)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x, sl = blockIdx.y;
+    if (p >= VW) return;
+    float v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int s = sl * K + k;
+        v[k] = s < S ? src[(size_t)s * VW + p] : 0.f;
+    }
+    float4* d = reinterpret_cast<float4*>(dst + ((size_t)sl * VW + p) * K);
+#pragma unroll
+    for (int h = 0; h < K / 4; h++) d[h] = make_float4(v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]);
 }
 
 // One sinogram by orbits of the quarter turn.  With V views over 360 degrees
@@ -601,7 +638,7 @@ static void launch_bp_groups_k(const BpGroups& ga, int K, hipStream_t s)
     hipLaunchKernelGGL((k_backproject_groups<WIDE, K0, FW>), dim3(ga.pref[ga.n]), dim3(kBpTile * kBpTile), 0, s, ga);
 }
 
-vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s)
+vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s, bool il)
 {
     if (ga.n < 1 || ga.n > kMaxBpGroups) return set_error(VCT_ERR_ARG, "%d groups (1..%d)", ga.n, kMaxBpGroups);
     if (K < 2) K = 2;
@@ -620,10 +657,39 @@ vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s)
         if (total > INT_MAX) return set_error(VCT_ERR_ARG, "grouped backprojection too large");
         ga.pref[g + 1] = (int)total;
     }
-    if (wide && exact) launch_bp_groups_k<true, false, 2>(ga, K, s);
+    if (il) {
+        const dim3 grid(ga.pref[ga.n]), blk(kBpTile * kBpTile);
+#define VCT_BP_IL(KK)                                                                                   \
+    case KK:                                                                                            \
+        if (wide && exact) hipLaunchKernelGGL((k_backproject_groups<true, KK, false, true>), grid, blk, 0, s, ga);  \
+        else if (wide) hipLaunchKernelGGL((k_backproject_groups<true, KK, true, true>), grid, blk, 0, s, ga);  \
+        else if (exact) hipLaunchKernelGGL((k_backproject_groups<false, KK, false, true>), grid, blk, 0, s, ga); \
+        else hipLaunchKernelGGL((k_backproject_groups<false, KK, true, true>), grid, blk, 0, s, ga);   \
+        break;
+        switch (K) {
+            VCT_BP_IL(4) VCT_BP_IL(8) VCT_BP_IL(12) VCT_BP_IL(16)
+        default: return set_error(VCT_ERR_ARG, "interleaved slices of %d sinograms (4, 8, 12 or 16)", K);
+        }
+#undef VCT_BP_IL
+    } else if (wide && exact) launch_bp_groups_k<true, false, 2>(ga, K, s);
     else if (wide) launch_bp_groups_k<true, true, 2>(ga, K, s);
     else if (exact) launch_bp_groups_k<false, false, 2>(ga, K, s);
     else launch_bp_groups_k<false, true, 2>(ga, K, s);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status launch_interleave_sinos(const float* src, int S, int VW, int K, float* dst, hipStream_t s)
+{
+    if (S < 1 || VW < 1) return set_error(VCT_ERR_ARG, "empty sinograms");
+    const dim3 grid((VW + 255) / 256, (S + K - 1) / K), blk(256);
+    switch (K) {
+    case 4: hipLaunchKernelGGL(k_interleave_sinos<4>, grid, blk, 0, s, src, S, VW, dst); break;
+    case 8: hipLaunchKernelGGL(k_interleave_sinos<8>, grid, blk, 0, s, src, S, VW, dst); break;
+    case 12: hipLaunchKernelGGL(k_interleave_sinos<12>, grid, blk, 0, s, src, S, VW, dst); break;
+    case 16: hipLaunchKernelGGL(k_interleave_sinos<16>, grid, blk, 0, s, src, S, VW, dst); break;
+    default: return set_error(VCT_ERR_ARG, "interleaved slices of %d sinograms (4, 8, 12 or 16)", K);
+    }
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }

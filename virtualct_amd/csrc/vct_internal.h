@@ -194,7 +194,9 @@ vct_status launch_pack_i16(const int16_t* src, int M, int N, uint16_t* dst, hipS
 vct_status launch_pack_f32(const float* src, int M, int N, float* dst, hipStream_t s);
 vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipStream_t s);
 vct_status launch_backproject(const BackprojArgs& a, int S, hipStream_t s);
-vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s);
+// il: every group's sinograms interleaved in slices of K (4, 8, 12 or 16)
+vct_status launch_backproject_groups(BpGroups& ga, int K, hipStream_t s, bool il = false);
+vct_status launch_interleave_sinos(const float* src, int S, int VW, int K, float* dst, hipStream_t s);
 vct_status launch_beam_hardening(const float* in, float* out, int n, double alpha, double beta, hipStream_t s);
 vct_status launch_lengthproj(const LengthprojArgs& a, hipStream_t s);
 vct_status plan_filter_matrix(vct_plan* plan, int filter, float** d_bpk, float** d_toep);
