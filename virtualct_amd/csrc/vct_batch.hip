@@ -1303,7 +1303,10 @@ __global__ __launch_bounds__(ADJ ? kAtileThreads : kTileThreads) void k_gather_t
             float2 f[U][MP / 2];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const float* base = ADJ ? lds + q[u].x * MP : lds + (q[u].x >> 6) * ROW + (q[u].x & 63) * MP;
+                // 24-bit multiplies (full rate; v_mul_lo_u32 is quarter rate):
+                // slots and tile rows are far below 2^24
+                const float* base = ADJ ? lds + __umul24(q[u].x, (unsigned)MP)
+                                        : lds + __umul24(q[u].x >> 6, (unsigned)ROW) + (q[u].x & 63) * MP;
                 const float2* p = reinterpret_cast<const float2*>(base);
 #pragma unroll
                 for (int h = 0; h < MP / 2; h++) f[u][h] = p[h];
