@@ -403,31 +403,36 @@ __device__ __forceinline__ void bp_multi_tile(const BackprojArgs& a, int S, int 
         // software pipeline: the geometry and gathers of view v + 1 are
         // issued before the terms of view v are summed (the last view loads
         // itself again), so the gathers' latency overlaps the arithmetic
-        BpView g = bp_view<WIDE, FW>(a, xx, yy, 0);
-        float r0[K], r1[K];
-        taps(g.off, r0, r1);
-#pragma unroll 1
-        for (int v = 0; v < a.use_views; v++) {
-            const BpView gn = bp_view<WIDE, FW>(a, xx, yy, min(v + 1, a.use_views - 1));
-            float n0[K], n1[K];
-            taps(gn.off, n0, n1);
+        auto add = [&](const BpView& g, const float (&t0)[K], const float (&t1)[K]) {
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 float term;
                 if constexpr (FW) {
-                    term = fmaf(g.w1s, r1[k], g.w0s * r0[k]);
+                    term = fmaf(g.w1s, t1[k], g.w0s * t0[k]);
                 } else {
-                    const double sum = g.w0 * r0[k] + g.w * r1[k];  // + 0.0 left out as above
+                    const double sum = g.w0 * t0[k] + g.w * t1[k];  // + 0.0 left out as above
                     term = (float)(g.rL2 * sum * a.dbeta_view * a.xfactor);
                 }
                 acc[k] = g.ok ? acc[k] + term : acc[k];
             }
-            g = gn;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                r0[k] = n0[k];
-                r1[k] = n1[k];
-            }
+        };
+        // two views per iteration in two register sets: view v + 1 is
+        // fetched while view v is summed, then view v + 2 while view v + 1
+        // is (no copies between the sets); an odd count's last fetch
+        // re-reads the last view and is not summed
+        const int nv = a.use_views;
+        BpView g = bp_view<WIDE, FW>(a, xx, yy, 0), gn;
+        float r0[K], r1[K], n0[K], n1[K];
+        taps(g.off, r0, r1);
+#pragma unroll 1
+        for (int v = 0; v < nv; v += 2) {
+            gn = bp_view<WIDE, FW>(a, xx, yy, min(v + 1, nv - 1));
+            taps(gn.off, n0, n1);
+            add(g, r0, r1);
+            if (v + 1 == nv) break;
+            g = bp_view<WIDE, FW>(a, xx, yy, min(v + 2, nv - 1));
+            taps(g.off, r0, r1);
+            add(gn, n0, n1);
         }
     }
     if (in_img) {
@@ -520,36 +525,36 @@ __global__ __launch_bounds__(256) void k_backproject_sym(BackprojArgs a)
         // the pixel-view geometry in the float-weight form: the sums already
         // run in another order than the reference's, and its roundings (~1e-7
         // relative per term) stay far below that difference
-        BpView g = bp_view<WIDE, true>(a, xx, yy, vb);
-        float r0[4], r1[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const float* row = row_of(vb, k) + g.col;
-            r0[k] = row[0];
-            r1[k] = row[1];
-        }
-#pragma unroll 1
-        for (int v = vb; v < ve; v++) {
-            const int vn = min(v + 1, ve - 1);
-            const BpView gn = bp_view<WIDE, true>(a, xx, yy, vn);
-            float n0[4], n1[4];
+        auto fetch = [&](int v, const BpView& g, float (&t0)[4], float (&t1)[4]) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const float* row = row_of(vn, k) + gn.col;
-                n0[k] = row[0];
-                n1[k] = row[1];
+                const float* row = row_of(v, k) + g.col;
+                t0[k] = row[0];
+                t1[k] = row[1];
             }
+        };
+        auto add = [&](const BpView& g, const float (&t0)[4], const float (&t1)[4]) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const float term = fmaf(g.w1s, r1[k], g.w0s * r0[k]);
+                const float term = fmaf(g.w1s, t1[k], g.w0s * t0[k]);
                 acc[k] = g.ok ? acc[k] + term : acc[k];
             }
-            g = gn;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                r0[k] = n0[k];
-                r1[k] = n1[k];
-            }
+        };
+        // two views per iteration in two register sets (as in bp_multi_tile)
+        BpView g = bp_view<WIDE, true>(a, xx, yy, vb), gn;
+        float r0[4], r1[4], n0[4], n1[4];
+        fetch(vb, g, r0, r1);
+#pragma unroll 1
+        for (int v = vb; v < ve; v += 2) {
+            const int v1 = min(v + 1, ve - 1);
+            gn = bp_view<WIDE, true>(a, xx, yy, v1);
+            fetch(v1, gn, n0, n1);
+            add(g, r0, r1);
+            if (v + 1 == ve) break;
+            const int v2 = min(v + 2, ve - 1);
+            g = bp_view<WIDE, true>(a, xx, yy, v2);
+            fetch(v2, g, r0, r1);
+            add(gn, n0, n1);
         }
     }
     if (wave > 0) {
