@@ -456,8 +456,8 @@ def main():
              "per_launch": "P_mask x V x 8 B + N^2 x 4 B = %.3f GB" % (bp_bytes / 1e9),
              "note": "the taps come from the 1.95 MB filtered sinogram held in L2 (16.6 MB of traffic "
                      "past the L2s per launch), so the algorithmic rate can exceed the HBM peak; the "
-                     "kernel is bound by VALU issue (76% busy, 19 VALU instructions per masked "
-                     "pixel-view) and the texture data path (81% busy)"},
+                     "kernel is bound by VALU issue (76% busy, 17 VALU instructions per masked "
+                     "pixel-view) and the texture data path (75% busy)"},
         ],
         "cpu_baseline": None,
     }
