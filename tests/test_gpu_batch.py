@@ -546,18 +546,23 @@ def test_sysmat_rows_equal_reference_segment_lists(gpu, golden):
     sm.close()
 
 
-@pytest.mark.parametrize("exact", [False, True])
-def test_interleaved_grouped_backprojection_equals_plain(gpu, exact):
+@pytest.mark.parametrize("exact,wide", [(False, False), (True, False), (False, True)])
+def test_interleaved_grouped_backprojection_equals_plain(gpu, exact, wide):
     """Sinograms interleaved in slices of k (4, 8, 12, 16; the last slice
     zero-padded) give the grouped backprojection's images bit for bit, with
     float-weight and exact terms, at 256^2 and 768^2 over 50/N cm plus an odd
-    size; the interleaved array is the permuted stack."""
+    size, and with a 128^2 x 0.6 cm group whose fan needs the wide-angle
+    arctangent (which the whole launch then takes); the interleaved array is
+    the permuted stack."""
     import torch
     ops, batch, _ = _mods()
     from virtualct_amd import _lib, default_geometry
     gen = torch.Generator(device=gpu).manual_seed(13)
     sizes, Ss = (256, 768, 97), (5, 9, 3)
-    plans = [ops.plan_for(default_geometry(recon_size=N, pixel_cm=50.0 / N, n_views=120)) for N in sizes]
+    pixs = [50.0 / N for N in sizes]
+    if wide:
+        sizes, Ss, pixs = sizes + (128,), Ss + (4,), pixs + [0.6]
+    plans = [ops.plan_for(default_geometry(recon_size=N, pixel_cm=pix, n_views=120)) for N, pix in zip(sizes, pixs)]
     if exact:
         for p in plans:
             p.set_option(_lib.OPT_EXACT_BACKPROJECT, 1)

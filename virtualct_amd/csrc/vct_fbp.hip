@@ -58,26 +58,40 @@ __host__ __device__ __forceinline__ size_t pack_index(int tile, int k, int rr, i
 // all but ~0.06% of sinogram values (the device logf differs on about a third).
 __device__ __forceinline__ float logf_rounded(float x) { return (float)log((double)x); }
 
+// One workgroup per 32 rows x 32 channels: the rows are read along the
+// channels (coalesced), pre-weighted into LDS, and written out as the 4 x 64
+// float4 of their packed operand block (contiguous), so the writes are whole
+// lines instead of 4-byte scatters.
 __global__ __launch_bounds__(256) void k_filter_prep(FilterPrepArgs a)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;
-    if (k >= a.Kp) return;
-    float v = 0.f;
-    if (r < a.R && k < a.W) {
-        const double p = a.proj[(size_t)r * a.W + k];
-        double y;
-        if (a.bhc) {
-            // beam-hardening correction, M/beam_hardening_correction.cpp:51-55 (whose
-            // log of a float is the single-precision overload)
-            y = -(double)logf_rounded((float)p);
-            y = fabs(y) + (a.bhc_alpha * pow(fabs(y), a.bhc_beta));
-        } else {
-            y = -log(p);
+    __shared__ float t[32][33];
+    const int kc = blockIdx.x, tr = blockIdx.y;
+    for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+        const int rr = e >> 5, kk = e & 31;
+        const int r = tr * 32 + rr, k = kc * 32 + kk;
+        float v = 0.f;
+        if (r < a.R && k < a.W) {
+            const double p = a.proj[(size_t)r * a.W + k];
+            double y;
+            if (a.bhc) {
+                // beam-hardening correction, M/beam_hardening_correction.cpp:51-55 (whose
+                // log of a float is the single-precision overload)
+                y = -(double)logf_rounded((float)p);
+                y = fabs(y) + (a.bhc_alpha * pow(fabs(y), a.bhc_beta));
+            } else {
+                y = -log(p);
+            }
+            v = (float)(y * a.coeff[k]);
         }
-        v = (float)(y * a.coeff[k]);
+        t[rr][kk] = v;
     }
-    a.apk[pack_index(r >> 5, k, r & 31, a.Kp)] = v;
+    __syncthreads();
+    // float4 (q, l) of the tile holds k = 8q + 2j + (l >> 5), j = 0..3, of
+    // row l & 31 (pack_index)
+    const int i = threadIdx.x, qq = i >> 6, l = i & 63, rr = l & 31, hi = l >> 5;
+    const float* row = t[rr] + 8 * qq + hi;
+    float4* out = reinterpret_cast<float4*>(a.apk) + ((size_t)tr * (a.Kp / 8) + kc * 4 + qq) * 64 + l;
+    *out = make_float4(row[0], row[2], row[4], row[6]);
 }
 
 constexpr int kFiltWaves = 4;  // K quarters of one 32x32 output tile, one wave each
@@ -601,7 +615,7 @@ vct_status launch_filter(const FilterPrepArgs& p, const FilterGemmArgs& g, hipSt
         return set_error(VCT_ERR_INTERNAL, "filter K padding %d not a multiple of %d", p.Kp, 8 * kFiltWaves);
     if ((size_t)(p.Kp + 31) * sizeof(float) > 48 * 1024) return set_error(VCT_ERR_ARG, "too many channels for the filter");
     const int Rp = (p.R + 31) / 32 * 32;
-    hipLaunchKernelGGL(k_filter_prep, dim3((p.Kp + 255) / 256, Rp), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_filter_prep, dim3(p.Kp / 32, Rp / 32), dim3(256), 0, s, p);
     VCT_HIP(hipGetLastError());
     const int Wp = (g.W + 31) / 32 * 32;
     const int ct = Wp / 32;
