@@ -117,7 +117,7 @@ int main(int argc, char* argv[])
     if (in.sino_fact != 0) H = H / in.sino_fact;
     const int W = in.width;
     if (in.prior == 1 || in.prior == 2) {
-        std::fprintf(stderr, "ERROR: prior images (DICOM / MVCT) are not supported by this build\n");
+        std::fprintf(stderr, "ERROR: prior images (DICOM / MVCT) are not supported by this program\n");
         return 1;
     }
     std::fprintf(stderr, "%s\n", in.inpr == "PR" ? "Pre-treatment CT " : "In-treatment CT ");

@@ -300,7 +300,7 @@ vct_status vct_init(int device, vct_ctx** out)
     hipDeviceProp_t prop;
     VCT_HIP(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-        return set_error(VCT_ERR_UNSUPPORTED, "device %d is %s; this build targets gfx950", device, prop.gcnArchName);
+        return set_error(VCT_ERR_UNSUPPORTED, "device %d is %s; the library is compiled for gfx950 only", device, prop.gcnArchName);
     void* noalloc = nullptr;
     VCT_CHECK(noalloc_scratch_for(device, &noalloc));  // with the device current
     vct_ctx* c = new vct_ctx();
