@@ -1216,9 +1216,21 @@ struct TileArgs {
 // Each lane then sums one run of a slice in FP64, 8 entries in flight, and
 // writes its partial sums as stage-1 row run, phantom b.  The sums are those
 // of k_gather over the same runs, term for term in the same order.
+// Several groups (one per geometry) in one launch: workgroup l belongs to the
+// group gi with pref[gi] <= l < pref[gi + 1] (a scalar search).  Each group's
+// range is a multiple of 8 workgroups, so l - pref[gi] keeps l's XCD.
+struct TileGroups {
+    TileArgs a[kMaxGroups];
+    int pref[kMaxGroups + 1];
+    int n;
+};
+
 template <int M, bool ADJ>
-__global__ __launch_bounds__(ADJ ? kAtileThreads : kTileThreads) void k_gather_tile(TileArgs a)
+__global__ __launch_bounds__(ADJ ? kAtileThreads : kTileThreads) void k_gather_tile(TileGroups ga)
 {
+    int gi = 0;
+    while (gi + 1 < ga.n && (int)blockIdx.x >= ga.pref[gi + 1]) gi++;
+    const TileArgs& a = ga.a[gi];
     constexpr int MP = (M + 1) & ~1;  // floats per voxel or ray in LDS
     constexpr int ROW = 64 * MP + 2;  // floats per tile row (forward)
     constexpr int U = kTileUnroll;
@@ -1226,7 +1238,7 @@ __global__ __launch_bounds__(ADJ ? kAtileThreads : kTileThreads) void k_gather_t
     constexpr uint32_t PAD = ADJ ? (uint32_t)kAdjPadSlot : kTilePadVoxel;
     __shared__ float lds[LDSF];
     __shared__ int32_t tab[2 * kAdjViews + 2];  // adjoint: slot_off, slot_clo, first view of the block
-    const int g = blockIdx.x, xcd = g % kXcds, slot = g / kXcds;
+    const int g = (int)blockIdx.x - ga.pref[gi], xcd = g % kXcds, slot = g / kXcds;
     const int k = slot / a.B, b = slot % a.B;
     if (k >= a.kmax) return;  // the whole workgroup
     const int t = a.xcd_tiles[xcd * a.kmax + k];
@@ -2059,17 +2071,31 @@ static const vct_tiled* lds_form(const vct_sysmat* m, bool adjoint)
     return &t;
 }
 
-// Stage 1 of one group over the LDS-resident form: B workgroups per block,
-// kmax blocks per XCD.
-static vct_status run_gather_tile(const vct_batch_group& G, const vct_tiled& t, bool adjoint, int M, hipStream_t s)
+// Stage 1 of the groups over their LDS-resident forms in one launch: per
+// group B workgroups per block, kmax blocks per XCD.
+static vct_status run_gather_tile(const vct_batch_group* groups, const int* idx, const vct_tiled* const* forms, int n,
+                                  bool adjoint, int M, hipStream_t s)
 {
-    TileArgs ta{G.d_in, G.d_work, t.ell, t.slice_ptr, t.slice_len, t.slice_runs, t.tile_slices, t.xcd_tiles,
-                t.kmax, t.ntx, G.sysmat->plan->g.recon_size, G.n_phantoms, t.slot_off, t.slot_clo, t.blk_v0,
-                G.sysmat->plan->g.n_channels};
-    const unsigned grid = (unsigned)(kXcds * t.kmax * G.n_phantoms);
+    static_assert(sizeof(TileGroups) <= 4096, "kernel arguments");
+    TileGroups tg{};
+    tg.n = n;
+    long long total = 0;
+    for (int q = 0; q < n; q++) {
+        const vct_batch_group& G = groups[idx[q]];
+        const vct_tiled& t = *forms[q];
+        tg.a[q] = TileArgs{G.d_in, G.d_work, t.ell, t.slice_ptr, t.slice_len, t.slice_runs, t.tile_slices,
+                           t.xcd_tiles, t.kmax, t.ntx, G.sysmat->plan->g.recon_size, G.n_phantoms, t.slot_off,
+                           t.slot_clo, t.blk_v0, G.sysmat->plan->g.n_channels};
+        tg.pref[q] = (int)total;
+        total += (long long)kXcds * t.kmax * G.n_phantoms;  // a multiple of kXcds
+        if (total > INT_MAX) return set_error(VCT_ERR_ARG, "stage-1 launch too large");
+    }
+    tg.pref[n] = (int)total;
+    if (total == 0) return VCT_OK;
+    const unsigned grid = (unsigned)total;
 #define VCT_TILE_M(MM)                                                                                      \
-    if (M == MM && !adjoint) hipLaunchKernelGGL((k_gather_tile<MM, false>), dim3(grid), dim3(kTileThreads), 0, s, ta); \
-    if (M == MM && adjoint) hipLaunchKernelGGL((k_gather_tile<MM, true>), dim3(grid), dim3(kAtileThreads), 0, s, ta);
+    if (M == MM && !adjoint) hipLaunchKernelGGL((k_gather_tile<MM, false>), dim3(grid), dim3(kTileThreads), 0, s, tg); \
+    if (M == MM && adjoint) hipLaunchKernelGGL((k_gather_tile<MM, true>), dim3(grid), dim3(kAtileThreads), 0, s, tg);
     VCT_TILE_M(1) VCT_TILE_M(2) VCT_TILE_M(3) VCT_TILE_M(4)
     VCT_TILE_M(5) VCT_TILE_M(6) VCT_TILE_M(7) VCT_TILE_M(8)
 #undef VCT_TILE_M
@@ -2117,7 +2143,7 @@ static vct_status launch_gather(const vct_batch_group* groups, int n_groups, int
         }
     }
     hipStream_t s = (hipStream_t)stream;
-    for (int q = 0; q < n_tiles; q++) VCT_CHECK(run_gather_tile(groups[tiles[q]], *tforms[q], adjoint, M, s));
+    if (n_tiles) VCT_CHECK(run_gather_tile(groups, tiles, tforms, n_tiles, adjoint, M, s));
     VCT_CHECK(run_gather(a, items_a, M, lanes, blocks, s));
     return run_gather(b, items_b, M, lanes, blocks, s);
 }
