@@ -382,9 +382,11 @@ int64_t vct_sysmat_segments(const vct_sysmat* m);
 vct_status vct_sysmat_row(const vct_sysmat* m, int ray, uint32_t* h_voxel, float* h_length, int capacity,
                           int* n_out, void* stream);
 size_t vct_sysmat_bytes(const vct_sysmat* m);
-/* Bytes of the partial-sum workspace a batched gather over this system
- * matrix needs for n_phantoms phantoms in the blocked form (0 if it has
- * none); one workspace serves both directions. */
+/* Bytes of the workspace a batched gather over this system matrix needs for
+ * n_phantoms phantoms in the blocked form: the stage-1 partial sums and,
+ * with the adjoint's LDS-resident form, a phantom-major copy of the adjoint
+ * weights (n_rays * n_phantoms * 32 bytes).  One workspace serves both
+ * directions. */
 size_t vct_sysmat_workspace(const vct_sysmat* m, int n_phantoms);
 
 /* One group of phantoms sharing a system matrix.  Batched arrays are

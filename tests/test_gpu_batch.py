@@ -480,7 +480,8 @@ def test_tile_stage1_equals_blocked_stage1(gpu, N, pix, B, M):
     batch.batch_paths([(smt, x, Lt, B)], M)
     batch.batch_paths([(smn, x, Ln, B)], M)
     assert torch.equal(Lt, Ln)
-    assert torch.equal(smt.workspace(B, 0), smn.workspace(B, 0))
+    n = smn.workspace(B, 0).numel()  # the partial sums (the LDS forms' workspace adds a weight copy)
+    assert torch.equal(smt.workspace(B, 0)[:n], smn.workspace(B, 0))
     smt.close()
     smn.close()
 
@@ -517,7 +518,8 @@ def test_adjoint_lds_stage1_equals_blocked_stage1(gpu, N, pix, B, M, built):
     batch.batch_adjoint([(smt, w, gt, B)], M)
     batch.batch_adjoint([(smn, w, gn, B)], M)
     assert torch.equal(gt, gn)
-    assert torch.equal(smt.workspace(B, 0), smn.workspace(B, 0))
+    n = smn.workspace(B, 0).numel()  # the partial sums (then the phantom-major weight copy)
+    assert torch.equal(smt.workspace(B, 0)[:n], smn.workspace(B, 0))
     smt.close()
     smn.close()
 

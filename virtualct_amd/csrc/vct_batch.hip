@@ -1203,7 +1203,29 @@ struct TileArgs {
     const int32_t* slot_clo;
     const int32_t* blk_v0;
     int W;
+    const float* in_pm;          // adjoint: the weights phantom-major, [B][rays][8] (k_weights_pm)
+    int rays;
 };
+
+// Adjoint: the weights of the batched layout copied phantom-major,
+// out[b][ray][0..7] = the values load_slot<M> gives for (ray, b), so the
+// table fill of one phantom reads consecutive rays from consecutive 32 bytes
+// (in the batched rows, one phantom's values of consecutive rays are 8 * B
+// floats apart: a cache line per ray).  Lanes of a ray read its row
+// together; lpr lanes per ray, 256 / lpr rays per workgroup.
+template <int M>
+__global__ __launch_bounds__(256) void k_weights_pm(const float* __restrict__ in, int rays, int B, int lpr,
+                                                   float* __restrict__ out)
+{
+    const int b = threadIdx.x % lpr;
+    const int ray = blockIdx.x * (blockDim.x / lpr) + threadIdx.x / lpr;
+    if (b >= B || ray >= rays) return;
+    float v[8];
+    load_slot<M>(in, ray, B, b, v);
+    float4* d = reinterpret_cast<float4*>(out + ((size_t)b * rays + ray) * 8);
+    d[0] = make_float4(v[0], v[1], v[2], v[3]);
+    d[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
 
 // Stage 1 with the block's data in LDS.  Workgroup g runs on XCD g % 8
 // (round-robin dispatch) and takes phantom b of block xcd_tiles[xcd][k], so
@@ -1268,7 +1290,11 @@ __global__ __launch_bounds__(ADJ ? kAtileThreads : kTileThreads) void k_gather_t
                         else hi = mid;
                     }
                     const size_t ray = (size_t)(v0 + lo) * a.W + tab[kAdjViews + 1 + lo] + (s - tab[lo]);
-                    load_slot<M>(a.in, ray, a.B, b, v[u]);
+                    const float4* src = reinterpret_cast<const float4*>(a.in_pm + ((size_t)b * a.rays + ray) * 8);
+                    const float4 lo4 = src[0];
+                    v[u][0] = lo4.x; v[u][1] = lo4.y; v[u][2] = lo4.z; v[u][3] = lo4.w;
+                    const float4 hi4 = M > 4 ? src[1] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    v[u][4] = hi4.x; v[u][5] = hi4.y; v[u][6] = hi4.z; v[u][7] = hi4.w;
                 }
             }
 #pragma unroll
@@ -2017,11 +2043,20 @@ size_t vct_sysmat_bytes(const vct_sysmat* m)
     return b;
 }
 
+// Workspace layout: the stage-1 partial sums (rows of 8 * B floats), then,
+// with the adjoint's LDS-resident form, the phantom-major copy of the
+// adjoint weights (k_weights_pm) at a 256-byte boundary.
+static size_t pm_offset(const vct_sysmat* m, int B)
+{
+    const size_t rows1 = m->fwd.rows1 > m->adj.rows1 ? m->fwd.rows1 : m->adj.rows1;
+    return (rows1 * (size_t)B * 8 * sizeof(float) + 255) / 256 * 256;
+}
+
 size_t vct_sysmat_workspace(const vct_sysmat* m, int n_phantoms)
 {
     if (!m || n_phantoms < 1) return 0;
-    const size_t rows1 = m->fwd.rows1 > m->adj.rows1 ? m->fwd.rows1 : m->adj.rows1;
-    return rows1 * (size_t)n_phantoms * 8 * sizeof(float);
+    const size_t pm = m->atile.ell ? (size_t)m->nrays * n_phantoms * 8 * sizeof(float) : 0;
+    return pm_offset(m, n_phantoms) + pm;
 }
 
 // lanes_per_row 1: one phantom per group, each row split over kSplit lanes
@@ -2083,9 +2118,23 @@ static vct_status run_gather_tile(const vct_batch_group* groups, const int* idx,
     for (int q = 0; q < n; q++) {
         const vct_batch_group& G = groups[idx[q]];
         const vct_tiled& t = *forms[q];
+        float* pm = nullptr;
+        if (adjoint) {
+            // the weights phantom-major, after the partial sums of the workspace
+            pm = reinterpret_cast<float*>(reinterpret_cast<char*>(G.d_work) + pm_offset(G.sysmat, G.n_phantoms));
+            int lpr = 1;
+            while (lpr < G.n_phantoms) lpr <<= 1;
+            const int rays = G.sysmat->nrays, per = 256 / lpr;
+            const dim3 grid((rays + per - 1) / per);
+#define VCT_PM_M(MM) \
+    if (M == MM) hipLaunchKernelGGL(k_weights_pm<MM>, grid, dim3(256), 0, s, G.d_in, rays, G.n_phantoms, lpr, pm);
+            VCT_PM_M(1) VCT_PM_M(2) VCT_PM_M(3) VCT_PM_M(4)
+            VCT_PM_M(5) VCT_PM_M(6) VCT_PM_M(7) VCT_PM_M(8)
+#undef VCT_PM_M
+        }
         tg.a[q] = TileArgs{G.d_in, G.d_work, t.ell, t.slice_ptr, t.slice_len, t.slice_runs, t.tile_slices,
                            t.xcd_tiles, t.kmax, t.ntx, G.sysmat->plan->g.recon_size, G.n_phantoms, t.slot_off,
-                           t.slot_clo, t.blk_v0, G.sysmat->plan->g.n_channels};
+                           t.slot_clo, t.blk_v0, G.sysmat->plan->g.n_channels, pm, G.sysmat->nrays};
         tg.pref[q] = (int)total;
         total += (long long)kXcds * t.kmax * G.n_phantoms;  // a multiple of kXcds
         if (total > INT_MAX) return set_error(VCT_ERR_ARG, "stage-1 launch too large");
