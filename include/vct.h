@@ -409,9 +409,12 @@ typedef struct vct_batch_group {
 
 /* Batched replay of the cached segments, replacing reprojection_TOMO_2
  * (M/reprojection_TOMO.cpp:291-352) for many phantoms and geometries in one
- * persistent launch (two with blocked groups; up to 16 groups; lanes_per_row
- * in {8,16,32,64}, or 1 when every group holds one phantom: each row is then
- * split over 16 lanes; blocks <= 0 picks the persistent grid).
+ * persistent launch (up to 16 groups; lanes_per_row in {8,16,32,64}, or 1
+ * when every group holds one phantom: each row is then split over 16 lanes;
+ * blocks <= 0 picks the persistent grid).  Groups with a workspace gather in
+ * two stages: stage 1 of the groups with an LDS-resident form in one launch
+ * (in the adjoint after copying each group's weights phantom-major into its
+ * workspace), the others' stage 1 in the persistent launch, then stage 2. */
  * out[ray][b][m] = sum_seg in[vox][b][m]*len / xfactor (the tracer's units). */
 vct_status vct_batch_paths(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row,
                            int blocks, void* stream);
