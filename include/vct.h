@@ -414,7 +414,7 @@ typedef struct vct_batch_group {
  * blocks <= 0 picks the persistent grid).  Groups with a workspace gather in
  * two stages: stage 1 of the groups with an LDS-resident form in one launch
  * (in the adjoint after copying each group's weights phantom-major into its
- * workspace), the others' stage 1 in the persistent launch, then stage 2. */
+ * workspace), the others' stage 1 in the persistent launch, then stage 2.
  * out[ray][b][m] = sum_seg in[vox][b][m]*len / xfactor (the tracer's units). */
 vct_status vct_batch_paths(const vct_batch_group* groups, int n_groups, int M, int lanes_per_row,
                            int blocks, void* stream);
